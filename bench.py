@@ -1,0 +1,157 @@
+"""Benchmark: decoded temporal edges/s of the plain neighbor-access pass.
+
+One step = one full neighbor-access (RA) pass of the GPU engine over every
+contact of the workload, with the compressed graph resident in HBM.  The
+workload is the plain at-scale configuration: Chung-Lu, exponent 2.1, 2^21
+vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks (``chunglu21``),
+generated and encoded by the format kit before the timed region.
+
+Prints one JSON line with the driver's contract fields plus:
+  roofline      the decode kernel against MI355X HBM bandwidth (8 TB/s):
+                algorithmic bytes per launch (|.graph| + |.timestamps| + the
+                Elias-Fano index bits (n+1)(l_g+2 + l_t+2)/8) divided by the
+                average kernel duration measured with HIP events on the
+                engine's stream; `traffic` (rocprof FETCH_SIZE + WRITE_SIZE per
+                launch) is filled from --traffic-bytes when given, else null.
+  cpu_baseline  the CPU oracle's neighbor access (single thread, the build's
+                C restatement of the reference reader, kind "port") timed on
+                this host over a bounded vertex sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ds_dir: str, name: str, n: int, budget_s: float) -> dict:
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    orc = O.Oracle(ds_dir, name, n)
+    # Bounded sample: consecutive vertex blocks spread over the id range,
+    # grown until about budget_s of single-thread decode work.
+    rng = np.random.default_rng(7)
+    verts = rng.permutation(n).astype(np.int32)
+    done, contacts, elapsed = 0, 0, 0.0
+    step = 4096
+    while elapsed < budget_s and done < n:
+        chunk = verts[done:done + step]
+        t0 = time.perf_counter()
+        _, cnt = orc.neighbor_access(chunk)
+        elapsed += time.perf_counter() - t0
+        contacts += cnt
+        done += len(chunk)
+        step = min(step * 2, 1 << 18)
+    orc.close()
+    return {"value": contacts / elapsed, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"{done} random vertices ({contacts} contacts) of the same graph, "
+                      f"{elapsed:.1f} s single-thread oracle decode"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="chunglu21")
+    ap.add_argument("--data-root", default=None)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="HBM bytes per decode launch from rocprof FETCH_SIZE + WRITE_SIZE")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    from compresstemporalgraph_amd import datasets
+    from compresstemporalgraph_amd.engine import TemporalGraph
+
+    root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
+    t0 = time.perf_counter()
+    desc = datasets.ensure(root, args.workload)
+    log(f"[bench] dataset {args.workload}: n={desc['max_vertex']} contacts={desc['contacts']} "
+        f"ready in {time.perf_counter() - t0:.1f} s")
+    ds_dir = datasets.dataset_dir(root, args.workload)
+
+    t0 = time.perf_counter()
+    g = TemporalGraph(ds_dir, args.workload, device=0)
+    log(f"[bench] opened on GPU in {time.perf_counter() - t0:.2f} s: {g.info}")
+    info = g.info
+    n = info["n"]
+    contacts = info["contacts"]
+
+    for _ in range(args.warmup):
+        g.neighbor_access()
+    g.profile(True)
+    g.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cs, cnt = g.neighbor_access()
+    elapsed = time.perf_counter() - t0
+    kernel_ms, launches = g.profile_read(reset=True)
+    g.profile(False)
+    assert cnt == contacts
+    log(f"[bench] {args.steps} steps in {elapsed:.3f} s; kernel {kernel_ms / max(launches, 1):.3f} ms avg "
+        f"over {launches} launches; checksum {cs:#018x}")
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = contacts * args.steps / elapsed
+    avg_kernel_s = kernel_ms / max(launches, 1) / 1e3
+    index_bits = (n + 1) * (info["l_graph"] + 2 + info["l_timestamps"] + 2)
+    alg_bytes = info["graph_bytes"] + info["timestamps_bytes"] + index_bits / 8.0
+    achieved = alg_bytes / avg_kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": args.traffic_bytes,
+                "kernel": "k_ra", "kernel_ms": round(avg_kernel_s * 1e3, 4),
+                "algorithmic_bytes": int(alg_bytes)}
+    g.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(ds_dir, args.workload, n, args.cpu_budget)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} edges/s ({cpu['sample']})")
+
+    line = {
+        "metric": "decoded temporal edges/s",
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: neighbor-access (RA) full decode pass; Chung-Lu exponent 2.1, "
+                               f"2^21 vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks",
+                   "vertices": n, "contacts": contacts,
+                   "graph_bytes": info["graph_bytes"], "timestamps_bytes": info["timestamps_bytes"],
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

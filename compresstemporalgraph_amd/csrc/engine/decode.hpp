@@ -1,0 +1,417 @@
+// Device decode core for gfx950: bit peeks, instantaneous codes, Elias-Fano
+// offsets and the successor-list state machine.
+//
+// HBM layout (built once by the loader, files on disk stay unchanged):
+//  * .graph and .timestamps are stored as 64-bit words with their bytes
+//    swapped, so one aligned load yields 64 stream bits MSB-first (the
+//    stream's own bit order, dsiutils / bfs.cu:166-175).  A 64-bit window at
+//    any bit position is a funnel shift of two adjacent words.
+//  * Elias-Fano lower bits are stored the same way (they are MSB-first,
+//    bfs.cu:543-552); upper bits keep the file's little-endian words
+//    (LSB-first, bfs.cu:305-307, :446-448).
+//  * The select directory samples the position of every 64th one of the
+//    upper bits; a 64-one block spanning more than kSpanBits bits stores all
+//    64 positions instead (bit 63 of the sample flags it).  It plays the role
+//    of SimpleSelect (bfs.cu:281-541) but is built on the host at load time.
+//
+// Codes are decoded arithmetically from a 64-bit window with a count of
+// leading zeros instead of the reference's 16-bit lookup tables
+// (bfs.cu:202-256, :590-639); every codeword this format produces is at most
+// 51 bits long (zeta_2 of an int2nat'ed int32 difference).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tga {
+
+constexpr int32_t kInf = 0x7fffffff;
+constexpr int kMaxDepth = 3;          // maxrefcount (BVMultiGraph.java:163)
+constexpr int kWindow = 7;            // windowsize (BVMultiGraph.java:169)
+constexpr uint32_t kSpanBits = 1024;  // select blocks wider than this are stored exactly
+
+// Error bits reported through the per-handle device error word.
+enum : int32_t {
+  kErrOverrun = 1,      // a cursor ran past the end of a stream
+  kErrNeighbor = 2,     // decoded neighbor outside [0, n)
+  kErrDepth = 4,        // reference chain deeper than kMaxDepth or reference outside the window
+  kErrExhausted = 8,    // record yields fewer successors than its outdegree
+  kErrCode = 16,        // codeword longer than the format allows
+  kErrBlocks = 32,      // copy blocks inconsistent with the referenced list
+};
+
+struct EFDir {
+  const uint64_t* lower;  // byte-swapped words, MSB-first fields of width l
+  const uint64_t* upper;  // little-endian words
+  const uint64_t* samp;   // position of one #64k, or (1<<63) | spill base
+  const uint64_t* spill;
+  uint64_t upper_words;
+  int32_t l;
+};
+
+struct Streams {
+  const uint64_t* g;  // graph stream, byte-swapped words
+  const uint64_t* t;  // timestamp stream, byte-swapped words
+  uint64_t g_bits, t_bits;
+  EFDir gi, ti;
+  int32_t n;       // vertices; ids are [0, n)
+  int32_t min_ts;  // .timestamps header value
+  int32_t* err;    // sticky error word
+};
+
+__device__ __forceinline__ void raise_err(const Streams& S, int32_t bit) { atomicOr(S.err, bit); }
+
+// 64 stream bits starting at bit `pos` (MSB-first).  `pos` must be < bits;
+// the word arrays carry two words of zero padding.
+__device__ __forceinline__ uint64_t peek64(const uint64_t* w, uint64_t pos) {
+  const uint64_t i = pos >> 6;
+  const uint32_t s = (uint32_t)pos & 63u;
+  const uint64_t a = w[i];
+  const uint64_t b = w[i + 1];
+  return s ? (a << s) | (b >> (64u - s)) : a;
+}
+
+__device__ __forceinline__ int64_t nat2int(uint64_t v) { return (v & 1) ? -(int64_t)(v >> 1) - 1 : (int64_t)(v >> 1); }
+
+// Cursor over one stream with a bound check: a malformed stream can never
+// make a load leave the padded buffer.
+struct Rd {
+  const uint64_t* w;
+  uint64_t bits;
+};
+
+__device__ __forceinline__ uint64_t safe_peek(const Streams& S, Rd r, uint64_t pos) {
+  if (pos >= r.bits) {
+    raise_err(S, kErrOverrun);
+    return ~0ull;  // all ones: every code decodes to 0 with length 1
+  }
+  return peek64(r.w, pos);
+}
+
+__device__ __forceinline__ uint32_t rd_unary(const Streams& S, Rd r, uint64_t& pos) {
+  const uint64_t p = safe_peek(S, r, pos);
+  const uint32_t z = (uint32_t)__clzll(p);
+  if (z > 62) {
+    raise_err(S, kErrCode);
+    pos += 1;
+    return 0;
+  }
+  pos += z + 1;
+  return z;
+}
+
+__device__ __forceinline__ uint64_t rd_gamma(const Streams& S, Rd r, uint64_t& pos) {
+  const uint64_t p = safe_peek(S, r, pos);
+  const uint32_t z = (uint32_t)__clzll(p);
+  if (z > 31) {
+    raise_err(S, kErrCode);
+    pos += 1;
+    return 0;
+  }
+  pos += 2 * z + 1;
+  return ((p << z) >> (63 - z)) - 1;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& pos) {
+  const uint64_t p = safe_peek(S, r, pos);
+  const uint32_t h = (uint32_t)__clzll(p);
+  const uint32_t nb = h * K + K - 1;  // bits of the short form
+  if (h + nb + 2 > 64) {
+    raise_err(S, kErrCode);
+    pos += 1;
+    return 0;
+  }
+  const uint64_t left = 1ull << (h * K);
+  const uint64_t m = (p << (h + 1)) >> (64 - nb);
+  if (m < left) {
+    pos += h + 1 + nb;
+    return m + left - 1;
+  }
+  const uint64_t bit = (p << (h + 1 + nb)) >> 63;
+  pos += h + 2 + nb;
+  return (m << 1) + bit - 1;
+}
+
+// Position of the r-th (0-based) set bit of w (r < popcount(w)).
+__device__ __forceinline__ uint32_t select_in_word(uint64_t w, uint32_t r) {
+  uint32_t pos = 0;
+  uint32_t c = (uint32_t)__popcll(w & 0xffffffffull);
+  if (r >= c) { r -= c; w >>= 32; pos += 32; }
+  c = (uint32_t)__popcll(w & 0xffffull);
+  if (r >= c) { r -= c; w >>= 16; pos += 16; }
+  c = (uint32_t)__popcll(w & 0xffull);
+  if (r >= c) { r -= c; w >>= 8; pos += 8; }
+  c = (uint32_t)__popcll(w & 0xfull);
+  if (r >= c) { r -= c; w >>= 4; pos += 4; }
+  c = (uint32_t)__popcll(w & 0x3ull);
+  if (r >= c) { r -= c; w >>= 2; pos += 2; }
+  c = (uint32_t)(w & 1ull);
+  if (r >= c) { pos += 1; }
+  return pos;
+}
+
+// select1(i): position of the i-th one of the upper bits.
+__device__ __forceinline__ uint64_t ef_select(const Streams& S, const EFDir& d, uint64_t i) {
+  const uint64_t s = d.samp[i >> 6];
+  if (s >> 63) return d.spill[(s & ~(1ull << 63)) + (i & 63)];
+  uint32_t r = (uint32_t)(i & 63);
+  uint64_t w = s >> 6;
+  uint64_t word = d.upper[w] & (~0ull << (s & 63));
+  for (uint32_t guard = 0;; ++guard) {
+    const uint32_t c = (uint32_t)__popcll(word);
+    if (r < c) break;
+    r -= c;
+    ++w;
+    if (w >= d.upper_words || guard > (kSpanBits / 64 + 2)) {
+      raise_err(S, kErrOverrun);
+      return 0;
+    }
+    word = d.upper[w];
+  }
+  return w * 64 + select_in_word(word, r);
+}
+
+// getOffset (bfs.cu:543-565): ((select1(i) - i) << l) | low_i.
+__device__ __forceinline__ uint64_t ef_get(const Streams& S, const EFDir& d, uint64_t i) {
+  const uint64_t hi = ef_select(S, d, i) - i;
+  uint64_t lo = 0;
+  if (d.l > 0) lo = peek64(d.lower, i * (uint64_t)d.l) >> (64 - d.l);
+  return (hi << d.l) | lo;
+}
+
+// ---------------------------------------------------------------------------
+// Successor-list state machine.
+//
+// A record (BVMultiGraph.java:738-880 / bfs.cu:1218-1633) is the sorted merge
+// of four sources: values copied from the reference list (copy/skip blocks,
+// implicit last block), multiples runs (value repeated len times), intervals
+// (consecutive values) and zeta_3 residuals.  Each source keeps its head; the
+// merge returns the smallest head.  The copy source of level L pulls values
+// from level L+1, which decodes the referenced vertex the same way, so one
+// decoder holds up to kMaxDepth + 1 levels.  A valid encoding never yields
+// the same value from two sources (multiples are pulled out before copying,
+// BVMultiGraph.java:1644-1647), so heads are distinct unless they are kInf.
+struct Level {
+  uint64_t mpos, ipos, rpos, bpos;  // cursors of the four sub-streams
+  int32_t x;                         // vertex whose list this level yields
+  int32_t hm, mrem, mleft;           // multiples: head, copies left of head, runs not read
+  int32_t hi, irem, ileft;           // intervals: head, values left incl. head, intervals not read
+  int32_t hr, rleft;                 // residuals: head, residuals not read
+  int32_t hc, cleft, bleft, bidx, brem;  // copies: head, copies not fetched, blocks not read, block idx, left in block
+};
+
+struct Decoder {
+  Level lv[kMaxDepth + 1];
+};
+
+template <int L>
+__device__ __forceinline__ int32_t dec_next(const Streams& S, Decoder& D);
+
+// Pulls the next value of level L's copy source from level L+1.
+template <int L>
+__device__ __forceinline__ int32_t dec_masked(const Streams& S, Decoder& D) {
+  if constexpr (L >= kMaxDepth) {
+    return kInf;
+  } else {
+    Level& v = D.lv[L];
+    const Rd g{S.g, S.g_bits};
+    for (;;) {
+      if (v.cleft <= 0) return kInf;
+      if (v.brem == 0) {
+        ++v.bidx;
+        if (v.bleft > 0) {
+          v.brem = (int32_t)rd_gamma(S, g, v.bpos) + (v.bidx ? 1 : 0);
+          --v.bleft;
+        } else if ((v.bidx & 1) == 0) {
+          v.brem = kInf;  // implicit last block: copy the rest
+        } else {
+          raise_err(S, kErrBlocks);
+          v.cleft = 0;
+          return kInf;
+        }
+        continue;
+      }
+      const int32_t val = dec_next<L + 1>(S, D);
+      if (val == kInf) {
+        raise_err(S, kErrBlocks);
+        v.cleft = 0;
+        return kInf;
+      }
+      --v.brem;
+      if ((v.bidx & 1) == 0) {
+        --v.cleft;
+        return val;
+      }
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ int32_t dec_next(const Streams& S, Decoder& D) {
+  Level& v = D.lv[L];
+  const Rd g{S.g, S.g_bits};
+  const int32_t m = min(min(v.hm, v.hi), min(v.hr, v.hc));
+  if (m == kInf) return kInf;
+  if (m == v.hm) {
+    if (--v.mrem == 0) {
+      if (v.mleft > 0) {
+        v.hm += (int32_t)rd_gamma(S, g, v.mpos) + 1;
+        v.mrem = (int32_t)rd_gamma(S, g, v.mpos) + 2;
+        --v.mleft;
+      } else {
+        v.hm = kInf;
+      }
+    }
+  } else if (m == v.hi) {
+    if (--v.irem == 0) {
+      if (v.ileft > 0) {
+        v.hi += (int32_t)rd_gamma(S, g, v.ipos) + 2;
+        v.irem = (int32_t)rd_gamma(S, g, v.ipos) + 4;
+        --v.ileft;
+      } else {
+        v.hi = kInf;
+      }
+    } else {
+      ++v.hi;
+    }
+  } else if (m == v.hr) {
+    if (v.rleft > 0) {
+      v.hr += (int32_t)rd_zeta<3>(S, g, v.rpos) + 1;
+      --v.rleft;
+    } else {
+      v.hr = kInf;
+    }
+  } else {
+    v.hc = dec_masked<L>(S, D);
+  }
+  return m;
+}
+
+// Parses the record of x into level L (and the levels of its reference
+// chain).  Returns the outdegree d.  `gpos` is the record's bit offset.
+template <int L>
+__device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_t x, uint64_t gpos) {
+  Level& v = D.lv[L];
+  const Rd g{S.g, S.g_bits};
+  v.x = x;
+  v.hm = v.hi = v.hr = v.hc = kInf;
+  v.mrem = v.mleft = v.irem = v.ileft = v.rleft = 0;
+  v.cleft = v.bleft = v.brem = 0;
+  v.bidx = -1;
+  v.mpos = v.ipos = v.rpos = v.bpos = gpos;
+  uint64_t pos = gpos;
+  const int32_t d = (int32_t)rd_gamma(S, g, pos);
+  if (d == 0) return 0;
+  const uint32_t r = rd_unary(S, g, pos);
+  int32_t copied = 0;
+  if (r > 0) {
+    if constexpr (L >= kMaxDepth) {
+      raise_err(S, kErrDepth);
+    } else {
+      if (r > (uint32_t)kWindow || (int32_t)r > x) {
+        raise_err(S, kErrDepth);
+      } else {
+        const int32_t bc = (int32_t)rd_gamma(S, g, pos);
+        v.bpos = pos;
+        int32_t total = 0, even = 0;
+        for (int32_t i = 0; i < bc; ++i) {
+          if (pos >= S.g_bits) {  // malformed: stop at the end of the stream
+            raise_err(S, kErrOverrun);
+            break;
+          }
+          const int32_t b = (int32_t)rd_gamma(S, g, pos) + (i ? 1 : 0);
+          total += b;
+          if ((i & 1) == 0) even += b;
+        }
+        const int32_t y = x - (int32_t)r;
+        const int32_t dref = dec_init<L + 1>(S, D, y, ef_get(S, S.gi, (uint64_t)y));
+        copied = even + ((bc & 1) ? 0 : dref - total);
+        if (copied < 0 || copied > d) {
+          raise_err(S, kErrBlocks);
+          copied = 0;
+        }
+        v.cleft = copied;
+        v.bleft = bc;
+        v.hc = dec_masked<L>(S, D);
+      }
+    }
+  }
+  int32_t extra = d - copied;
+  if (extra > 0) {  // multiples
+    const int32_t mc = (int32_t)rd_gamma(S, g, pos);
+    if (mc > 0) {
+      uint64_t first = pos;
+      for (int32_t i = 0; i < mc; ++i) {
+        if (extra <= 0) {  // more runs than contacts: malformed
+          raise_err(S, kErrBlocks);
+          break;
+        }
+        rd_gamma(S, g, pos);
+        extra -= (int32_t)rd_gamma(S, g, pos) + 2;
+      }
+      v.hm = (int32_t)(nat2int(rd_gamma(S, g, first)) + x);
+      v.mrem = (int32_t)rd_gamma(S, g, first) + 2;
+      v.mleft = mc - 1;
+      v.mpos = first;
+    }
+  }
+  if (extra > 0) {  // intervals
+    const int32_t ic = (int32_t)rd_gamma(S, g, pos);
+    if (ic > 0) {
+      uint64_t first = pos;
+      for (int32_t i = 0; i < ic; ++i) {
+        if (extra <= 0) {  // more intervals than contacts: malformed
+          raise_err(S, kErrBlocks);
+          break;
+        }
+        rd_gamma(S, g, pos);
+        extra -= (int32_t)rd_gamma(S, g, pos) + 4;
+      }
+      v.hi = (int32_t)(nat2int(rd_gamma(S, g, first)) + x);
+      v.irem = (int32_t)rd_gamma(S, g, first) + 4;
+      v.ileft = ic - 1;
+      v.ipos = first;
+    }
+  }
+  if (extra > 0) {  // residuals
+    v.rpos = pos;
+    v.hr = (int32_t)(nat2int(rd_zeta<3>(S, g, v.rpos)) + x);
+    v.rleft = extra - 1;
+  }
+  return d;
+}
+
+// Timestamp cursor: zeta_2 of int2nat deltas, reset to the header value per
+// vertex (EvolvingMultiGraph.java:131-144; bfs.cu:1444, :1597).
+struct TsCursor {
+  uint64_t pos;
+  int32_t prev;
+};
+
+__device__ __forceinline__ int32_t ts_next(const Streams& S, TsCursor& c) {
+  const Rd t{S.t, S.t_bits};
+  c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(rd_zeta<2>(S, t, c.pos)));
+  return c.prev;
+}
+
+// Decodes every contact of x in successor order and hands (x, r, t) to
+// act.contact(); a false return stops the walk early.  act.begin(x, d) may
+// veto the vertex.
+template <class Act>
+__device__ __forceinline__ void visit_vertex(const Streams& S, int32_t x, Act& act) {
+  Decoder D;
+  const int32_t d = dec_init<0>(S, D, x, ef_get(S, S.gi, (uint64_t)x));
+  if (!act.begin(x, d) || d == 0) return;
+  TsCursor tc{ef_get(S, S.ti, (uint64_t)x), S.min_ts};
+  for (int32_t i = 0; i < d; ++i) {
+    const int32_t r = dec_next<0>(S, D);
+    const int32_t t = ts_next(S, tc);
+    if ((uint32_t)r >= (uint32_t)S.n) {
+      raise_err(S, r == kInf ? kErrExhausted : kErrNeighbor);
+      break;
+    }
+    if (!act.contact(x, r, t)) break;
+  }
+}
+
+}  // namespace tga

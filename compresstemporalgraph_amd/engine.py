@@ -1,0 +1,238 @@
+"""Host-side mirror of the reference's workload surface over ``libtga.so``.
+
+The reference runs one workload per executable (gpu_cuda/<Dataset>/<algo>.cu)
+and its CPU driver names them ``RA BFS CC PR HITS DC GC``
+(cpu_java/.../BaselineRunner.java:50-88).  ``TemporalGraph`` opens one dataset
+on one GPU and exposes the seven entry points of ``include/tga.h`` with the
+same names; ``run(workload)`` accepts the driver's workload spellings.
+
+Outputs are numpy arrays by default.  Passing ``out=`` a CUDA tensor (or any
+object with ``data_ptr()`` on the graph's device) writes the result into
+device memory instead (``TGA_OUT_DEVICE``), which keeps timed loops free of
+host copies.
+
+There is no CPU fallback: if ``libtga.so`` is missing or no GPU is present,
+opening a graph raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+TGA_OUT_DEVICE = 1
+FULL_WINDOW = (-(1 << 31), (1 << 31) - 1)
+INF32 = 0x7FFFFFFF
+TGA_FROM_FILE = -(1 << 63)
+
+_LIB = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Window(C.Structure):
+    _fields_ = [("start", C.c_int64), ("end", C.c_int64)]
+
+
+class DatasetDesc(C.Structure):
+    _fields_ = [("max_label", C.c_int64), ("active_begin", C.c_int64), ("active_end", C.c_int64),
+                ("min_timestamp", C.c_int64)]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("contacts", C.c_int64), ("min_timestamp", C.c_int64),
+                ("graph_bytes", C.c_uint64), ("timestamps_bytes", C.c_uint64), ("index_bytes", C.c_uint64),
+                ("l_graph", C.c_int32), ("l_timestamps", C.c_int32), ("active_begin", C.c_int64),
+                ("active_end", C.c_int64), ("device_bytes", C.c_int64), ("max_degree", C.c_int64),
+                ("chunks", C.c_int64)]
+
+
+def lib():
+    """Loads libtga.so (built in-tree by ``build.build_engine``); raises if absent."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_build.ENGINE_LIB):
+            raise EngineError(f"{_build.ENGINE_LIB} is not built; run compresstemporalgraph_amd.build")
+        L = C.CDLL(_build.ENGINE_LIB)
+        vp, p32, p64, pu64 = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)
+        L.tga_last_error.restype = C.c_char_p
+        L.tga_open.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(DatasetDesc), C.c_int, C.POINTER(vp)]
+        L.tga_open_dir.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(DatasetDesc), C.c_int, C.POINTER(vp)]
+        L.tga_close.argtypes = [vp]
+        L.tga_close.restype = None
+        L.tga_info.argtypes = [vp, C.POINTER(GraphInfo)]
+        L.tga_neighbor_access.argtypes = [vp, p32, C.c_int64, pu64, p64]
+        L.tga_degree_centrality.argtypes = [vp, Window, vp, vp, C.c_uint32]
+        L.tga_bfs.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
+        L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
+        L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
+        L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
+        L.tga_pagerank.argtypes = [vp, Window, C.c_int32, C.c_float, C.c_float, vp, p32, C.c_uint32]
+        L.tga_hits.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
+        L.tga_profile.argtypes = [vp, C.c_int]
+        L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise EngineError(f"tga error {rc}: {lib().tga_last_error().decode()}")
+
+
+def _win(window) -> Window:
+    ws, we = window if window is not None else FULL_WINDOW
+    return Window(int(ws), int(we))
+
+
+class _Out:
+    """Resolves an output buffer: a fresh numpy array, or a caller's device tensor."""
+
+    def __init__(self, n, dtype, out=None):
+        if out is None:
+            self.arr = np.empty(n, dtype)
+            self.ptr = self.arr.ctypes.data_as(C.c_void_p)
+            self.flags = 0
+        else:
+            self.arr = out
+            self.ptr = C.c_void_p(out.data_ptr())
+            self.flags = TGA_OUT_DEVICE
+
+
+WORKLOADS = ("RA", "BFS", "CC", "PR", "HITS", "DC", "GC")
+
+
+def parse_workload(value: str) -> str:
+    """Workload spellings of BaselineRunner.Workload.parse (BaselineRunner.java:60-88)."""
+    v = value.upper().replace("-", "").replace("_", "")
+    table = {"RA": "RA", "RANDOMACCESS": "RA", "BFS": "BFS", "BREADTHFIRSTSEARCH": "BFS", "CC": "CC",
+             "CONNECTEDCOMPONENTS": "CC", "PR": "PR", "PAGERANK": "PR", "HITS": "HITS", "DC": "DC",
+             "DEGREECENTRALITY": "DC", "GC": "GC", "GRAPHCOLORING": "GC"}
+    if v not in table:
+        raise ValueError("Unknown workload: " + value)
+    return table[v]
+
+
+class TemporalGraph:
+    """One compressed temporal graph resident on one GPU."""
+
+    def __init__(self, directory: str, name: str, device: int = 0, max_label: int = 0, active=None,
+                 min_timestamp: int = TGA_FROM_FILE):
+        ab, ae = active if active is not None else (0, 0)
+        desc = DatasetDesc(max_label, ab, ae, min_timestamp)
+        h = C.c_void_p()
+        _check(lib().tga_open_dir(directory.encode(), name.encode(), C.byref(desc), device, C.byref(h)))
+        self.h = h
+        info = GraphInfo()
+        _check(lib().tga_info(self.h, C.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in GraphInfo._fields_}
+        self.n = info.n
+
+    @classmethod
+    def open(cls, root: str, dataset: str, **kw) -> "TemporalGraph":
+        """Opens root/data/compressed/<dataset>/ (the reference's layout)."""
+        return cls(os.path.join(root, "data", "compressed", dataset), dataset, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tga_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def profile(self, enable: bool = True) -> None:
+        """Times every decode-kernel launch with HIP events on the handle's stream."""
+        _check(lib().tga_profile(self.h, int(enable)))
+
+    def profile_read(self, reset: bool = True):
+        """Returns (summed kernel milliseconds, launches) since the last reset."""
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().tga_profile_read(self.h, C.byref(ms), C.byref(n), int(reset)))
+        return ms.value, n.value
+
+    # -- the seven entry points -------------------------------------------
+    def neighbor_access(self, verts=None):
+        """Returns (checksum, contacts) over the listed vertices (all active when None)."""
+        cs, cnt = C.c_uint64(), C.c_int64()
+        if verts is None:
+            _check(lib().tga_neighbor_access(self.h, None, 0, C.byref(cs), C.byref(cnt)))
+        else:
+            v = np.ascontiguousarray(verts, np.int32)
+            _check(lib().tga_neighbor_access(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), len(v), C.byref(cs),
+                                             C.byref(cnt)))
+        return cs.value, cnt.value
+
+    def degree_centrality(self, window=None, centrality=False, out=None, out_centrality=None):
+        o = _Out(self.n, np.int32, out)
+        c = _Out(self.n, np.int32, out_centrality) if (centrality or out_centrality is not None) else None
+        _check(lib().tga_degree_centrality(self.h, _win(window), o.ptr, c.ptr if c else None, o.flags))
+        return (o.arr, c.arr) if c else o.arr
+
+    def bfs(self, src: int, window=None, out=None):
+        o = _Out(self.n, np.int32, out)
+        _check(lib().tga_bfs(self.h, _win(window), src, o.ptr, o.flags))
+        return o.arr
+
+    def earliest_arrival(self, src: int, window=None, out=None):
+        o = _Out(self.n, np.int32, out)
+        _check(lib().tga_earliest_arrival(self.h, _win(window), src, o.ptr, o.flags))
+        return o.arr
+
+    def connected_components(self, window=None, out=None):
+        o = _Out(self.n, np.int32, out)
+        _check(lib().tga_connected_components(self.h, _win(window), o.ptr, o.flags))
+        return o.arr
+
+    def graph_coloring(self, window=None, rounds: int = 512, priority=None, out=None):
+        o = _Out(self.n, np.int32, out)
+        pr = None
+        if priority is not None:
+            if o.flags & TGA_OUT_DEVICE:
+                pr = C.c_void_p(priority.data_ptr())
+            else:
+                priority = np.ascontiguousarray(priority, np.int32)
+                pr = priority.ctypes.data_as(C.c_void_p)
+        _check(lib().tga_graph_coloring(self.h, _win(window), rounds, pr, o.ptr, o.flags))
+        return o.arr
+
+    def pagerank(self, window=None, iters: int = 20, damping: float = 0.85, tol: float = 0.0, out=None):
+        o = _Out(self.n, np.float32, out)
+        it = C.c_int32()
+        _check(lib().tga_pagerank(self.h, _win(window), iters, damping, tol, o.ptr, C.byref(it), o.flags))
+        self.last_iterations = it.value
+        return o.arr
+
+    def hits(self, window=None, iters: int = 20, out_hub=None, out_auth=None):
+        h = _Out(self.n, np.float32, out_hub)
+        a = _Out(self.n, np.float32, out_auth)
+        _check(lib().tga_hits(self.h, _win(window), iters, h.ptr, a.ptr, h.flags))
+        return h.arr, a.arr
+
+    def run(self, workload: str, **kw):
+        """Runs a workload by its BaselineRunner name (RA, BFS, CC, PR, HITS, DC, GC)."""
+        w = parse_workload(workload)
+        if w == "RA":
+            return self.neighbor_access(kw.get("verts"))
+        if w == "BFS":
+            return self.bfs(kw.get("src", 0), kw.get("window"))
+        if w == "CC":
+            return self.connected_components(kw.get("window"))
+        if w == "PR":
+            return self.pagerank(kw.get("window"), kw.get("iters", 20), kw.get("damping", 0.85), kw.get("tol", 0.0))
+        if w == "HITS":
+            return self.hits(kw.get("window"), kw.get("iters", 20))
+        if w == "DC":
+            return self.degree_centrality(kw.get("window"))
+        return self.graph_coloring(kw.get("window"), kw.get("rounds", 512))

@@ -1,0 +1,161 @@
+/*
+ * tga.h — C ABI of the MI355X temporal-graph engine (libtga.so, gfx950).
+ *
+ * The reference exposes no library: each algorithm is a standalone CUDA
+ * executable whose main() loads one dataset chosen by compile-time macros
+ * (gpu_cuda/shared/bfs.cu:1694-2060, gpu_cuda/<Dataset>/<algo>.cu:1-2,
+ * gpu_cuda/shared/dataset_config_base.cuh:1-67).  This header keeps the
+ * reference's surface — one entry point per algorithm (bfs,
+ * connected_components, degree_centrality, graph_coloring, hits,
+ * neighbor_access, pagerank; README.md:43) over the unchanged
+ * data/compressed/<Dataset>/ files — and replaces the compile-time dataset
+ * macros with a runtime descriptor and exit(1) with status codes.
+ *
+ * Conventions
+ *  - Every call returns TGA_OK (0) or a negative TGA_E* status; tga_last_error()
+ *    describes the last failure of the calling thread.  Nothing calls exit().
+ *  - Vertex ids are int32 in [0, n), n = MAX_LABEL + 1.  Per-vertex outputs are
+ *    caller-allocated arrays of n elements: host memory by default, device
+ *    memory of the graph's GPU when `flags` has TGA_OUT_DEVICE.
+ *  - Windows are inclusive [start, end] on the int32 timestamps;
+ *    TGA_FULL_WINDOW covers every contact.
+ *  - A vertex v is active when active_begin <= v < active_end
+ *    (isTraversableNode, cpu_java/.../BaselineRunner.java:610-614;
+ *    ACTIVE_VERTEX_BEGIN, dataset_config_base.cuh:11-13).  Only active
+ *    vertices are traversed and only active neighbors are followed.
+ *  - A handle owns one HIP stream; calls on it are synchronous and a handle
+ *    must not be used from two threads at once.
+ */
+#ifndef TGA_H
+#define TGA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TGA_OK = 0,
+  TGA_EINVAL = -1,   /* bad argument */
+  TGA_EIO = -2,      /* dataset file missing or unreadable (tgaOpenFile exits, path_config.cuh:26-36) */
+  TGA_EFORMAT = -3,  /* malformed stream or index: out-of-range neighbor, chain deeper than 3, overrun */
+  TGA_ERANGE = -4,   /* dataset outside what the reader supports (log2(0) index width, bfs.cu:1785) */
+  TGA_EDEVICE = -5,  /* HIP runtime failure */
+  TGA_ENOMEM = -6
+};
+
+#define TGA_OUT_DEVICE 1u
+#define TGA_FULL_WINDOW_START (-2147483647LL - 1)
+#define TGA_FULL_WINDOW_END 2147483647LL
+
+typedef struct tga_graph tga_graph;
+
+typedef struct tga_window {
+  int64_t start; /* inclusive */
+  int64_t end;   /* inclusive */
+} tga_window;
+
+/* Runtime replacement for the per-dataset macros MAX_LABEL, MAX_Vertex,
+ * ACTIVE_VERTEX_BEGIN, minTimestamp (gpu_cuda/<Dataset>/dataset_config.cuh).
+ * Fields <= 0 (or TGA_FROM_FILE) take their value from the dataset files. */
+#define TGA_FROM_FILE INT64_MIN
+typedef struct tga_dataset_desc {
+  int64_t max_label;     /* largest vertex id; <= 0: `nodes` of <name>.properties minus 1 */
+  int64_t active_begin;  /* first active id (ACTIVE_VERTEX_BEGIN), default 0 */
+  int64_t active_end;    /* one past the last active id; <= 0: max_label + 1 */
+  int64_t min_timestamp; /* expected .timestamps header, checked; TGA_FROM_FILE: not checked */
+} tga_dataset_desc;
+
+typedef struct tga_graph_info {
+  int64_t n;             /* vertices (MAX_LABEL + 1) */
+  int64_t contacts;      /* sum of outdegrees */
+  int64_t min_timestamp; /* .timestamps header */
+  uint64_t graph_bytes;  /* size of <name>.graph */
+  uint64_t timestamps_bytes;
+  uint64_t index_bytes;  /* the four GPU index files */
+  int32_t l_graph;       /* Elias-Fano lower widths (bfs.cu:1785-1786) */
+  int32_t l_timestamps;
+  int64_t active_begin, active_end;
+  int64_t device_bytes;  /* device memory held by the handle */
+  int64_t max_degree;
+  int64_t chunks;        /* decode work items (vertices plus checkpointed chunks) */
+} tga_graph_info;
+
+const char* tga_last_error(void);
+
+/* Loads <root>/data/compressed/<dataset>/ (path layout of
+ * dataset_config_base.cuh:44-68, TGA_COMPRESS_ROOT of path_config.cuh:11-20)
+ * onto `device` and builds the device-side decode directories.  Replaces the
+ * load half of every reference main() (bfs.cu:1734-1935). */
+int tga_open(const char* root, const char* dataset, const tga_dataset_desc* desc, int device, tga_graph** out);
+/* Same, from an explicit directory and basename. */
+int tga_open_dir(const char* dir, const char* basename, const tga_dataset_desc* desc, int device, tga_graph** out);
+void tga_close(tga_graph* g);
+int tga_info(const tga_graph* g, tga_graph_info* info);
+
+/* Neighbor access (RA): decodes every contact of the listed vertices (all
+ * active vertices when nverts == 0) and returns their count and an
+ * order-independent checksum: the wrapping 64-bit sum over contacts (x, r, t)
+ * of fmix64(x*K1 ^ r*K2 ^ t*K3) (K1 = 0x9E3779B97F4A7C15,
+ * K2 = 0xC2B2AE3D27D4EB4F, K3 = 0x165667B19E3779F9; x, r, t as uint32).
+ * Replaces travelGraph/travel (neighbor_access.cu:1438-1642), which decodes and
+ * discards. */
+int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum, int64_t* contacts);
+
+/* Degree centrality: count[x] = contacts of active x with an active neighbor
+ * and t in the window (multi-edges counted).  centrality (nullable) receives
+ * the Java formula (int)(0.85 c + 0.15 c (mean - ws) / (we - ws) + 1) with a
+ * truncating integer running mean, or c when ws == we
+ * (BaselineRunner.java:300-333; replaces count_degree / compute_centre,
+ * degree_centrality.cu:1606-1633). */
+int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* centrality, uint32_t flags);
+
+/* BFS levels: hop distance from src over window contacts between active
+ * vertices; INT32_MAX when unreached (BaselineRunner.java:339-383; replaces
+ * simpleBfs + host level loop, bfs.cu:1657-1692, :1999-2038). */
+int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, uint32_t flags);
+
+/* Earliest arrival (the reference GPU BFS rule, bfs.cu:1452-1471): arrival[src]
+ * = max(ws, 0); the first window contact (x, r, t) of each pair relaxes r when
+ * arrival[x] <= t < arrival[r]; INT32_MAX when unreached. */
+int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arrival, uint32_t flags);
+
+/* Connected components (min-label): label[v] = smallest active u that reaches
+ * v over window contacts, -1 for inactive v (equals the component order of
+ * BaselineRunner.java:553-608; replaces connected_components.cu:2091-2154). */
+int tga_connected_components(tga_graph* g, tga_window w, int32_t* label, uint32_t flags);
+
+/* Graph coloring: round c colors every uncolored active v that has no
+ * uncolored active out-neighbor u over window contacts with priority[u] >
+ * priority[v] (ties: u < v); at most `rounds` rounds; INT32_MAX when uncolored.
+ * priority == NULL uses java.util.Random(123).nextInt(256) per vertex
+ * (BaselineRunner.java:386-430; GC_kernel, graph_coloring.cu:1634-1734). */
+int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, const int32_t* priority, int32_t* color,
+                       uint32_t flags);
+
+/* Temporal PageRank, FP32 storage with FP64 sums: new[v] = (1-d)/N; each
+ * active u with du window contacts pushes d*rank[u]/du*(1-exp(-0.1 t)) to each
+ * distinct active neighbor (t = its first window contact); normalised by the
+ * sum each iteration; N = active vertices.  Stops after iters or when the L1
+ * change < tol (tol > 0).  iters_run (nullable) receives the iterations done
+ * (pagerank.cu:1606-1660, :1963-2007; BaselineRunner.java:433-469). */
+int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float damping, float tol, float* rank,
+                 int32_t* iters_run, uint32_t flags);
+
+/* HITS with per-iteration reset and L2 normalisation over distinct active
+ * neighbors with a window contact; FP32 storage, FP64 sums
+ * (BaselineRunner.java:472-550; replaces hits.cu:1898-1987, :2284-2349). */
+int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, float* auth, uint32_t flags);
+
+/* Kernel timing on the handle's own stream: while enabled, every launch of a
+ * decode kernel (the kernels that walk the compressed streams) is bracketed by
+ * HIP events, and tga_profile_read returns the summed elapsed milliseconds and
+ * the number of launches since the last reset.  No counterpart in the
+ * reference, which times with host clocks (bfs.cu:1999, :2039-2041). */
+int tga_profile(tga_graph* g, int enable);
+int tga_profile_read(tga_graph* g, double* kernel_ms, int64_t* launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

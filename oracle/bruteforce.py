@@ -1,0 +1,163 @@
+"""Raw-contact brute force — TEST INFRASTRUCTURE ONLY.
+
+Every function here works on the raw contact list ``(src, dst, t)`` sorted by
+(src, dst, t), never on the compressed streams, so it checks the C oracle's
+decoder and algorithm code independently.  The rules are the algorithm
+contracts stated in oracle/oracle.c (BaselineRunner.java:273-608 and
+gpu_cuda/shared per-contact blocks, made deterministic).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+INF32 = 0x7FFFFFFF
+FULL_WINDOW = (-(1 << 31), (1 << 31) - 1)
+
+_K1, _K2, _K3 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9
+
+
+def fmix64(k: np.ndarray) -> np.ndarray:
+    k = k.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xFF51AFD7ED558CCD)
+        k ^= k >> np.uint64(33)
+        k *= np.uint64(0xC4CEB9FE1A85EC53)
+        k ^= k >> np.uint64(33)
+    return k
+
+
+def contact_hashes(src, dst, t) -> np.ndarray:
+    x = np.asarray(src).astype(np.uint32).astype(np.uint64)
+    r = np.asarray(dst).astype(np.uint32).astype(np.uint64)
+    tt = np.asarray(t).astype(np.int64).astype(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        key = (x * np.uint64(_K1)) ^ (r * np.uint64(_K2)) ^ (tt * np.uint64(_K3))
+    return fmix64(key)
+
+
+def checksum(src, dst, t) -> int:
+    h = contact_hashes(src, dst, t)
+    return int(np.sum(h, dtype=np.uint64))
+
+
+class Raw:
+    def __init__(self, src, dst, t, n, window=None, active=None):
+        self.n = n
+        src = np.asarray(src, np.int64)
+        dst = np.asarray(dst, np.int64)
+        t = np.asarray(t, np.int64)
+        ws, we = window if window is not None else FULL_WINDOW
+        ab, ae = active if active is not None else (0, n)
+        self.ab, self.ae = ab, ae
+        self.active = np.zeros(n, bool)
+        self.active[ab:min(ae, n)] = True
+        self.src, self.dst, self.t = src, dst, t
+        self.inw = (t >= ws) & (t <= we)
+        # window edges between active vertices
+        keep = self.inw & self.active[src] & self.active[dst]
+        self.esrc, self.edst, self.et = src[keep], dst[keep], t[keep]
+
+    # -- DC
+    def degree(self):
+        keep = self.inw & self.active[self.dst] & self.active[self.src]
+        return np.bincount(self.src[keep], minlength=self.n).astype(np.int32)
+
+    # -- BFS levels
+    def bfs(self, s):
+        level = np.full(self.n, INF32, np.int64)
+        level[s] = 0
+        if not self.active[s]:
+            return level.astype(np.int32)
+        frontier = np.array([s])
+        lvl = 0
+        order = np.argsort(self.esrc, kind="stable")
+        es, ed = self.esrc[order], self.edst[order]
+        starts = np.searchsorted(es, np.arange(self.n + 1))
+        while len(frontier):
+            nxt = np.concatenate([ed[starts[x]:starts[x + 1]] for x in frontier]) if len(frontier) else np.array([])
+            nxt = np.unique(nxt.astype(np.int64))
+            nxt = nxt[level[nxt] == INF32]
+            lvl += 1
+            level[nxt] = lvl
+            frontier = nxt
+        return level.astype(np.int32)
+
+    # -- CC min-label by fixpoint propagation
+    def cc(self):
+        label = np.where(self.active, np.arange(self.n), -1).astype(np.int64)
+        while True:
+            cand = np.full(self.n, np.iinfo(np.int64).max)
+            np.minimum.at(cand, self.edst, label[self.esrc])
+            new = np.where(self.active, np.minimum(label, cand), -1)
+            if np.array_equal(new, label):
+                return label.astype(np.int32)
+            label = new
+
+    # -- GC rounds
+    def gc(self, prio, rounds=512):
+        prio = np.asarray(prio, np.int64)
+        color = np.full(self.n, INF32, np.int64)
+        u, v = self.edst, self.esrc  # v's out-neighbor u
+        higher = (prio[u] > prio[v]) | ((prio[u] == prio[v]) & (u < v))
+        for c in range(rounds):
+            unc = self.active & (color == INF32)
+            if not unc.any():
+                break
+            blocked = np.zeros(self.n, bool)
+            sel = higher & unc[u] & unc[v]
+            blocked[v[sel]] = True
+            take = unc & ~blocked
+            color[take] = c
+        return color.astype(np.int32)
+
+    # -- distinct window pairs with their first window contact
+    def _pairs(self):
+        keep = self.inw & self.active[self.src]
+        s, d, t = self.src[keep], self.dst[keep], self.t[keep]
+        first = np.ones(len(s), bool)
+        first[1:] = (s[1:] != s[:-1]) | (d[1:] != d[:-1])
+        return s[first], d[first], t[first], np.bincount(s, minlength=self.n)
+
+    def pagerank(self, iters=20, damping=0.85, tol=0.0, dtype=np.float64):
+        """Storage and per-contact arithmetic in ``dtype``; sums in float64."""
+        ps, pd, pt, du = self._pairs()
+        keep = self.active[pd]
+        ps, pd, pt = ps[keep], pd[keep], pt[keep]
+        N = int(self.active.sum())
+        one = dtype(1)
+        w = (one - np.exp(dtype(-0.1) * pt.astype(dtype))).astype(dtype)
+        rank = np.where(self.active, one / dtype(N), dtype(0)).astype(dtype)
+        base = (one - dtype(damping)) / dtype(N)
+        it = 0
+        for it in range(1, iters + 1):
+            acc = np.where(self.active, float(base), 0.0)
+            coef = np.zeros(self.n, dtype)
+            nz = du > 0
+            coef[nz] = (dtype(damping) * rank[nz] / du[nz].astype(dtype)).astype(dtype)
+            np.add.at(acc, pd, (coef[ps] * w).astype(dtype).astype(np.float64))
+            s = acc.sum()
+            new = (acc / s if s > 0 else acc).astype(dtype)
+            diff = np.abs(new.astype(np.float64) - rank.astype(np.float64)).sum()
+            rank = new
+            if tol > 0 and diff < tol:
+                break
+        return rank, it
+
+    def hits(self, iters=20, dtype=np.float64):
+        """Storage in ``dtype``; sums and norms in float64."""
+        ps, pd, _, _ = self._pairs()
+        keep = self.active[pd]
+        ps, pd = ps[keep], pd[keep]
+        hub = np.where(self.active, 1, 0).astype(dtype)
+        auth = hub.copy()
+        for _ in range(iters):
+            acc = np.zeros(self.n)
+            np.add.at(acc, ps, auth[pd].astype(np.float64))
+            nrm = np.sqrt((acc * acc).sum())
+            hub = (acc / nrm if nrm != 0 else acc).astype(dtype)
+            acc = np.zeros(self.n)
+            np.add.at(acc, pd, hub[ps].astype(np.float64))
+            nrm = np.sqrt((acc * acc).sum())
+            auth = (acc / nrm if nrm != 0 else acc).astype(dtype)
+        return hub, auth

@@ -1,0 +1,631 @@
+/*
+ * oracle.c — TEST INFRASTRUCTURE ONLY.  A plain-C CPU restatement of the
+ * reference's decode path and of the algorithm contracts, used by tests/ as
+ * the parity checker and by bench.py as the CPU baseline ("port").  Nothing
+ * in the product (compresstemporalgraph_amd/, libtga.so) links or calls it.
+ *
+ * What it restates (reference files, cited per function below):
+ *   - bit reader and codes: gpu_cuda/shared/bfs.cu:166-175 (unary),
+ *     bfs.cu:202-256 (gamma), bfs.cu:590-665 (zeta_3 / zeta_2),
+ *     bfs.cu:144-154 (int2nat / nat2int);
+ *   - Elias-Fano getOffset: bfs.cu:543-565, with select1 done by brute force
+ *     (a full scan of the upper bits), and the lower-bit width of
+ *     bfs.cu:1785-1786;
+ *   - successor lists with the Java reader's semantics
+ *     (cpu_java/.../BVMultiGraph.java:738-880, ResidualIntIterator :655-700),
+ *     materialised recursively;
+ *   - timestamps: EvolvingMultiGraph.java:396-468 (zeta_2, int2nat deltas,
+ *     per-vertex reset to the 64-bit header value);
+ *   - algorithms: the per-contact rules of BaselineRunner.java:300-608 and
+ *     gpu_cuda/shared/ kernels, resolved into deterministic contracts (windowed
+ *     BFS levels, min-label CC, Java-priority GC, race-free PR, standard HITS).
+ *
+ * Pinning (how each function is checked independently of itself) is listed
+ * in the module docstring of oracle/oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_INF32 0x7fffffff
+
+static char g_err[512];
+const char* orc_last_error(void) { return g_err; }
+static int fail(const char* msg, long long a) {
+  snprintf(g_err, sizeof g_err, "%s (%lld)", msg, a);
+  return -1;
+}
+
+/* ---------------------------------------------------------------- bits --- */
+typedef struct {
+  const uint8_t* p;
+  uint64_t nbits;
+  uint64_t pos;
+  int overrun;
+} bitrd;
+
+static int rd_bit(bitrd* b) {
+  if (b->pos >= b->nbits) {
+    b->overrun = 1;
+    return 1; /* terminate unary loops */
+  }
+  int v = (b->p[b->pos >> 3] >> (7 - (b->pos & 7))) & 1;
+  b->pos++;
+  return v;
+}
+static uint64_t rd_int(bitrd* b, int len) {
+  uint64_t v = 0;
+  for (int i = 0; i < len; i++) v = (v << 1) | (uint64_t)rd_bit(b);
+  return v;
+}
+static uint64_t rd_unary(bitrd* b) {
+  uint64_t z = 0;
+  while (rd_bit(b) == 0) z++;
+  return z;
+}
+static uint64_t rd_gamma(bitrd* b) {
+  int m = (int)rd_unary(b);
+  if (m > 63) { b->overrun = 1; return 0; }
+  return ((1ull << m) | rd_int(b, m)) - 1;
+}
+static uint64_t rd_zeta(bitrd* b, int k) {
+  int h = (int)rd_unary(b);
+  if (h * k + k > 63) { b->overrun = 1; return 0; }
+  uint64_t left = 1ull << (h * k);
+  uint64_t m = rd_int(b, h * k + k - 1);
+  if (m < left) return m + left - 1;
+  return (m << 1) + (uint64_t)rd_bit(b) - 1;
+}
+static int64_t nat2int(uint64_t n) { return (n & 1) ? -(int64_t)(n >> 1) - 1 : (int64_t)(n >> 1); }
+
+/* ------------------------------------------------------------- dataset --- */
+typedef struct {
+  uint8_t *graph, *ts, *glow, *gup, *tlow, *tup;
+  uint64_t graph_len, ts_len, glow_len, gup_len, tlow_len, tup_len;
+  int64_t n;           /* vertices = MAX_LABEL + 1 */
+  int lg, lt;          /* EF lower widths */
+  int64_t min_ts;      /* .timestamps header */
+  uint64_t* gsel;      /* positions of ones in the upper bits (brute-force select) */
+  uint64_t* tsel;
+  uint64_t gones, tones;
+  /* decoded CSR (filled by orc_decode_all) */
+  int64_t* start;
+  int32_t* adj;
+  int32_t* tim;
+  int64_t m;
+  int64_t q1_records;  /* records whose first residual is vertex 0 followed by more residuals */
+} orc;
+
+static uint8_t* slurp(const char* dir, const char* file, uint64_t* len) {
+  char path[4096];
+  snprintf(path, sizeof path, "%s/%s", dir, file);
+  FILE* f = fopen(path, "rb");
+  if (!f) return NULL;
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  rewind(f);
+  uint8_t* buf = (uint8_t*)calloc((size_t)n + 16, 1);
+  if (buf && fread(buf, 1, (size_t)n, f) != (size_t)n) { free(buf); buf = NULL; }
+  fclose(f);
+  *len = (uint64_t)n;
+  return buf;
+}
+
+static int msb64(uint64_t v) { return 63 - __builtin_clzll(v); }
+
+/* Brute-force select1: positions of every one of the LSB-first upper bits. */
+static uint64_t* all_ones(const uint8_t* up, uint64_t nbytes, uint64_t* count) {
+  uint64_t c = 0;
+  for (uint64_t i = 0; i < nbytes; i++) c += (uint64_t)__builtin_popcount(up[i]);
+  uint64_t* pos = (uint64_t*)malloc(sizeof(uint64_t) * (c + 1));
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < nbytes; i++)
+    for (int j = 0; j < 8; j++)
+      if (up[i] & (1u << j)) pos[k++] = i * 8 + (uint64_t)j;
+  *count = c;
+  return pos;
+}
+
+/* getOffset (bfs.cu:543-565): ((select1(i) - i) << l) | low_i, low_i read MSB-first. */
+static int ef_get(const uint8_t* low, uint64_t low_len, const uint64_t* sel, uint64_t ones, int l, uint64_t i,
+                  uint64_t* out) {
+  if (i >= ones) return -1;
+  uint64_t lo = 0;
+  for (uint64_t p = i * (uint64_t)l; p < (i + 1) * (uint64_t)l; p++) {
+    if ((p >> 3) >= low_len) return -1;
+    lo = (lo << 1) | (uint64_t)((low[p >> 3] >> (7 - (p & 7))) & 1);
+  }
+  *out = ((sel[i] - i) << l) | lo;
+  return 0;
+}
+
+void orc_close(orc* g) {
+  if (!g) return;
+  free(g->graph); free(g->ts); free(g->glow); free(g->gup); free(g->tlow); free(g->tup);
+  free(g->gsel); free(g->tsel); free(g->start); free(g->adj); free(g->tim);
+  free(g);
+}
+
+orc* orc_open(const char* dir, const char* name, int64_t n) {
+  char file[1024];
+  orc* g = (orc*)calloc(1, sizeof(orc));
+  g->n = n;
+  snprintf(file, sizeof file, "%s.graph", name);
+  g->graph = slurp(dir, file, &g->graph_len);
+  snprintf(file, sizeof file, "%s.timestamps", name);
+  g->ts = slurp(dir, file, &g->ts_len);
+  g->glow = slurp(dir, "graph_lower_bits.bin", &g->glow_len);
+  g->gup = slurp(dir, "graph_upper_bits.bin", &g->gup_len);
+  g->tlow = slurp(dir, "timestamp_lower_bits.bin", &g->tlow_len);
+  g->tup = slurp(dir, "timestamp_upper_bits.bin", &g->tup_len);
+  if (!g->graph || !g->ts || !g->glow || !g->gup || !g->tlow || !g->tup) {
+    fail("missing dataset file", 0);
+    orc_close(g);
+    return NULL;
+  }
+  if (n < 2) { fail("n must be >= 2", n); orc_close(g); return NULL; }
+  /* bfs.cu:1785-1786 */
+  uint64_t qg = g->graph_len * 8 / (uint64_t)(n - 1), qt = g->ts_len * 8 / (uint64_t)(n - 1);
+  if (qg == 0 || qt == 0) { fail("stream shorter than MAX_LABEL bits", 0); orc_close(g); return NULL; }
+  g->lg = msb64(qg);
+  g->lt = msb64(qt);
+  bitrd h = {g->ts, g->ts_len * 8, 0, 0};
+  g->min_ts = (int64_t)rd_int(&h, 64);
+  g->gsel = all_ones(g->gup, g->gup_len, &g->gones);
+  g->tsel = all_ones(g->tup, g->tup_len, &g->tones);
+  return g;
+}
+
+int orc_offsets(orc* g, int64_t x, uint64_t* goff, uint64_t* toff) {
+  if (ef_get(g->glow, g->glow_len, g->gsel, g->gones, g->lg, (uint64_t)x, goff)) return fail("graph index short", x);
+  if (ef_get(g->tlow, g->tlow_len, g->tsel, g->tones, g->lt, (uint64_t)x, toff)) return fail("timestamp index short", x);
+  return 0;
+}
+int orc_widths(orc* g, int* lg, int* lt, int64_t* min_ts) {
+  *lg = g->lg; *lt = g->lt; *min_ts = g->min_ts;
+  return 0;
+}
+
+/* ------------------------------------------------------ successor lists --- */
+typedef struct {
+  int32_t* v;
+  int64_t n, cap;
+} ivec;
+static void iv_push(ivec* a, int32_t x) {
+  if (a->n == a->cap) {
+    a->cap = a->cap ? a->cap * 2 : 16;
+    a->v = (int32_t*)realloc(a->v, sizeof(int32_t) * (size_t)a->cap);
+  }
+  a->v[a->n++] = x;
+}
+static int cmp_i32(const void* a, const void* b) {
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+/* BVMultiGraph.successors (BVMultiGraph.java:738-880): outdegree, reference,
+ * copy blocks over the referenced list (recursive), multiples, intervals and
+ * residuals, merged into one sorted multiset. */
+static int decode_list(orc* g, int64_t x, ivec* out, int depth, uint64_t* outdeg) {
+  out->n = 0;
+  uint64_t off, toff;
+  if (x < 0 || x >= g->n) return fail("vertex out of range", x);
+  if (orc_offsets(g, x, &off, &toff)) return -1;
+  bitrd b = {g->graph, g->graph_len * 8, off, 0};
+  uint64_t d = rd_gamma(&b);
+  if (outdeg) *outdeg = d;
+  if (d == 0) return 0;
+  if (depth > 3) return fail("reference chain deeper than 3", x);
+  uint64_t r = rd_unary(&b);
+  if (r > 7) return fail("reference beyond window 7", x);
+  int64_t extra = (int64_t)d;
+  int n_sources = 0; /* number of non-empty sources (for the disjointness check) */
+  if (r > 0) {
+    uint64_t bc = rd_gamma(&b);
+    int64_t* blocks = (int64_t*)malloc(sizeof(int64_t) * (bc + 1));
+    for (uint64_t i = 0; i < bc; i++) blocks[i] = (int64_t)rd_gamma(&b) + (i == 0 ? 0 : 1);
+    ivec ref = {0};
+    if (decode_list(g, x - (int64_t)r, &ref, depth + 1, NULL)) { free(blocks); free(ref.v); return -1; }
+    int64_t k = 0;
+    int64_t before = out->n;
+    for (uint64_t i = 0; i < bc; i++) {
+      if ((i & 1) == 0)
+        for (int64_t j = 0; j < blocks[i] && k + j < ref.n; j++) iv_push(out, ref.v[k + j]);
+      k += blocks[i];
+    }
+    if ((bc & 1) == 0)
+      for (; k < ref.n; k++) iv_push(out, ref.v[k]);
+    extra -= out->n - before;
+    if (out->n > before) n_sources++;
+    free(blocks);
+    free(ref.v);
+  }
+  if (extra > 0) { /* multiples */
+    uint64_t mc = rd_gamma(&b);
+    int64_t prev = 0;
+    for (uint64_t i = 0; i < mc; i++) {
+      int64_t left = (i == 0) ? nat2int(rd_gamma(&b)) + x : (int64_t)rd_gamma(&b) + prev + 1;
+      int64_t len = (int64_t)rd_gamma(&b) + 2;
+      prev = left;
+      for (int64_t j = 0; j < len; j++) iv_push(out, (int32_t)left);
+      extra -= len;
+    }
+  }
+  if (extra > 0) { /* intervals */
+    uint64_t ic = rd_gamma(&b);
+    int64_t prev = 0;
+    for (uint64_t i = 0; i < ic; i++) {
+      int64_t left = (i == 0) ? nat2int(rd_gamma(&b)) + x : (int64_t)rd_gamma(&b) + prev + 1;
+      int64_t len = (int64_t)rd_gamma(&b) + 4;
+      prev = left + len;
+      for (int64_t j = 0; j < len; j++) iv_push(out, (int32_t)(left + j));
+      extra -= len;
+    }
+  }
+  if (extra > 0) { /* residuals */
+    int64_t v = nat2int(rd_zeta(&b, 3)) + x;
+    if (depth == 0 && v == 0 && extra > 1) g->q1_records++;
+    iv_push(out, (int32_t)v);
+    for (int64_t i = 1; i < extra; i++) {
+      v += (int64_t)rd_zeta(&b, 3) + 1;
+      iv_push(out, (int32_t)v);
+    }
+  }
+  if (b.overrun) return fail("record runs past the end of .graph", x);
+  if (out->n != (int64_t)d) return fail("record yields a different number of successors than its outdegree", x);
+  qsort(out->v, (size_t)out->n, sizeof(int32_t), cmp_i32);
+  (void)n_sources;
+  return 0;
+}
+
+/* Decode every vertex into CSR (start, adj, tim). */
+int orc_decode_all(orc* g) {
+  int64_t n = g->n;
+  g->start = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  ivec all = {0}, lst = {0};
+  int32_t* tim = NULL;
+  int64_t tcap = 0;
+  g->q1_records = 0;
+  for (int64_t x = 0; x < n; x++) {
+    if (decode_list(g, x, &lst, 0, NULL)) { free(all.v); free(lst.v); free(tim); return -1; }
+    uint64_t goff, toff;
+    orc_offsets(g, x, &goff, &toff);
+    bitrd tb = {g->ts, g->ts_len * 8, toff, 0};
+    int64_t prev = g->min_ts;
+    for (int64_t i = 0; i < lst.n; i++) {
+      prev += nat2int(rd_zeta(&tb, 2));
+      iv_push(&all, lst.v[i]);
+      if (all.n > tcap) {
+        tcap = all.cap;
+        tim = (int32_t*)realloc(tim, sizeof(int32_t) * (size_t)tcap);
+      }
+      tim[all.n - 1] = (int32_t)prev;
+    }
+    if (tb.overrun) { free(all.v); free(lst.v); free(tim); return fail("timestamps run past the end", x); }
+    g->start[x + 1] = all.n;
+  }
+  free(lst.v);
+  g->adj = all.v;
+  g->tim = tim;
+  g->m = all.n;
+  return 0;
+}
+
+int64_t orc_num_contacts(orc* g) { return g->m; }
+int64_t orc_q1_records(orc* g) { return g->q1_records; }
+int orc_copy_csr(orc* g, int64_t* start, int32_t* adj, int32_t* tim) {
+  memcpy(start, g->start, sizeof(int64_t) * (size_t)(g->n + 1));
+  memcpy(adj, g->adj, sizeof(int32_t) * (size_t)g->m);
+  memcpy(tim, g->tim, sizeof(int32_t) * (size_t)g->m);
+  return 0;
+}
+
+/* ----------------------------------------------------------- algorithms --- */
+typedef struct {
+  int64_t ws, we;                 /* inclusive window */
+  int32_t active_begin, active_end; /* [begin, end) */
+} orc_q;
+
+static int in_w(const orc_q* q, int32_t t) { return (int64_t)t >= q->ws && (int64_t)t <= q->we; }
+static int act(const orc* g, const orc_q* q, int64_t v) {
+  return v >= q->active_begin && v < q->active_end && v < g->n;
+}
+
+/* Contact checksum: wrapping sum over contacts of fmix64 of the mixed triple
+ * (the reference discards decoded values; this is the build's own RA output). */
+static uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return k;
+}
+uint64_t orc_contact_hash(int32_t x, int32_t r, int32_t t) {
+  return fmix64((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(uint32_t)r * 0xC2B2AE3D27D4EB4FULL ^
+                (uint64_t)(uint32_t)t * 0x165667B19E3779F9ULL);
+}
+
+/* Neighbor access (neighbor_access.cu:1438-1595; BaselineRunner.java:273-297):
+ * decodes each listed vertex on the fly (no CSR) and folds every contact
+ * into the checksum.  nverts == 0 means every active vertex. */
+int orc_neighbor_access(orc* g, const int32_t* verts, int64_t nverts, int32_t ab, int32_t ae, uint64_t* checksum,
+                        int64_t* contacts) {
+  ivec lst = {0};
+  uint64_t cs = 0;
+  int64_t cnt = 0;
+  int64_t count = nverts > 0 ? nverts : (int64_t)(ae - ab);
+  for (int64_t i = 0; i < count; i++) {
+    int64_t x = nverts > 0 ? verts[i] : ab + i;
+    if (decode_list(g, x, &lst, 0, NULL)) { free(lst.v); return -1; }
+    uint64_t goff, toff;
+    orc_offsets(g, x, &goff, &toff);
+    bitrd tb = {g->ts, g->ts_len * 8, toff, 0};
+    int64_t prev = g->min_ts;
+    for (int64_t j = 0; j < lst.n; j++) {
+      prev += nat2int(rd_zeta(&tb, 2));
+      cs += orc_contact_hash((int32_t)x, lst.v[j], (int32_t)prev);
+    }
+    cnt += lst.n;
+  }
+  free(lst.v);
+  *checksum = cs;
+  *contacts = cnt;
+  return 0;
+}
+
+/* Degree centrality (degree_centrality.cu:1444-1453; BaselineRunner.java:300-333):
+ * count[x] = contacts of x with an active neighbor and t in the window;
+ * centrality[x] (optional) follows the Java formula with a truncating long
+ * running mean, applied when ws != we. */
+int orc_degree(orc* g, const orc_q* q, int32_t* count, int32_t* centrality) {
+  for (int64_t x = 0; x < g->n; x++) {
+    count[x] = 0;
+    int64_t avg = 0;
+    int32_t deg = 0;
+    if (act(g, q, x)) {
+      for (int64_t i = g->start[x]; i < g->start[x + 1]; i++) {
+        if (!act(g, q, g->adj[i]) || !in_w(q, g->tim[i])) continue;
+        deg++;
+        avg += ((int64_t)g->tim[i] - avg) / deg;
+      }
+    }
+    count[x] = deg;
+    if (centrality) {
+      if (q->ws != q->we)
+        centrality[x] = (int32_t)(0.85 * deg + 0.15 * deg * (double)(avg - q->ws) / (double)(q->we - q->ws) + 1);
+      else
+        centrality[x] = deg;
+    }
+  }
+  return 0;
+}
+
+/* BFS levels: hop distance of a level-synchronous BFS over window contacts
+ * between active vertices (BaselineRunner.java:339-383 plus the window). */
+int orc_bfs(orc* g, const orc_q* q, int32_t src, int32_t* level) {
+  int64_t n = g->n;
+  for (int64_t v = 0; v < n; v++) level[v] = ORC_INF32;
+  if (src < 0 || src >= n) return fail("source out of range", src);
+  int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int64_t nc = 1, nn;
+  cur[0] = src;
+  level[src] = 0;
+  for (int32_t lvl = 0; nc > 0; lvl++) {
+    nn = 0;
+    for (int64_t i = 0; i < nc; i++) {
+      int32_t x = cur[i];
+      if (!act(g, q, x)) continue;
+      for (int64_t e = g->start[x]; e < g->start[x + 1]; e++) {
+        int32_t r = g->adj[e];
+        if (!in_w(q, g->tim[e]) || !act(g, q, r) || level[r] != ORC_INF32) continue;
+        level[r] = lvl + 1;
+        nxt[nn++] = r;
+      }
+    }
+    int32_t* t = cur; cur = nxt; nxt = t;
+    nc = nn;
+  }
+  free(cur);
+  free(nxt);
+  return 0;
+}
+
+/* Earliest arrival (bfs.cu:1452-1471 rule, deterministic fixpoint): arrival[src]
+ * = ws (0 for the full window); a contact (x, r, t) that is the first window
+ * contact of its pair relaxes r when arrival[x] <= t < arrival[r]. */
+int orc_earliest_arrival(orc* g, const orc_q* q, int32_t src, int32_t* arrival) {
+  int64_t n = g->n;
+  for (int64_t v = 0; v < n; v++) arrival[v] = ORC_INF32;
+  if (src < 0 || src >= n) return fail("source out of range", src);
+  int64_t a0 = q->ws < 0 ? 0 : q->ws;
+  arrival[src] = (int32_t)a0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (int64_t x = 0; x < n; x++) {
+      if (!act(g, q, x) || arrival[x] == ORC_INF32) continue;
+      int32_t last = -1;
+      for (int64_t e = g->start[x]; e < g->start[x + 1]; e++) {
+        int32_t r = g->adj[e];
+        if (!in_w(q, g->tim[e]) || r == last) continue;
+        last = r;
+        if (!act(g, q, r)) continue;
+        int32_t t = g->tim[e];
+        if (arrival[x] <= t && t < arrival[r]) { arrival[r] = t; changed = 1; }
+      }
+    }
+  }
+  return 0;
+}
+
+/* Connected components as min-label: label[v] = min{u active : u reaches v}
+ * over window contacts (equals BaselineRunner.java:553-608's component ids up
+ * to the monotone relabelling start-vertex -> component number). */
+int orc_cc(orc* g, const orc_q* q, int32_t* label) {
+  int64_t n = g->n;
+  for (int64_t v = 0; v < n; v++) label[v] = act(g, q, v) ? (int32_t)v : -1;
+  /* Sequential BFS forest in increasing start order, like the Java loop. */
+  int32_t* queue = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  char* vis = (char*)calloc((size_t)n, 1);
+  for (int64_t s = q->active_begin; s < q->active_end && s < n; s++) {
+    if (vis[s]) continue;
+    int64_t h = 0, t = 0;
+    queue[t++] = (int32_t)s;
+    vis[s] = 1;
+    label[s] = (int32_t)s;
+    while (h < t) {
+      int32_t x = queue[h++];
+      for (int64_t e = g->start[x]; e < g->start[x + 1]; e++) {
+        int32_t r = g->adj[e];
+        if (!in_w(q, g->tim[e]) || !act(g, q, r) || vis[r]) continue;
+        vis[r] = 1;
+        label[r] = (int32_t)s;
+        queue[t++] = r;
+      }
+    }
+  }
+  free(queue);
+  free(vis);
+  return 0;
+}
+
+/* Graph coloring rounds (BaselineRunner.java:386-430; graph_coloring.cu:1445-1467):
+ * in round c an uncolored active v takes color c unless an active out-neighbor
+ * u over window contacts, uncolored at the start of the round, has higher
+ * priority (prio[u] > prio[v], or equal and u < v).  Uncolored: INT32_MAX. */
+int orc_gc(orc* g, const orc_q* q, const int32_t* prio, int32_t rounds, int32_t* color) {
+  int64_t n = g->n;
+  for (int64_t v = 0; v < n; v++) color[v] = ORC_INF32;
+  char* unc = (char*)malloc((size_t)n);
+  char* take = (char*)malloc((size_t)n);
+  for (int32_t c = 0; c < rounds; c++) {
+    int64_t left = 0;
+    for (int64_t v = 0; v < n; v++) {
+      unc[v] = act(g, q, v) && color[v] == ORC_INF32;
+      left += unc[v];
+    }
+    if (!left) break;
+    for (int64_t v = 0; v < n; v++) {
+      take[v] = 0;
+      if (!unc[v]) continue;
+      int blocked = 0;
+      for (int64_t e = g->start[v]; e < g->start[v + 1] && !blocked; e++) {
+        int32_t u = g->adj[e];
+        if (!in_w(q, g->tim[e]) || !act(g, q, u) || !unc[u]) continue;
+        if (prio[u] > prio[v] || (prio[u] == prio[v] && u < v)) blocked = 1;
+      }
+      take[v] = !blocked;
+    }
+    for (int64_t v = 0; v < n; v++)
+      if (take[v]) color[v] = c;
+  }
+  free(unc);
+  free(take);
+  return 0;
+}
+
+/* Window contact count per vertex (all neighbors), the PageRank divisor. */
+static int64_t wdeg(const orc* g, const orc_q* q, int64_t u) {
+  int64_t c = 0;
+  for (int64_t e = g->start[u]; e < g->start[u + 1]; e++) c += in_w(q, g->tim[e]);
+  return c;
+}
+
+/* Temporal PageRank (pagerank.cu:1453-1456, :1606-1660; BaselineRunner.java:433-469),
+ * race-free: new[v] = (1-d)/N; every active u with window degree du > 0 pushes
+ * d * rank[u] / du * (1 - exp(-0.1 t)) to each distinct active neighbor, t being
+ * that neighbor's first window contact; then new is divided by its sum.
+ * N = number of active vertices.  T is the storage and per-contact arithmetic
+ * type (float or double); sums (per-vertex accumulation, the normalising sum,
+ * the L1 change) are carried in double.  tol > 0 stops once the L1 change of
+ * the normalised vector drops below tol.  Returns the iterations run. */
+#define PR_BODY(T, EXP)                                                                       \
+  int64_t n = g->n, N = 0;                                                                    \
+  for (int64_t v = 0; v < n; v++) N += act(g, q, v);                                          \
+  if (N == 0) return fail("no active vertices", 0);                                           \
+  T* rank = (T*)calloc((size_t)n, sizeof(T));                                                 \
+  double* acc = (double*)calloc((size_t)n, sizeof(double));                                   \
+  int64_t* du = (int64_t*)calloc((size_t)n, sizeof(int64_t));                                 \
+  for (int64_t v = 0; v < n; v++) { rank[v] = act(g, q, v) ? (T)1 / (T)N : (T)0; du[v] = wdeg(g, q, v); } \
+  const T base = ((T)1 - (T)damping) / (T)N;                                                  \
+  int32_t it;                                                                                 \
+  for (it = 0; it < iters; it++) {                                                            \
+    for (int64_t v = 0; v < n; v++) acc[v] = act(g, q, v) ? (double)base : 0.0;               \
+    for (int64_t u = 0; u < n; u++) {                                                         \
+      if (!act(g, q, u) || du[u] == 0) continue;                                              \
+      const T coef = (T)damping * rank[u] / (T)du[u];                                         \
+      int32_t last = -1;                                                                      \
+      for (int64_t e = g->start[u]; e < g->start[u + 1]; e++) {                               \
+        int32_t r = g->adj[e];                                                                \
+        if (!in_w(q, g->tim[e]) || r == last) continue;                                       \
+        last = r;                                                                             \
+        if (!act(g, q, r)) continue;                                                          \
+        const T w = (T)1 - EXP((T)(-0.1) * (T)g->tim[e]);                                     \
+        acc[r] += (double)(coef * w);                                                         \
+      }                                                                                       \
+    }                                                                                         \
+    double s = 0;                                                                             \
+    for (int64_t v = 0; v < n; v++) s += acc[v];                                              \
+    double diff = 0;                                                                          \
+    for (int64_t v = 0; v < n; v++) {                                                         \
+      const T nv = (T)(s > 0 ? acc[v] / s : acc[v]);                                          \
+      diff += fabs((double)nv - (double)rank[v]);                                             \
+      rank[v] = nv;                                                                           \
+    }                                                                                         \
+    if (tol > 0 && diff < tol) { it++; break; }                                               \
+  }                                                                                           \
+  for (int64_t v = 0; v < n; v++) out[v] = rank[v];                                           \
+  free(rank); free(acc); free(du);                                                            \
+  return it;
+
+int orc_pagerank_f64(orc* g, const orc_q* q, int32_t iters, double damping, double tol, double* out) {
+  PR_BODY(double, exp)
+}
+int orc_pagerank_f32(orc* g, const orc_q* q, int32_t iters, double damping, double tol, float* out) {
+  PR_BODY(float, expf)
+}
+
+/* HITS with per-iteration reset and L2 normalisation (BaselineRunner.java:472-550),
+ * over distinct active neighbors with at least one window contact.  T is the
+ * storage type; sums and norms are carried in double. */
+#define HITS_BODY(T)                                                                          \
+  int64_t n = g->n;                                                                           \
+  double* acc = (double*)calloc((size_t)n, sizeof(double));                                   \
+  for (int64_t v = 0; v < n; v++) { hub[v] = act(g, q, v) ? 1 : 0; auth[v] = hub[v]; }        \
+  for (int32_t it = 0; it < iters; it++) {                                                    \
+    for (int64_t u = 0; u < n; u++) {                                                         \
+      acc[u] = 0;                                                                             \
+      if (!act(g, q, u)) continue;                                                            \
+      int32_t last = -1;                                                                      \
+      for (int64_t e = g->start[u]; e < g->start[u + 1]; e++) {                               \
+        int32_t r = g->adj[e];                                                                \
+        if (!in_w(q, g->tim[e]) || r == last) continue;                                       \
+        last = r;                                                                             \
+        if (act(g, q, r)) acc[u] += (double)auth[r];                                          \
+      }                                                                                       \
+    }                                                                                         \
+    double norm = 0;                                                                          \
+    for (int64_t v = 0; v < n; v++) norm += acc[v] * acc[v];                                  \
+    norm = sqrt(norm);                                                                        \
+    for (int64_t v = 0; v < n; v++) hub[v] = (T)(norm != 0 ? acc[v] / norm : acc[v]);         \
+    for (int64_t v = 0; v < n; v++) acc[v] = 0;                                               \
+    for (int64_t u = 0; u < n; u++) {                                                         \
+      if (!act(g, q, u)) continue;                                                            \
+      int32_t last = -1;                                                                      \
+      for (int64_t e = g->start[u]; e < g->start[u + 1]; e++) {                               \
+        int32_t r = g->adj[e];                                                                \
+        if (!in_w(q, g->tim[e]) || r == last) continue;                                       \
+        last = r;                                                                             \
+        if (act(g, q, r)) acc[r] += (double)hub[u];                                           \
+      }                                                                                       \
+    }                                                                                         \
+    norm = 0;                                                                                 \
+    for (int64_t v = 0; v < n; v++) norm += acc[v] * acc[v];                                  \
+    norm = sqrt(norm);                                                                        \
+    for (int64_t v = 0; v < n; v++) auth[v] = (T)(norm != 0 ? acc[v] / norm : acc[v]);        \
+  }                                                                                           \
+  free(acc);                                                                                  \
+  return 0;
+
+int orc_hits_f64(orc* g, const orc_q* q, int32_t iters, double* hub, double* auth) { HITS_BODY(double) }
+int orc_hits_f32(orc* g, const orc_q* q, int32_t iters, float* hub, float* auth) { HITS_BODY(float) }
