@@ -1,5 +1,5 @@
 // Device decode core for gfx950: bit peeks, instantaneous codes, Elias-Fano
-// offsets and the successor-list state machine.
+// offsets, the successor-list state machine and decode checkpoints.
 //
 // HBM layout (built once by the loader, files on disk stay unchanged):
 //  * .graph and .timestamps are stored as 64-bit words with their bytes
@@ -17,7 +17,10 @@
 // Codes are decoded arithmetically from a 64-bit window with a count of
 // leading zeros instead of the reference's 16-bit lookup tables
 // (bfs.cu:202-256, :590-639); every codeword this format produces is at most
-// 51 bits long (zeta_2 of an int2nat'ed int32 difference).
+// 51 bits long (zeta_2 of an int2nat'ed int32 difference).  The two hot
+// sub-streams (timestamps, and the residuals of the top-level list) are read
+// through a register bit buffer that loads one 64-bit word per 64 bits
+// consumed; the rarer sub-streams use two-word peeks.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,7 +36,7 @@ constexpr uint32_t kSpanBits = 1024;  // select blocks wider than this are store
 enum : int32_t {
   kErrOverrun = 1,      // a cursor ran past the end of a stream
   kErrNeighbor = 2,     // decoded neighbor outside [0, n)
-  kErrDepth = 4,        // reference chain deeper than kMaxDepth or reference outside the window
+  kErrDepth = 4,        // reference chain deeper than the decoder or reference outside the window
   kErrExhausted = 8,    // record yields fewer successors than its outdegree
   kErrCode = 16,        // codeword longer than the format allows
   kErrBlocks = 32,      // copy blocks inconsistent with the referenced list
@@ -49,8 +52,8 @@ struct EFDir {
 };
 
 struct Streams {
-  const uint64_t* g;  // graph stream, byte-swapped words
-  const uint64_t* t;  // timestamp stream, byte-swapped words
+  const uint64_t* g;  // graph stream, byte-swapped words (+2 zero words)
+  const uint64_t* t;  // timestamp stream, byte-swapped words (+2 zero words)
   uint64_t g_bits, t_bits;
   EFDir gi, ti;
   int32_t n;       // vertices; ids are [0, n)
@@ -60,8 +63,7 @@ struct Streams {
 
 __device__ __forceinline__ void raise_err(const Streams& S, int32_t bit) { atomicOr(S.err, bit); }
 
-// 64 stream bits starting at bit `pos` (MSB-first).  `pos` must be < bits;
-// the word arrays carry two words of zero padding.
+// 64 stream bits starting at bit `pos` (MSB-first).
 __device__ __forceinline__ uint64_t peek64(const uint64_t* w, uint64_t pos) {
   const uint64_t i = pos >> 6;
   const uint32_t s = (uint32_t)pos & 63u;
@@ -72,8 +74,51 @@ __device__ __forceinline__ uint64_t peek64(const uint64_t* w, uint64_t pos) {
 
 __device__ __forceinline__ int64_t nat2int(uint64_t v) { return (v & 1) ? -(int64_t)(v >> 1) - 1 : (int64_t)(v >> 1); }
 
-// Cursor over one stream with a bound check: a malformed stream can never
-// make a load leave the padded buffer.
+// Codeword decoders on a 64-bit window `p`; `len` receives the codeword
+// length.  A window too short for the code (corrupt stream) flags kErrCode
+// and yields 0 with length 1.
+__device__ __forceinline__ uint64_t dec_gamma(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t z = (uint32_t)__clzll(p);
+  if (z > 31) {
+    raise_err(S, kErrCode);
+    len = 1;
+    return 0;
+  }
+  len = 2 * z + 1;
+  return ((p << z) >> (63 - z)) - 1;
+}
+
+__device__ __forceinline__ uint32_t dec_unary(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t z = (uint32_t)__clzll(p);
+  if (z > 62) {
+    raise_err(S, kErrCode);
+    len = 1;
+    return 0;
+  }
+  len = z + 1;
+  return z;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t dec_zeta(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t h = (uint32_t)__clzll(p);
+  const uint32_t nb = h * K + K - 1;  // bits of the short form
+  if (h + nb + 2 > 64) {
+    raise_err(S, kErrCode);
+    len = 1;
+    return 0;
+  }
+  const uint64_t left = 1ull << (h * K);
+  const uint64_t m = (p << (h + 1)) >> (64 - nb);
+  if (m < left) {
+    len = h + 1 + nb;
+    return m + left - 1;
+  }
+  len = h + 2 + nb;
+  return (m << 1) + ((p << (h + 1 + nb)) >> 63) - 1;
+}
+
+// Two-word-peek cursor reads (bound checked against the stream length).
 struct Rd {
   const uint64_t* w;
   uint64_t bits;
@@ -88,48 +133,64 @@ __device__ __forceinline__ uint64_t safe_peek(const Streams& S, Rd r, uint64_t p
 }
 
 __device__ __forceinline__ uint32_t rd_unary(const Streams& S, Rd r, uint64_t& pos) {
-  const uint64_t p = safe_peek(S, r, pos);
-  const uint32_t z = (uint32_t)__clzll(p);
-  if (z > 62) {
-    raise_err(S, kErrCode);
-    pos += 1;
-    return 0;
-  }
-  pos += z + 1;
-  return z;
+  uint32_t len;
+  const uint32_t v = dec_unary(S, safe_peek(S, r, pos), len);
+  pos += len;
+  return v;
 }
 
 __device__ __forceinline__ uint64_t rd_gamma(const Streams& S, Rd r, uint64_t& pos) {
-  const uint64_t p = safe_peek(S, r, pos);
-  const uint32_t z = (uint32_t)__clzll(p);
-  if (z > 31) {
-    raise_err(S, kErrCode);
-    pos += 1;
-    return 0;
-  }
-  pos += 2 * z + 1;
-  return ((p << z) >> (63 - z)) - 1;
+  uint32_t len;
+  const uint64_t v = dec_gamma(S, safe_peek(S, r, pos), len);
+  pos += len;
+  return v;
 }
 
 template <int K>
 __device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& pos) {
-  const uint64_t p = safe_peek(S, r, pos);
-  const uint32_t h = (uint32_t)__clzll(p);
-  const uint32_t nb = h * K + K - 1;  // bits of the short form
-  if (h + nb + 2 > 64) {
-    raise_err(S, kErrCode);
-    pos += 1;
-    return 0;
+  uint32_t len;
+  const uint64_t v = dec_zeta<K>(S, safe_peek(S, r, pos), len);
+  pos += len;
+  return v;
+}
+
+// Register bit buffer over one stream: words a = w[idx], b = w[idx + 1] are
+// held, `sh` is the bit offset into a.  Advancing past a loads one word.
+struct BitBuf {
+  const uint64_t* w;
+  uint64_t idx;
+  uint64_t a, b;
+  uint32_t sh;
+  uint32_t last;  // index of the last readable word (the padding word)
+
+  __device__ __forceinline__ void init(const uint64_t* words, uint64_t nbits, uint64_t pos) {
+    w = words;
+    last = (uint32_t)min((nbits >> 6) + 1, (uint64_t)0xffffffffu);
+    idx = pos >> 6;
+    if (idx >= last) idx = last - 1;
+    sh = (uint32_t)pos & 63u;
+    a = w[idx];
+    b = w[idx + 1];
   }
-  const uint64_t left = 1ull << (h * K);
-  const uint64_t m = (p << (h + 1)) >> (64 - nb);
-  if (m < left) {
-    pos += h + 1 + nb;
-    return m + left - 1;
+  __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
+  __device__ __forceinline__ void skip(uint32_t n) {  // n <= 64
+    sh += n;
+    if (sh >= 64) {
+      sh -= 64;
+      a = b;
+      ++idx;
+      b = (idx < last) ? w[idx + 1] : 0;
+    }
   }
-  const uint64_t bit = (p << (h + 1 + nb)) >> 63;
-  pos += h + 2 + nb;
-  return (m << 1) + bit - 1;
+  __device__ __forceinline__ uint64_t pos() const { return idx * 64 + sh; }
+};
+
+template <int K>
+__device__ __forceinline__ uint64_t bb_zeta(const Streams& S, BitBuf& b) {
+  uint32_t len;
+  const uint64_t v = dec_zeta<K>(S, b.peek(), len);
+  b.skip(len);
+  return v;
 }
 
 // Position of the r-th (0-based) set bit of w (r < popcount(w)).
@@ -187,30 +248,46 @@ __device__ __forceinline__ uint64_t ef_get(const Streams& S, const EFDir& d, uin
 // implicit last block), multiples runs (value repeated len times), intervals
 // (consecutive values) and zeta_3 residuals.  Each source keeps its head; the
 // merge returns the smallest head.  The copy source of level L pulls values
-// from level L+1, which decodes the referenced vertex the same way, so one
-// decoder holds up to kMaxDepth + 1 levels.  A valid encoding never yields
-// the same value from two sources (multiples are pulled out before copying,
-// BVMultiGraph.java:1644-1647), so heads are distinct unless they are kInf.
+// from level L+1, which decodes the referenced vertex the same way, so a
+// decoder for chains of depth MAXD holds MAXD + 1 levels.  MAXD = 0 is the
+// flat decoder for records without a reference.  A valid encoding never
+// yields the same value from two sources (multiples are pulled out before
+// copying, BVMultiGraph.java:1644-1647), so heads are distinct unless kInf.
 struct Level {
-  uint64_t mpos, ipos, rpos, bpos;  // cursors of the four sub-streams
-  int32_t x;                         // vertex whose list this level yields
-  int32_t hm, mrem, mleft;           // multiples: head, copies left of head, runs not read
-  int32_t hi, irem, ileft;           // intervals: head, values left incl. head, intervals not read
-  int32_t hr, rleft;                 // residuals: head, residuals not read
-  int32_t hc, cleft, bleft, bidx, brem;  // copies: head, copies not fetched, blocks not read, block idx, left in block
+  uint64_t mpos, ipos, rpos, bpos;       // cursors of the four sub-streams
+  int32_t x;                              // vertex whose list this level yields
+  int32_t hm, mrem, mleft;                // multiples: head, copies left of head, runs not read
+  int32_t hi, irem, ileft;                // intervals: head, values left incl. head, intervals not read
+  int32_t hr, rleft;                      // residuals: head, residuals not read
+  int32_t hc, cleft, bleft, bidx, brem;   // copies: head, copies left, blocks not read, block index, left in block
+  int32_t pad_[2];
 };
 
-struct Decoder {
+template <int MAXD>
+struct Dec {
+  Level lv[MAXD + 1];
+  BitBuf rb;  // residual cursor of level 0 (lv[0].rpos is stale while decoding)
+};
+
+// Saved decoder state at a chunk boundary of a long list (see engine.hip).
+struct Ckpt {
+  int32_t x;      // vertex
+  int32_t k0;     // first contact of the chunk
+  int32_t k1;     // one past its last contact
+  int32_t depth;  // levels in use - 1
+  uint64_t tpos;  // timestamp cursor
+  int32_t tprev;  // timestamp before contact k0
+  int32_t nchunk; // chunks of x (stored in every chunk of x)
   Level lv[kMaxDepth + 1];
 };
 
-template <int L>
-__device__ __forceinline__ int32_t dec_next(const Streams& S, Decoder& D);
+template <int L, int MAXD>
+__device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D);
 
 // Pulls the next value of level L's copy source from level L+1.
-template <int L>
-__device__ __forceinline__ int32_t dec_masked(const Streams& S, Decoder& D) {
-  if constexpr (L >= kMaxDepth) {
+template <int L, int MAXD>
+__device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
+  if constexpr (L >= MAXD) {
     return kInf;
   } else {
     Level& v = D.lv[L];
@@ -231,7 +308,7 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Decoder& D) {
         }
         continue;
       }
-      const int32_t val = dec_next<L + 1>(S, D);
+      const int32_t val = dec_next<L + 1, MAXD>(S, D);
       if (val == kInf) {
         raise_err(S, kErrBlocks);
         v.cleft = 0;
@@ -246,13 +323,26 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Decoder& D) {
   }
 }
 
-template <int L>
-__device__ __forceinline__ int32_t dec_next(const Streams& S, Decoder& D) {
+template <int L, int MAXD>
+__device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   Level& v = D.lv[L];
   const Rd g{S.g, S.g_bits};
   const int32_t m = min(min(v.hm, v.hi), min(v.hr, v.hc));
   if (m == kInf) return kInf;
-  if (m == v.hm) {
+  if (m == v.hr) {
+    if (v.rleft > 0) {
+      uint64_t z;
+      if constexpr (L == 0) {
+        z = bb_zeta<3>(S, D.rb);
+      } else {
+        z = rd_zeta<3>(S, g, v.rpos);
+      }
+      v.hr += (int32_t)z + 1;
+      --v.rleft;
+    } else {
+      v.hr = kInf;
+    }
+  } else if (m == v.hm) {
     if (--v.mrem == 0) {
       if (v.mleft > 0) {
         v.hm += (int32_t)rd_gamma(S, g, v.mpos) + 1;
@@ -274,23 +364,25 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Decoder& D) {
     } else {
       ++v.hi;
     }
-  } else if (m == v.hr) {
-    if (v.rleft > 0) {
-      v.hr += (int32_t)rd_zeta<3>(S, g, v.rpos) + 1;
-      --v.rleft;
-    } else {
-      v.hr = kInf;
-    }
   } else {
-    v.hc = dec_masked<L>(S, D);
+    v.hc = dec_masked<L, MAXD>(S, D);
   }
   return m;
 }
 
-// Parses the record of x into level L (and the levels of its reference
-// chain).  Returns the outdegree d.  `gpos` is the record's bit offset.
-template <int L>
-__device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_t x, uint64_t gpos) {
+// Next successor without consuming it.
+template <int MAXD>
+__device__ __forceinline__ int32_t dec_peek(const Dec<MAXD>& D) {
+  const Level& v = D.lv[0];
+  return min(min(v.hm, v.hi), min(v.hr, v.hc));
+}
+
+// Parses the record of x at bit offset `gpos` into level L (and the levels of
+// its reference chain).  Returns the outdegree d; `depth` receives the
+// deepest level used.
+template <int L, int MAXD>
+__device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos,
+                                            int32_t& depth) {
   Level& v = D.lv[L];
   const Rd g{S.g, S.g_bits};
   v.x = x;
@@ -301,11 +393,12 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_
   v.mpos = v.ipos = v.rpos = v.bpos = gpos;
   uint64_t pos = gpos;
   const int32_t d = (int32_t)rd_gamma(S, g, pos);
-  if (d == 0) return 0;
+  if (d <= 0) return 0;
+  if (L > depth) depth = L;
   const uint32_t r = rd_unary(S, g, pos);
   int32_t copied = 0;
   if (r > 0) {
-    if constexpr (L >= kMaxDepth) {
+    if constexpr (L >= MAXD) {
       raise_err(S, kErrDepth);
     } else {
       if (r > (uint32_t)kWindow || (int32_t)r > x) {
@@ -324,7 +417,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_
           if ((i & 1) == 0) even += b;
         }
         const int32_t y = x - (int32_t)r;
-        const int32_t dref = dec_init<L + 1>(S, D, y, ef_get(S, S.gi, (uint64_t)y));
+        const int32_t dref = dec_init<L + 1, MAXD>(S, D, y, ef_get(S, S.gi, (uint64_t)y), depth);
         copied = even + ((bc & 1) ? 0 : dref - total);
         if (copied < 0 || copied > d) {
           raise_err(S, kErrBlocks);
@@ -332,7 +425,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_
         }
         v.cleft = copied;
         v.bleft = bc;
-        v.hc = dec_masked<L>(S, D);
+        v.hc = dec_masked<L, MAXD>(S, D);
       }
     }
   }
@@ -378,33 +471,53 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Decoder& D, int32_
     v.hr = (int32_t)(nat2int(rd_zeta<3>(S, g, v.rpos)) + x);
     v.rleft = extra - 1;
   }
+  if constexpr (L == 0) D.rb.init(S.g, S.g_bits, v.rpos);
   return d;
 }
 
 // Timestamp cursor: zeta_2 of int2nat deltas, reset to the header value per
 // vertex (EvolvingMultiGraph.java:131-144; bfs.cu:1444, :1597).
 struct TsCursor {
-  uint64_t pos;
+  BitBuf b;
   int32_t prev;
 };
 
 __device__ __forceinline__ int32_t ts_next(const Streams& S, TsCursor& c) {
-  const Rd t{S.t, S.t_bits};
-  c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(rd_zeta<2>(S, t, c.pos)));
+  c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(bb_zeta<2>(S, c.b)));
   return c.prev;
 }
 
-// Decodes every contact of x in successor order and hands (x, r, t) to
-// act.contact(); a false return stops the walk early.  act.begin(x, d) may
-// veto the vertex.
-template <class Act>
-__device__ __forceinline__ void visit_vertex(const Streams& S, int32_t x, Act& act) {
-  Decoder D;
-  const int32_t d = dec_init<0>(S, D, x, ef_get(S, S.gi, (uint64_t)x));
-  if (!act.begin(x, d) || d == 0) return;
-  TsCursor tc{ef_get(S, S.ti, (uint64_t)x), S.min_ts};
-  for (int32_t i = 0; i < d; ++i) {
-    const int32_t r = dec_next<0>(S, D);
+// Saves the decoder at a chunk boundary.
+template <int MAXD>
+__device__ __forceinline__ void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t depth, const TsCursor& tc, int32_t x,
+                                          int32_t k0) {
+  c->x = x;
+  c->k0 = k0;
+  c->depth = depth;
+  c->tpos = tc.b.pos();
+  c->tprev = tc.prev;
+#pragma unroll
+  for (int l = 0; l <= MAXD; ++l) c->lv[l] = D.lv[l];
+  c->lv[0].rpos = D.rb.pos();
+}
+
+template <int MAXD>
+__device__ __forceinline__ void ckpt_load(const Streams& S, const Ckpt* c, Dec<MAXD>& D, TsCursor& tc) {
+#pragma unroll
+  for (int l = 0; l <= MAXD; ++l)
+    if (l <= c->depth) D.lv[l] = c->lv[l];
+  D.rb.init(S.g, S.g_bits, D.lv[0].rpos);
+  tc.b.init(S.t, S.t_bits, c->tpos);
+  tc.prev = c->tprev;
+}
+
+// Walks contacts [k, kend) of one work item (a whole record or a chunk) and
+// hands (x, r, t) to act.contact(); a false return stops the walk.
+template <int MAXD, class Act>
+__device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& tc, int32_t x, int32_t k, int32_t kend,
+                                     Act& act) {
+  for (; k < kend; ++k) {
+    const int32_t r = dec_next<0, MAXD>(S, D);
     const int32_t t = ts_next(S, tc);
     if ((uint32_t)r >= (uint32_t)S.n) {
       raise_err(S, r == kInf ? kErrExhausted : kErrNeighbor);
@@ -412,6 +525,38 @@ __device__ __forceinline__ void visit_vertex(const Streams& S, int32_t x, Act& a
     }
     if (!act.contact(x, r, t)) break;
   }
+}
+
+// Runs one work item.  code >= 0: the whole record of vertex `code`;
+// code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked)
+// may veto the item; act.end(x, chunked) runs after the walk.
+template <int MAXD, class Act>
+__device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act) {
+  Dec<MAXD> D;
+  TsCursor tc;
+  int32_t x, k = 0, kend;
+  const bool chunked = code < 0;
+  if (!chunked) {
+    x = code;
+    if (!act.begin(x, false)) return;
+    int32_t depth = 0;
+    kend = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
+    if (kend == 0) {
+      act.end(x, false);
+      return;
+    }
+    tc.b.init(S.t, S.t_bits, ef_get(S, S.ti, (uint64_t)x));
+    tc.prev = S.min_ts;
+  } else {
+    const Ckpt* c = ck + (-code - 1);
+    x = c->x;
+    if (!act.begin(x, true)) return;
+    k = c->k0;
+    kend = c->k1;
+    ckpt_load<MAXD>(S, c, D, tc);
+  }
+  walk<MAXD>(S, D, tc, x, k, kend, act);
+  act.end(x, chunked);
 }
 
 }  // namespace tga

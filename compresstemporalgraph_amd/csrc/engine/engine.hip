@@ -1,12 +1,24 @@
-// MI355X engine: loader, decode directories, algorithm kernels and the C ABI
-// of include/tga.h.
+// MI355X engine: loader, decode schedule, algorithm kernels and the C ABI of
+// include/tga.h.
 //
-// Every algorithm is one or more passes of `visit_vertex` (decode.hpp) over a
-// set of source vertices, with the per-contact rule of the algorithm plugged
-// in as a small functor.  Frontier algorithms (BFS, CC, GC, earliest arrival)
-// keep their frontier as a device queue; the host reads one 4-byte count per
-// level to size the next launch (the reference copies the queue size back per
-// level too, bfs.cu:2028).
+// Work decomposition.  Every algorithm is one or more passes over "work
+// items".  An item is either a whole successor record (small vertices) or a
+// chunk of a long record that starts from a saved decoder state (Ckpt,
+// decode.hpp).  At open the loader
+//   1. reads every record header on the GPU (outdegree d, reference r),
+//   2. derives each record's reference-chain depth on the host,
+//   3. decodes every record with d > kChunk once, sequentially, saving the
+//      decoder state every kChunk contacts at the next change of neighbor
+//      (so a neighbor's contacts never straddle two chunks), and
+//   4. builds two item lists, one for flat records (no reference: a
+//      one-level decoder with few registers) and one for records with a
+//      reference chain (a four-level decoder), each ordered by decreasing
+//      cost so waves get items of similar length.
+// Full passes (RA, DC, PR, HITS) launch over the two lists.  Frontier
+// algorithms (BFS, CC, GC, earliest arrival) expand their vertex frontier into
+// flat/chain item queues each round; the host reads a few 4-byte counts per
+// round to size the next launches (the reference copies the queue size back
+// per level too, bfs.cu:2028).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -31,14 +44,18 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(call)                                                                 \
-  do {                                                                                \
-    hipError_t e_ = (call);                                                           \
-    if (e_ != hipSuccess)                                                             \
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
       return fail(TGA_EDEVICE, std::string(#call " failed: ") + hipGetErrorString(e_)); \
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int32_t kChunk = 256;  // contacts per checkpointed chunk
+
+// vinfo[x] codes for vertices whose record is not chunked.
+constexpr int32_t kSmallFlat = -1, kSmallChain = -2, kEmpty = -3;
 
 struct Window {
   int64_t ws, we;
@@ -67,75 +84,131 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Index i of a vertex set that is either an explicit list or a range.
-__device__ __forceinline__ int32_t item(const int32_t* list, int32_t base, int64_t i) {
-  return list ? list[i] : base + (int32_t)i;
-}
-
 // Appends v to a device queue.
 __device__ __forceinline__ void push(int32_t* q, uint32_t* qn, int32_t v) { q[atomicAdd(qn, 1u)] = v; }
 
-// ---------------------------------------------------------------- kernels ---
+// An item list and its length.
+struct Items {
+  const int32_t* code;
+  uint32_t count;
+};
 
-// Outdegree of every vertex from its record header; validates the index.
-__global__ void __launch_bounds__(kBlock) k_outdeg(Streams S, unsigned long long* total, unsigned long long* maxd) {
+#define FOR_ITEMS(I, i) \
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (I).count; i += gridDim.x * blockDim.x)
+
+// ------------------------------------------------------- schedule kernels ---
+
+// Record header of every vertex: outdegree and reference distance.
+__global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int8_t* ref) {
   const Rd g{S.g, S.g_bits};
-  unsigned long long sum = 0, mx = 0;
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < S.n; x += (int64_t)gridDim.x * blockDim.x) {
     uint64_t pos = ef_get(S, S.gi, (uint64_t)x);
-    const uint64_t d = rd_gamma(S, g, pos);
-    ef_get(S, S.ti, (uint64_t)x);
-    sum += d;
-    mx = d > mx ? d : mx;
-  }
-  sum = wave_sum(sum);
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(mx, off, 64);
-    mx = o > mx ? o : mx;
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(total, sum);
-    atomicMax(maxd, mx);
+    const int32_t d = (int32_t)rd_gamma(S, g, pos);
+    int32_t r = 0;
+    if (d > 0) r = (int32_t)rd_unary(S, g, pos);
+    ef_get(S, S.ti, (uint64_t)x);  // validates the timestamp index entry
+    deg[x] = d;
+    ref[x] = (int8_t)min(r, 127);
   }
 }
 
-struct RaAct {
-  uint64_t cs = 0;
-  uint64_t cnt = 0;
-  __device__ bool begin(int32_t, int32_t d) {
-    cnt += (uint64_t)d;
-    return true;
+// Decodes each long record once and saves a checkpoint every kChunk contacts,
+// moved forward to the next change of neighbor.
+template <int MAXD>
+__global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* bigv, const int64_t* slot, int32_t nbig,
+                                                   Ckpt* ck, int32_t* nchunk_out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbig) return;
+  const int32_t x = bigv[i];
+  Ckpt* out = ck + slot[i];
+  Dec<MAXD> D;
+  TsCursor tc;
+  int32_t depth = 0;
+  const int32_t d = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
+  tc.b.init(S.t, S.t_bits, ef_get(S, S.ti, (uint64_t)x));
+  tc.prev = S.min_ts;
+  ckpt_save<MAXD>(out, D, depth, tc, x, 0);
+  int32_t nch = 1, target = kChunk, prev = -1;
+  for (int32_t k = 0; k < d; ++k) {
+    if (k >= target) {
+      const int32_t nr = dec_peek<MAXD>(D);
+      if (nr != prev) {
+        out[nch - 1].k1 = k;
+        ckpt_save<MAXD>(out + nch, D, depth, tc, x, k);
+        ++nch;
+        target = k + kChunk;
+      }
+    }
+    prev = dec_next<0, MAXD>(S, D);
+    ts_next(S, tc);
   }
+  out[nch - 1].k1 = d;
+  for (int32_t j = 0; j < nch; ++j) out[j].nchunk = nch;
+  nchunk_out[i] = nch;
+}
+
+// Expands a vertex set (an explicit list, or the range [base, base + count))
+// into flat and chain item queues.  keep_empty keeps records without contacts
+// (graph coloring colors them).  Long records also go to bigq when given.
+__global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t base, uint32_t count,
+                                                   const int32_t* vinfo, const Ckpt* ck, int32_t* flatq,
+                                                   int32_t* chainq, uint32_t* cnt, int32_t* bigq, int keep_empty) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const int32_t x = front ? front[i] : base + (int32_t)i;
+    const int32_t info = vinfo[x];
+    if (info == kSmallFlat || (info == kEmpty && keep_empty)) {
+      push(flatq, &cnt[0], x);
+    } else if (info == kSmallChain) {
+      push(chainq, &cnt[1], x);
+    } else if (info >= 0) {
+      const Ckpt* c = ck + info;
+      const int32_t nch = c->nchunk;
+      const bool chain = c->depth > 0;
+      const uint32_t at = atomicAdd(&cnt[chain ? 1 : 0], (uint32_t)nch);
+      int32_t* q = chain ? chainq : flatq;
+      for (int32_t j = 0; j < nch; ++j) q[at + j] = -(info + j) - 1;
+      if (bigq) push(bigq, &cnt[2], x);
+    }
+  }
+}
+
+// -------------------------------------------------------------- actions ---
+
+struct RaAct {
+  uint64_t cs = 0, cnt = 0;
+  __device__ bool begin(int32_t, bool) { return true; }
   __device__ bool contact(int32_t x, int32_t r, int32_t t) {
     cs += contact_hash(x, r, t);
+    ++cnt;
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_ra(Streams S, const int32_t* list, int32_t base, int64_t count,
-                                               unsigned long long* out) {
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items I, unsigned long long* out) {
   RaAct a;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t x = item(list, base, i);
-    if ((uint32_t)x >= (uint32_t)S.n) {
-      raise_err(S, kErrNeighbor);
-      continue;
-    }
-    visit_vertex(S, x, a);
-  }
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
   const unsigned long long cs = wave_sum((unsigned long long)a.cs);
   const unsigned long long cnt = wave_sum((unsigned long long)a.cnt);
-  if ((threadIdx.x & 63) == 0) {
+  if ((threadIdx.x & 63) == 0 && cnt) {
     atomicAdd(&out[0], cs);
     atomicAdd(&out[1], cnt);
   }
 }
 
+// Degree count (and, for whole records, the centrality formula).
 struct DcAct {
   Window W;
+  int32_t* count;
+  int32_t* centr;
   int32_t deg = 0;
   int64_t avg = 0;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) {
+    deg = 0;
+    avg = 0;
+    return W.active(x);
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (W.in(t) && W.active(r)) {
       ++deg;
@@ -143,9 +216,12 @@ struct DcAct {
     }
     return true;
   }
+  __device__ void end(int32_t x, bool chunked);
 };
 
 #pragma clang fp contract(off)
+// Java's (int)(0.85 * c + 0.15 * c * (mean - ws) / (we - ws) + 1), evaluated
+// left to right in double without fused multiply-adds (BaselineRunner.java:325-331).
 __device__ int32_t dc_formula(int32_t deg, int64_t avg, int64_t ws, int64_t we) {
   if (ws == we) return deg;
   const double c = 0.85 * (double)deg + 0.15 * (double)deg * (double)(avg - ws) / (double)(we - ws) + 1.0;
@@ -153,13 +229,35 @@ __device__ int32_t dc_formula(int32_t deg, int64_t avg, int64_t ws, int64_t we) 
 }
 #pragma clang fp contract(on)
 
-__global__ void __launch_bounds__(kBlock) k_dc(Streams S, Window W, int32_t* count, int32_t* centr) {
-  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < S.n; x += (int64_t)gridDim.x * blockDim.x) {
-    DcAct a{W};
-    if (W.active((int32_t)x)) visit_vertex(S, (int32_t)x, a);
-    count[x] = a.deg;
-    if (centr) centr[x] = dc_formula(a.deg, a.avg, W.ws, W.we);
+// Writes only the outputs it was given (k_dc_long passes none and reads
+// deg / avg itself).
+__device__ void DcAct::end(int32_t x, bool chunked) {
+  if (!count) return;
+  if (chunked) {
+    if (deg) atomicAdd(&count[x], deg);
+  } else {
+    count[x] = deg;
+    if (centr) centr[x] = dc_formula(deg, avg, W.ws, W.we);
   }
+}
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_dc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* count,
+                                               int32_t* centr) {
+  DcAct a{W, count, centr};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// The running mean of long records is sequential: one thread per record.
+template <int MAXD>
+__global__ void __launch_bounds__(64) k_dc_long(Streams S, const int32_t* bigv, int32_t nbig, Window W,
+                                                int32_t* centr) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbig) return;
+  DcAct a{W, nullptr, nullptr};
+  const int32_t x = bigv[i];
+  run_item<MAXD>(S, nullptr, x, a);
+  centr[x] = W.active(x) ? dc_formula(a.deg, a.avg, W.ws, W.we) : dc_formula(0, 0, W.ws, W.we);
 }
 
 struct BfsAct {
@@ -168,34 +266,36 @@ struct BfsAct {
   int32_t* next;
   uint32_t* nsize;
   int32_t lvl;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) { return W.active(x); }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (W.in(t) && W.active(r) && level[r] == kInf && atomicCAS(&level[r], kInf, lvl + 1) == kInf)
       push(next, nsize, r);
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_bfs(Streams S, Window W, const int32_t* front, int64_t fsize,
-                                                int32_t* level, int32_t* next, uint32_t* nsize, int32_t lvl) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < fsize; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t x = front[i];
-    if (!W.active(x)) continue;
-    BfsAct a{W, level, next, nsize, lvl};
-    visit_vertex(S, x, a);
-  }
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_bfs(Streams S, const Ckpt* ck, Items I, Window W, int32_t* level,
+                                                int32_t* next, uint32_t* nsize, int32_t lvl) {
+  BfsAct a{W, level, next, nsize, lvl};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 // Min-label propagation: a vertex whose label drops is queued once per round.
 struct CcAct {
   Window W;
-  int32_t lx;
   int32_t* label;
   int32_t* stamp;
   int32_t round;
   int32_t* next;
   uint32_t* nsize;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  int32_t lx = 0;
+  __device__ bool begin(int32_t x, bool) {
+    if (!W.active(x)) return false;
+    lx = label[x];
+    return true;
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (W.in(t) && W.active(r) && lx < label[r]) {
       const int32_t old = atomicMin(&label[r], lx);
@@ -203,67 +303,97 @@ struct CcAct {
     }
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_cc(Streams S, Window W, const int32_t* front, int32_t base, int64_t fsize,
-                                               int32_t* label, int32_t* stamp, int32_t round, int32_t* next,
-                                               uint32_t* nsize) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < fsize; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t x = item(front, base, i);
-    if (!W.active(x)) continue;
-    CcAct a{W, label[x], label, stamp, round, next, nsize};
-    visit_vertex(S, x, a);
-  }
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* label,
+                                               int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize) {
+  CcAct a{W, label, stamp, round, next, nsize};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
 // uncolored at the start of the round (its color is kInf, or `round` if it was
-// colored earlier in this same round).
+// colored earlier in this same round).  Chunks of a long record report to
+// blocked[x]; k_gc_finish decides those vertices.
 struct GcAct {
   Window W;
   const int32_t* prio;
-  const int32_t* color;
-  int32_t v, pv, round;
-  bool blocked = false;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  int32_t* color;
+  int32_t* blocked;
+  int32_t round;
+  int32_t* next;
+  uint32_t* nsize;
+  int32_t v = 0, pv = 0;
+  bool blk = false;
+  __device__ bool begin(int32_t x, bool chunked) {
+    if (!W.active(x) || color[x] != kInf) return false;
+    if (chunked && blocked[x]) return false;  // another chunk already found a blocker
+    v = x;
+    pv = prio[x];
+    blk = false;
+    return true;
+  }
   __device__ bool contact(int32_t, int32_t u, int32_t t) {
     if (W.in(t) && W.active(u)) {
       const int32_t pu = prio[u];
       if ((pu > pv || (pu == pv && u < v)) && color[u] >= round) {
-        blocked = true;
+        blk = true;
         return false;
       }
     }
     return true;
   }
+  __device__ void end(int32_t x, bool chunked) {
+    if (chunked) {
+      if (blk) atomicOr(&blocked[x], 1);
+    } else if (!blk) {
+      color[x] = round;
+    } else {
+      push(next, nsize, x);
+    }
+  }
 };
 
-__global__ void __launch_bounds__(kBlock) k_gc(Streams S, Window W, const int32_t* work, int32_t base, int64_t wsize,
-                                               const int32_t* prio, int32_t* color, int32_t round, int32_t* next,
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_gc(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
+                                               int32_t* color, int32_t* blocked, int32_t round, int32_t* next,
                                                uint32_t* nsize) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < wsize; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t v = item(work, base, i);
-    if (!W.active(v) || color[v] != kInf) continue;
-    GcAct a{W, prio, color, v, prio[v], round};
-    visit_vertex(S, v, a);
-    if (!a.blocked)
-      color[v] = round;
-    else
-      push(next, nsize, v);
+  GcAct a{W, prio, color, blocked, round, next, nsize};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+__global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint32_t nbig, Window W, int32_t* color,
+                                                      int32_t* blocked, int32_t round, int32_t* next,
+                                                      uint32_t* nsize) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += gridDim.x * blockDim.x) {
+    const int32_t x = bigq[i];
+    if (!W.active(x) || color[x] != kInf) continue;
+    if (blocked[x]) {
+      blocked[x] = 0;
+      push(next, nsize, x);
+    } else {
+      color[x] = round;
+    }
   }
 }
 
 // Earliest arrival (bfs.cu:1452-1471 rule): first window contact per pair.
 struct EaAct {
   Window W;
-  int32_t ax;
   int32_t* arr;
   int32_t* stamp;
   int32_t round;
   int32_t* next;
   uint32_t* nsize;
-  int32_t last = -1;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  int32_t ax = 0, last = -1;
+  __device__ bool begin(int32_t x, bool) {
+    if (!W.active(x)) return false;
+    ax = arr[x];
+    last = -1;
+    return true;
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (!W.in(t) || r == last) return true;
     last = r;
@@ -273,69 +403,58 @@ struct EaAct {
     }
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_ea(Streams S, Window W, const int32_t* front, int64_t fsize, int32_t* arr,
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_ea(Streams S, const Ckpt* ck, Items I, Window W, int32_t* arr,
                                                int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < fsize; i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t x = front[i];
-    if (!W.active(x)) continue;
-    EaAct a{W, arr[x], arr, stamp, round, next, nsize};
-    visit_vertex(S, x, a);
-  }
+  EaAct a{W, arr, stamp, round, next, nsize};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
-// Window contact count of every active vertex (PageRank divisor).
+// Window contact count (PageRank divisor).
 struct WdegAct {
   Window W;
+  int32_t* du;
   int32_t c = 0;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) {
+    c = 0;
+    return W.active(x);
+  }
   __device__ bool contact(int32_t, int32_t, int32_t t) {
     c += W.in(t) ? 1 : 0;
     return true;
   }
+  __device__ void end(int32_t x, bool chunked) {
+    if (chunked) {
+      if (c) atomicAdd(&du[x], c);
+    } else {
+      du[x] = c;
+    }
+  }
 };
 
-__global__ void __launch_bounds__(kBlock) k_wdeg(Streams S, Window W, int32_t* du, int full_window) {
-  const Rd g{S.g, S.g_bits};
-  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < S.n; x += (int64_t)gridDim.x * blockDim.x) {
-    int32_t c = 0;
-    if (W.active((int32_t)x)) {
-      if (full_window) {
-        uint64_t pos = ef_get(S, S.gi, (uint64_t)x);
-        c = (int32_t)rd_gamma(S, g, pos);
-      } else {
-        WdegAct a{W};
-        visit_vertex(S, (int32_t)x, a);
-        c = a.c;
-      }
-    }
-    du[x] = c;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_fill_active(Window W, int32_t n, double* acc, double inside) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
-    acc[v] = W.active((int32_t)v) ? inside : 0.0;
-}
-
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_fill(T* a, int64_t n, T val) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
-    a[v] = val;
-}
-
-__global__ void __launch_bounds__(kBlock) k_fill_active_f32(Window W, int32_t n, float* a, float inside) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
-    a[v] = W.active((int32_t)v) ? inside : 0.0f;
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_wdeg(Streams S, const Ckpt* ck, Items I, Window W, int32_t* du) {
+  WdegAct a{W, du};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 struct PrAct {
   Window W;
-  float coef;
+  const float* rank;
+  const int32_t* du;
+  float damping;
   double* acc;
+  float coef = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) {
+    if (!W.active(x) || du[x] == 0) return false;
+    coef = damping * rank[x] / (float)du[x];
+    last = -1;
+    return true;
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (!W.in(t) || r == last) return true;
     last = r;
@@ -345,59 +464,100 @@ struct PrAct {
     }
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_pr_push(Streams S, Window W, const float* rank, const int32_t* du,
-                                                    float damping, double* acc) {
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < S.n; u += (int64_t)gridDim.x * blockDim.x) {
-    if (!W.active((int32_t)u) || du[u] == 0) continue;
-    PrAct a{W, damping * rank[u] / (float)du[u], acc};
-    visit_vertex(S, (int32_t)u, a);
-  }
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_pr_push(Streams S, const Ckpt* ck, Items I, Window W, const float* rank,
+                                                    const int32_t* du, float damping, double* acc) {
+  PrAct a{W, rank, du, damping, acc};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 struct HubAct {
   Window W;
   const float* auth;
-  double acc = 0;
+  double* acc;
+  double sum = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) {
+    sum = 0;
+    last = -1;
+    return W.active(x);
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (!W.in(t) || r == last) return true;
     last = r;
-    if (W.active(r)) acc += (double)auth[r];
+    if (W.active(r)) sum += (double)auth[r];
     return true;
+  }
+  __device__ void end(int32_t x, bool chunked) {
+    if (chunked)
+      atomicAdd(&acc[x], sum);
+    else
+      acc[x] = sum;
   }
 };
 
-__global__ void __launch_bounds__(kBlock) k_hits_pull(Streams S, Window W, const float* auth, double* acc) {
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < S.n; u += (int64_t)gridDim.x * blockDim.x) {
-    HubAct a{W, auth};
-    if (W.active((int32_t)u)) visit_vertex(S, (int32_t)u, a);
-    acc[u] = a.acc;
-  }
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_hits_pull(Streams S, const Ckpt* ck, Items I, Window W, const float* auth,
+                                                      double* acc) {
+  HubAct a{W, auth, acc};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 struct AuthAct {
   Window W;
-  double h;
+  const float* hub;
   double* acc;
+  double h = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t, int32_t) { return true; }
+  __device__ bool begin(int32_t x, bool) {
+    if (!W.active(x)) return false;
+    h = (double)hub[x];
+    last = -1;
+    return true;
+  }
   __device__ bool contact(int32_t, int32_t r, int32_t t) {
     if (!W.in(t) || r == last) return true;
     last = r;
     if (W.active(r)) atomicAdd(&acc[r], h);
     return true;
   }
+  __device__ void end(int32_t, bool) {}
 };
 
-__global__ void __launch_bounds__(kBlock) k_hits_push(Streams S, Window W, const float* hub, double* acc) {
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < S.n; u += (int64_t)gridDim.x * blockDim.x) {
-    if (!W.active((int32_t)u)) continue;
-    AuthAct a{W, (double)hub[u], acc};
-    visit_vertex(S, (int32_t)u, a);
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_hits_push(Streams S, const Ckpt* ck, Items I, Window W, const float* hub,
+                                                      double* acc) {
+  AuthAct a{W, hub, acc};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// ------------------------------------------------------ elementwise, sums ---
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fill(T* a, int64_t n, T val) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    a[v] = val;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fill_active(Window W, int32_t n, T* a, T inside, T outside) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    a[v] = W.active((int32_t)v) ? inside : outside;
+}
+
+__global__ void k_init_labels(Window W, int32_t n, int32_t* label, int32_t* stamp) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    label[v] = W.active((int32_t)v) ? (int32_t)v : -1;
+    stamp[v] = -1;
   }
+}
+
+__global__ void k_copy_deg(Window W, int32_t n, const int32_t* deg, int32_t* du) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    du[v] = W.active((int32_t)v) ? deg[v] : 0;
 }
 
 // Deterministic two-stage reductions (fixed grid, fixed tree order).
@@ -522,9 +682,21 @@ struct tga_graph {
   Streams S{};
   std::vector<void*> allocs;
   int32_t* d_err = nullptr;
+  // schedule
+  Ckpt* ck = nullptr;
+  int64_t nck = 0;
+  int32_t* vinfo = nullptr;  // per vertex: kSmallFlat / kSmallChain / kEmpty / first checkpoint
+  int32_t* vdeg = nullptr;   // per vertex outdegree
+  int32_t* items_flat = nullptr;
+  int32_t* items_chain = nullptr;
+  uint32_t n_flat = 0, n_chain = 0;
+  int32_t* big_flat = nullptr;  // long records, by class
+  int32_t* big_chain = nullptr;
+  int32_t n_big_flat = 0, n_big_chain = 0;
   // scratch (n-sized unless stated)
-  int32_t *q0 = nullptr, *q1 = nullptr, *i0 = nullptr, *i1 = nullptr;
-  uint32_t* cnt = nullptr;  // 4 counters
+  int32_t *q0 = nullptr, *q1 = nullptr, *i0 = nullptr, *i1 = nullptr, *i2 = nullptr;
+  int32_t *xq_flat = nullptr, *xq_chain = nullptr, *xq_big = nullptr;  // expanded item queues
+  uint32_t* cnt = nullptr;  // 8 counters
   unsigned long long* u64s = nullptr;  // 4 values
   double* acc = nullptr;
   double* partial = nullptr;  // kRedBlocks
@@ -532,6 +704,7 @@ struct tga_graph {
   float *f0 = nullptr, *f1 = nullptr;
   int32_t* prio = nullptr;  // default GC priorities, built on first use
   int64_t device_bytes = 0;
+  std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
   bool prof = false;
   double prof_ms = 0;
@@ -578,17 +751,22 @@ struct tga_graph {
     return TGA_OK;
   }
   template <class T>
-  int upload(const T*& dst, const std::vector<T>& src) {
-    T* p = nullptr;
-    int rc = alloc(p, src.size());
+  int upload(T*& dst, const std::vector<T>& src) {
+    int rc = alloc(dst, src.size());
     if (rc) return rc;
-    HIP_TRY(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-    dst = p;
+    if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     return TGA_OK;
+  }
+  template <class T>
+  int upload_const(const T*& dst, const std::vector<T>& src) {
+    T* p = nullptr;
+    int rc = upload(p, src);
+    dst = p;
+    return rc;
   }
   int grid(int64_t items) const {
     const int64_t b = (items + kBlock - 1) / kBlock;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 16));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1 << 20));
   }
   Window window(tga_window w) const { return Window{w.start, w.end, ab, ae}; }
   // Fails with the decode error bits raised since the last clear, if any.
@@ -610,10 +788,20 @@ struct tga_graph {
                            (flags & TGA_OUT_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream));
     return TGA_OK;
   }
-  int read_count(uint32_t* dev, uint32_t& host) {
-    HIP_TRY(hipMemcpyAsync(&host, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  int read_counts(const uint32_t* dev, uint32_t* host, int k) {
+    HIP_TRY(hipMemcpyAsync(host, dev, sizeof(uint32_t) * (size_t)k, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return TGA_OK;
+  }
+  Items flat_items() const { return Items{items_flat, n_flat}; }
+  Items chain_items() const { return Items{items_chain, n_chain}; }
+  // Expands a vertex set into xq_flat / xq_chain (and xq_big); returns the counts.
+  int expand(const int32_t* front, int32_t base, uint32_t count, bool keep_empty, bool want_big, uint32_t hc[3]) {
+    HIP_TRY(hipMemsetAsync(cnt, 0, 3 * sizeof(uint32_t), stream));
+    k_expand<<<grid(count), kBlock, 0, stream>>>(front, base, count, vinfo, ck, xq_flat, xq_chain, cnt,
+                                                 want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return read_counts(cnt, hc, 3);
   }
   ~tga_graph() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -623,6 +811,123 @@ struct tga_graph {
 };
 
 extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
+
+// Builds the work schedule: headers, chain depths, checkpoints, item lists.
+static int build_schedule(tga_graph* g) {
+  const int64_t n = g->n;
+  const Streams& S = g->S;
+  int32_t* d_deg = nullptr;
+  int8_t* d_ref = nullptr;
+  int rc = g->alloc(d_deg, (size_t)n);
+  if (!rc) rc = g->alloc(d_ref, (size_t)n);
+  if (rc) return rc;
+  k_headers<<<g->grid(n), kBlock, 0, g->stream>>>(S, d_deg, d_ref);
+  HIP_TRY(hipGetLastError());
+  std::vector<int32_t> deg((size_t)n);
+  std::vector<int8_t> ref((size_t)n);
+  HIP_TRY(hipMemcpyAsync(deg.data(), d_deg, (size_t)n * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(ref.data(), d_ref, (size_t)n, hipMemcpyDeviceToHost, g->stream));
+  if ((rc = g->check_err())) return rc;
+  g->vdeg = d_deg;
+
+  // Chain depth of every record: r > 0 follows the reference x - r.
+  std::vector<int8_t> depth((size_t)n, 0);
+  int64_t contacts = 0, maxd = 0;
+  for (int64_t x = 0; x < n; ++x) {
+    if (deg[(size_t)x] < 0) return fail(TGA_EFORMAT, "negative outdegree at vertex " + std::to_string(x));
+    contacts += deg[(size_t)x];
+    maxd = std::max<int64_t>(maxd, deg[(size_t)x]);
+    const int r = ref[(size_t)x];
+    if (deg[(size_t)x] == 0 || r == 0) continue;
+    if (r > kWindow || r > x) return fail(TGA_EFORMAT, "reference outside the window at vertex " + std::to_string(x));
+    const int dp = depth[(size_t)(x - r)] + 1;
+    if (dp > kMaxDepth) return fail(TGA_EFORMAT, "reference chain deeper than 3 at vertex " + std::to_string(x));
+    depth[(size_t)x] = (int8_t)dp;
+  }
+  g->info.contacts = contacts;
+  g->info.max_degree = maxd;
+
+  // Long records and their checkpoint slots (at most ceil(d / kChunk) chunks).
+  std::vector<int32_t> bigf, bigc;
+  std::vector<int64_t> slotf, slotc;
+  int64_t slots = 0;
+  for (int64_t x = 0; x < n; ++x) {
+    if (deg[(size_t)x] <= kChunk) continue;
+    auto& v = depth[(size_t)x] ? bigc : bigf;
+    auto& s = depth[(size_t)x] ? slotc : slotf;
+    v.push_back((int32_t)x);
+    s.push_back(slots);
+    slots += (deg[(size_t)x] + kChunk - 1) / kChunk;
+  }
+  g->nck = slots;
+  if ((rc = g->alloc(g->ck, (size_t)std::max<int64_t>(slots, 1)))) return rc;
+  HIP_TRY(hipMemsetAsync(g->ck, 0, sizeof(Ckpt) * (size_t)std::max<int64_t>(slots, 1), g->stream));
+  const int32_t *d_bigf = nullptr, *d_bigc = nullptr;
+  const int64_t *d_slotf = nullptr, *d_slotc = nullptr;
+  if ((rc = g->upload_const(d_bigf, bigf)) || (rc = g->upload_const(d_bigc, bigc)) ||
+      (rc = g->upload_const(d_slotf, slotf)) || (rc = g->upload_const(d_slotc, slotc)))
+    return rc;
+  g->big_flat = const_cast<int32_t*>(d_bigf);
+  g->big_chain = const_cast<int32_t*>(d_bigc);
+  g->n_big_flat = (int32_t)bigf.size();
+  g->n_big_chain = (int32_t)bigc.size();
+  int32_t* d_nch = nullptr;
+  if ((rc = g->alloc(d_nch, bigf.size() + bigc.size()))) return rc;
+  if (!bigf.empty())
+    k_ckpt_build<0><<<(int)((bigf.size() + 63) / 64), 64, 0, g->stream>>>(S, d_bigf, d_slotf, (int32_t)bigf.size(),
+                                                                          g->ck, d_nch);
+  if (!bigc.empty())
+    k_ckpt_build<kMaxDepth><<<(int)((bigc.size() + 63) / 64), 64, 0, g->stream>>>(
+        S, d_bigc, d_slotc, (int32_t)bigc.size(), g->ck, d_nch + bigf.size());
+  HIP_TRY(hipGetLastError());
+  std::vector<int32_t> nch_host(bigf.size() + bigc.size());
+  if (!nch_host.empty())
+    HIP_TRY(hipMemcpyAsync(nch_host.data(), d_nch, nch_host.size() * 4, hipMemcpyDeviceToHost, g->stream));
+  if ((rc = g->check_err())) return rc;
+
+  // Chunk counts of every long record (stored in each of its checkpoints).
+  std::vector<int32_t> vinfo((size_t)n);
+  for (int64_t x = 0; x < n; ++x)
+    vinfo[(size_t)x] = deg[(size_t)x] == 0 ? kEmpty : (depth[(size_t)x] ? kSmallChain : kSmallFlat);
+  std::vector<int32_t> nchunk((size_t)std::max<int64_t>(slots, 1), 0);
+  for (size_t i = 0; i < bigf.size(); ++i) nchunk[(size_t)slotf[i]] = nch_host[i];
+  for (size_t i = 0; i < bigc.size(); ++i) nchunk[(size_t)slotc[i]] = nch_host[bigf.size() + i];
+  std::vector<int32_t> itf, itc;
+  auto chunks_of = [&](const std::vector<int32_t>& big, const std::vector<int64_t>& slot,
+                       std::vector<int32_t>& out) -> int {
+    for (size_t i = 0; i < big.size(); ++i) {
+      const int32_t nch = nchunk[(size_t)slot[i]];
+      if (nch < 1 || nch > (deg[(size_t)big[i]] + kChunk - 1) / kChunk)
+        return fail(TGA_EFORMAT, "checkpoint build failed at vertex " + std::to_string(big[i]));
+      vinfo[(size_t)big[i]] = (int32_t)slot[i];
+      for (int32_t j = 0; j < nch; ++j) out.push_back(-(int32_t)(slot[i] + j) - 1);
+    }
+    return TGA_OK;
+  };
+  if ((rc = chunks_of(bigf, slotf, itf)) || (rc = chunks_of(bigc, slotc, itc))) return rc;
+  // Whole records, longest first; within a length class by id (index locality).
+  std::vector<int32_t> small[2];
+  for (int64_t x = 0; x < n; ++x)
+    if (deg[(size_t)x] > 0 && deg[(size_t)x] <= kChunk) small[depth[(size_t)x] ? 1 : 0].push_back((int32_t)x);
+  for (auto& v : small)
+    std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+      return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
+    });
+  itf.insert(itf.end(), small[0].begin(), small[0].end());
+  itc.insert(itc.end(), small[1].begin(), small[1].end());
+  g->info.chunks = (int64_t)(itf.size() + itc.size());
+  g->h_items.assign((size_t)n, 0);
+  for (int64_t x = 0; x < n; ++x) {
+    const int32_t v = vinfo[(size_t)x];
+    g->h_items[(size_t)x] = v == kEmpty ? 0 : (v < 0 ? 1 : nchunk[(size_t)v]);
+  }
+  g->n_flat = (uint32_t)itf.size();
+  g->n_chain = (uint32_t)itc.size();
+  if ((rc = g->upload(g->items_flat, itf)) || (rc = g->upload(g->items_chain, itc)) ||
+      (rc = g->upload(g->vinfo, vinfo)))
+    return rc;
+  return TGA_OK;
+}
 
 static int open_impl(const std::string& dir, const std::string& name, const tga_dataset_desc* desc, int device,
                      tga_graph** out) {
@@ -688,13 +993,15 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   g->n = n;
   g->ab = (int32_t)std::max<int64_t>(0, desc ? desc->active_begin : 0);
   g->ae = (int32_t)((desc && desc->active_end > 0) ? std::min<int64_t>(desc->active_end, n) : n);
+  if (g->ab > g->ae) g->ab = g->ae;
   g->min_ts = (int32_t)min_ts;
 
   Streams& S = g->S;
-  if ((rc = g->upload(S.g, gw)) || (rc = g->upload(S.t, tw)) || (rc = g->upload(S.gi.lower, glw)) ||
-      (rc = g->upload(S.gi.upper, guw)) || (rc = g->upload(S.gi.samp, gsamp)) || (rc = g->upload(S.gi.spill, gspill)) ||
-      (rc = g->upload(S.ti.lower, tlw)) || (rc = g->upload(S.ti.upper, tuw)) || (rc = g->upload(S.ti.samp, tsamp)) ||
-      (rc = g->upload(S.ti.spill, tspill)))
+  if ((rc = g->upload_const(S.g, gw)) || (rc = g->upload_const(S.t, tw)) || (rc = g->upload_const(S.gi.lower, glw)) ||
+      (rc = g->upload_const(S.gi.upper, guw)) || (rc = g->upload_const(S.gi.samp, gsamp)) ||
+      (rc = g->upload_const(S.gi.spill, gspill)) || (rc = g->upload_const(S.ti.lower, tlw)) ||
+      (rc = g->upload_const(S.ti.upper, tuw)) || (rc = g->upload_const(S.ti.samp, tsamp)) ||
+      (rc = g->upload_const(S.ti.spill, tspill)))
     return bail(rc);
   S.g_bits = (uint64_t)graph.size() * 8;
   S.t_bits = (uint64_t)ts.size() * 8;
@@ -708,26 +1015,18 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   S.err = g->d_err;
   if (hipMemset(g->d_err, 0, sizeof(int32_t)) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipMemset failed"));
 
-  if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
-      (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->cnt, 4)) || (rc = g->alloc(g->u64s, 4)) ||
-      (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) || (rc = g->alloc(g->scal, 4)) ||
-      (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
-    return bail(rc);
+  if ((rc = build_schedule(g))) return bail(rc);
 
-  // Header pass: contacts and maximum outdegree; validates the index.
-  if (hipMemsetAsync(g->u64s, 0, 4 * sizeof(unsigned long long), g->stream) != hipSuccess)
-    return bail(fail(TGA_EDEVICE, "hipMemsetAsync failed"));
-  k_outdeg<<<g->grid(n), kBlock, 0, g->stream>>>(S, g->u64s, g->u64s + 1);
-  unsigned long long hv[2] = {0, 0};
-  if (hipMemcpyAsync(hv, g->u64s, sizeof hv, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
-      hipStreamSynchronize(g->stream) != hipSuccess)
-    return bail(fail(TGA_EDEVICE, "header pass failed"));
-  if ((rc = g->check_err())) return bail(rc);
+  const size_t qn = (size_t)n + (size_t)g->nck;
+  if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
+      (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq_flat, qn)) ||
+      (rc = g->alloc(g->xq_chain, qn)) || (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
+      (rc = g->alloc(g->u64s, 4)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
+      (rc = g->alloc(g->scal, 4)) || (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
+    return bail(rc);
 
   tga_graph_info& I = g->info;
   I.n = n;
-  I.contacts = (int64_t)hv[0];
-  I.max_degree = (int64_t)hv[1];
   I.min_timestamp = min_ts;
   I.graph_bytes = graph.size();
   I.timestamps_bytes = ts.size();
@@ -736,7 +1035,6 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   I.l_timestamps = lt;
   I.active_begin = g->ab;
   I.active_end = g->ae;
-  I.chunks = n;
   I.device_bytes = g->device_bytes;
   *out = g;
   return TGA_OK;
@@ -772,23 +1070,60 @@ extern "C" int tga_info(const tga_graph* g, tga_graph_info* info) {
 
 static bool bad_window(tga_window w) { return w.start > w.end; }
 
+// Launches kernel K<0> over flat items and K<kMaxDepth> over chain items.
+#define LAUNCH_BOTH(K, IF, IC, ...)                                                               \
+  do {                                                                                            \
+    const Items if_ = (IF), ic_ = (IC);                                                           \
+    if (if_.count > 0) {                                                                          \
+      g->kbegin();                                                                                \
+      K<0><<<g->grid(if_.count), kBlock, 0, g->stream>>>(g->S, g->ck, if_, __VA_ARGS__);          \
+      g->kend();                                                                                  \
+    }                                                                                             \
+    if (ic_.count > 0) {                                                                          \
+      g->kbegin();                                                                                \
+      K<kMaxDepth><<<g->grid(ic_.count), kBlock, 0, g->stream>>>(g->S, g->ck, ic_, __VA_ARGS__);  \
+      g->kend();                                                                                  \
+    }                                                                                             \
+    HIP_TRY(hipGetLastError());                                                                   \
+  } while (0)
+
+#define LAUNCH_STATIC(K, ...) LAUNCH_BOTH(K, g->flat_items(), g->chain_items(), __VA_ARGS__)
+#define LAUNCH_QUEUED(K, HC, ...) \
+  LAUNCH_BOTH(K, (Items{g->xq_flat, (HC)[0]}), (Items{g->xq_chain, (HC)[1]}), __VA_ARGS__)
+
 extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
                                    int64_t* contacts) {
   if (!g || (nverts > 0 && !verts) || nverts < 0) return fail(TGA_EINVAL, "tga_neighbor_access: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
-  const int32_t* list = nullptr;
-  int64_t count = g->ae - g->ab;
-  if (nverts > 0) {
-    if (nverts > g->n) return fail(TGA_EINVAL, "tga_neighbor_access: more vertices than the graph holds");
-    HIP_TRY(hipMemcpyAsync(g->q0, verts, (size_t)nverts * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
-    list = g->q0;
-    count = nverts;
-  }
   HIP_TRY(hipMemsetAsync(g->u64s, 0, 2 * sizeof(unsigned long long), g->stream));
-  g->kbegin();
-  k_ra<<<g->grid(count), kBlock, 0, g->stream>>>(g->S, list, g->ab, count, g->u64s);
-  g->kend();
-  HIP_TRY(hipGetLastError());
+  if (nverts > 0) {
+    // Listed vertices: expanded into flat/chain items like a frontier, in
+    // batches whose expansion fits the item queues (a list may repeat a long
+    // record many times).
+    for (int64_t i = 0; i < nverts; ++i)
+      if (verts[i] < 0 || verts[i] >= g->n) return fail(TGA_EINVAL, "tga_neighbor_access: vertex out of range");
+    const int64_t cap_items = g->n + g->nck, cap_verts = g->n;
+    int64_t i0 = 0;
+    while (i0 < nverts) {
+      int64_t i1 = i0, items = 0;
+      while (i1 < nverts && i1 - i0 < cap_verts && items + g->h_items[(size_t)verts[i1]] <= cap_items)
+        items += g->h_items[(size_t)verts[i1++]];
+      HIP_TRY(hipMemcpyAsync(g->q0, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
+                             g->stream));
+      uint32_t hc[3];
+      int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), false, false, hc);
+      if (rc) return rc;
+      LAUNCH_QUEUED(k_ra, hc, g->u64s);
+      i0 = i1;
+    }
+  } else if (g->ab == 0 && g->ae == g->n) {
+    LAUNCH_STATIC(k_ra, g->u64s);
+  } else {
+    uint32_t hc[3];
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), false, false, hc);
+    if (rc) return rc;
+    LAUNCH_QUEUED(k_ra, hc, g->u64s);
+  }
   unsigned long long hv[2];
   HIP_TRY(hipMemcpyAsync(hv, g->u64s, sizeof hv, hipMemcpyDeviceToHost, g->stream));
   int rc = g->check_err();
@@ -801,12 +1136,25 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
 extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* centrality, uint32_t flags) {
   if (!g || !count || bad_window(w)) return fail(TGA_EINVAL, "tga_degree_centrality: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
-  g->kbegin();
-  k_dc<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->S, g->window(w), g->i0, centrality ? g->i1 : nullptr);
-  g->kend();
-  HIP_TRY(hipGetLastError());
+  const Window W = g->window(w);
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, 0);
+  int32_t* centr = centrality ? g->i1 : nullptr;
+  if (centr) {
+    // Vertices with no decoded record keep the formula's value for a zero count.
+    const int32_t c0 = (w.start == w.end) ? 0 : 1;
+    k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(centr, g->n, c0);
+  }
+  LAUNCH_STATIC(k_dc, W, g->i0, centr);
+  if (centr) {
+    if (g->n_big_flat)
+      k_dc_long<0><<<(g->n_big_flat + 63) / 64, 64, 0, g->stream>>>(g->S, g->big_flat, g->n_big_flat, W, centr);
+    if (g->n_big_chain)
+      k_dc_long<kMaxDepth><<<(g->n_big_chain + 63) / 64, 64, 0, g->stream>>>(g->S, g->big_chain, g->n_big_chain, W,
+                                                                              centr);
+    HIP_TRY(hipGetLastError());
+  }
   int rc = g->out_copy(count, g->i0, (size_t)g->n * 4, flags);
-  if (!rc && centrality) rc = g->out_copy(centrality, g->i1, (size_t)g->n * 4, flags);
+  if (!rc && centrality) rc = g->out_copy(centrality, centr, (size_t)g->n * 4, flags);
   if (rc) return rc;
   return g->check_err();
 }
@@ -822,13 +1170,12 @@ extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, 
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t lvl = 0; fsize > 0; ++lvl) {
-    HIP_TRY(hipMemsetAsync(g->cnt, 0, sizeof(uint32_t), g->stream));
-    g->kbegin();
-    k_bfs<<<g->grid(fsize), kBlock, 0, g->stream>>>(g->S, W, cur, fsize, g->i0, nxt, g->cnt, lvl);
-    g->kend();
-    HIP_TRY(hipGetLastError());
-    int rc = g->read_count(g->cnt, fsize);
+    uint32_t hc[3];
+    int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_bfs, hc, W, g->i0, nxt, g->cnt + 4, lvl);
+    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
     std::swap(cur, nxt);
   }
   int rc = g->out_copy(level, g->i0, (size_t)g->n * 4, flags);
@@ -849,25 +1196,17 @@ extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t round = 0; fsize > 0; ++round) {
-    HIP_TRY(hipMemsetAsync(g->cnt, 0, sizeof(uint32_t), g->stream));
-    g->kbegin();
-    k_ea<<<g->grid(fsize), kBlock, 0, g->stream>>>(g->S, W, cur, fsize, g->i0, g->i1, round, nxt, g->cnt);
-    g->kend();
-    HIP_TRY(hipGetLastError());
-    int rc = g->read_count(g->cnt, fsize);
+    uint32_t hc[3];
+    int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_ea, hc, W, g->i0, g->i1, round, nxt, g->cnt + 4);
+    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
     std::swap(cur, nxt);
   }
   int rc = g->out_copy(arrival, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
   return g->check_err();
-}
-
-__global__ void k_init_labels(Window W, int32_t n, int32_t* label, int32_t* stamp) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    label[v] = W.active((int32_t)v) ? (int32_t)v : -1;
-    stamp[v] = -1;
-  }
 }
 
 extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* label, uint32_t flags) {
@@ -878,13 +1217,12 @@ extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* lab
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t fsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; fsize > 0; ++round) {
-    HIP_TRY(hipMemsetAsync(g->cnt, 0, sizeof(uint32_t), g->stream));
-    g->kbegin();
-    k_cc<<<g->grid(fsize), kBlock, 0, g->stream>>>(g->S, W, cur, g->ab, fsize, g->i0, g->i1, round, nxt, g->cnt);
-    g->kend();
-    HIP_TRY(hipGetLastError());
-    int rc = g->read_count(g->cnt, fsize);
+    uint32_t hc[3];
+    int rc = g->expand(cur, g->ab, fsize, false, false, hc);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round, nxt, g->cnt + 4);
+    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
     cur = nxt;
     nxt = (nxt == g->q0) ? g->q1 : g->q0;
   }
@@ -929,23 +1267,27 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
     if (!g->prio) {
       std::vector<int32_t> h((size_t)g->n);
       java_random_ints(123, 256, g->n, h.data());
-      int rc = g->alloc(g->prio, (size_t)g->n);
+      int rc = g->upload(g->prio, h);
       if (rc) return rc;
-      HIP_TRY(hipMemcpy(g->prio, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     }
     prio = g->prio;
   }
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i2, g->n, 0);
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t wsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; round < rounds && wsize > 0; ++round) {
-    HIP_TRY(hipMemsetAsync(g->cnt, 0, sizeof(uint32_t), g->stream));
-    g->kbegin();
-    k_gc<<<g->grid(wsize), kBlock, 0, g->stream>>>(g->S, W, cur, g->ab, wsize, prio, g->i0, round, nxt, g->cnt);
-    g->kend();
-    HIP_TRY(hipGetLastError());
-    int rc = g->read_count(g->cnt, wsize);
+    uint32_t hc[3];
+    int rc = g->expand(cur, g->ab, wsize, true, true, hc);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round, nxt, g->cnt + 4);
+    if (hc[2] > 0) {
+      k_gc_finish<<<g->grid(hc[2]), kBlock, 0, g->stream>>>(g->xq_big, hc[2], W, g->i0, g->i2, round, nxt,
+                                                             g->cnt + 4);
+      HIP_TRY(hipGetLastError());
+    }
+    if ((rc = g->read_counts(g->cnt + 4, &wsize, 1))) return rc;
     cur = nxt;
     nxt = (nxt == g->q0) ? g->q1 : g->q0;
   }
@@ -973,18 +1315,20 @@ extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float dam
   const int64_t N = g->ae - g->ab;
   if (N <= 0) return fail(TGA_EINVAL, "tga_pagerank: no active vertices");
   const bool full = w.start <= TGA_FULL_WINDOW_START && w.end >= TGA_FULL_WINDOW_END;
-  g->kbegin();
-  k_wdeg<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->S, W, g->i0, full ? 1 : 0);
-  g->kend();
+  int32_t* du = g->i0;
+  if (full) {
+    k_copy_deg<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->vdeg, du);
+  } else {
+    k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(du, g->n, 0);
+    LAUNCH_STATIC(k_wdeg, W, du);
+  }
   const float inv = 1.0f / (float)N;
   const float base = (1.0f - damping) / (float)N;
-  k_fill_active_f32<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, inv);
+  k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, inv, 0.0f);
   int32_t it = 0;
   for (; it < iters; ++it) {
-    k_fill_active<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base);
-    g->kbegin();
-    k_pr_push<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->S, W, g->f0, g->i0, damping, g->acc);
-    g->kend();
+    k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
+    LAUNCH_STATIC(k_pr_push, W, g->f0, du, damping, g->acc);
     int rc = reduce(g, 0, g->acc, nullptr, nullptr, g->scal, false);
     if (!rc && tol > 0) rc = reduce(g, 2, g->acc, g->f0, g->scal, g->scal + 1, false);
     if (rc) return rc;
@@ -1010,19 +1354,16 @@ extern "C" int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, f
   if (!g || !hub || !auth || bad_window(w) || iters < 0) return fail(TGA_EINVAL, "tga_hits: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
   const Window W = g->window(w);
-  k_fill_active_f32<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, 1.0f);  // hub
-  k_fill_active_f32<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f1, 1.0f);  // auth
+  k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, 1.0f, 0.0f);  // hub
+  k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f1, 1.0f, 0.0f);  // auth
   for (int32_t it = 0; it < iters; ++it) {
-    g->kbegin();
-    k_hits_pull<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->S, W, g->f1, g->acc);
-    g->kend();
+    k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
+    LAUNCH_STATIC(k_hits_pull, W, g->f1, g->acc);
     int rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
     k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f0);
     k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
-    g->kbegin();
-    k_hits_push<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->S, W, g->f0, g->acc);
-    g->kend();
+    LAUNCH_STATIC(k_hits_push, W, g->f0, g->acc);
     rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
     k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f1);

@@ -765,3 +765,35 @@ extern "C" int tgaf_java_random_ints(int64_t seed, int32_t bound, int64_t count,
   }
   return TGAF_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Code writer for tests and tools: kind 0 unary, 1 gamma, k >= 2 zeta_k
+// (k < 64), 64 a 64-bit big-endian integer.
+
+extern "C" int tgaf_encode_codes(const int32_t* kinds, const uint64_t* values, int64_t n, uint8_t* out,
+                                 int64_t cap, uint64_t* nbits) {
+  if (n < 0 || (n > 0 && (!kinds || !values)) || !out || !nbits) return fail(TGAF_EINVAL, "tgaf_encode_codes: bad arguments");
+  BitWriter w;
+  for (int64_t i = 0; i < n; ++i) {
+    const int k = kinds[i];
+    if (k == 0) {
+      if (values[i] > (1u << 20)) return fail(TGAF_ERANGE, "unary value too large");
+      w.unary(values[i]);
+    } else if (k == 1) {
+      if (values[i] == UINT64_MAX) return fail(TGAF_ERANGE, "gamma value too large");
+      w.gamma(values[i]);
+    } else if (k >= 2 && k < 64) {
+      if (values[i] >= (1ull << 62)) return fail(TGAF_ERANGE, "zeta value too large");
+      w.zeta(values[i], k);
+    } else if (k == 64) {
+      w.put(values[i], 64);
+    } else {
+      return fail(TGAF_EINVAL, "unknown code kind");
+    }
+  }
+  *nbits = w.bits();
+  std::vector<uint8_t>& b = w.finish();
+  if ((int64_t)b.size() > cap) return fail(TGAF_ERANGE, "output buffer too small");
+  std::memcpy(out, b.data(), b.size());
+  return TGAF_OK;
+}

@@ -69,6 +69,8 @@ def _lib():
         lib.tgaf_read_raw_gz.argtypes = [C.c_char_p, C.c_int, p64, p32, p32, p64]
         lib.tgaf_decode_table.argtypes = [C.c_int, p32]
         lib.tgaf_java_random_ints.argtypes = [C.c_int64, C.c_int32, C.c_int64, p32]
+        lib.tgaf_encode_codes.argtypes = [p32, C.POINTER(C.c_uint64), C.c_int64, C.POINTER(C.c_uint8), C.c_int64,
+                                          C.POINTER(C.c_uint64)]
         _LIB = lib
     return _LIB
 
@@ -154,3 +156,15 @@ def java_random_ints(seed: int, bound: int, count: int) -> np.ndarray:
     out = np.empty(count, np.int32)
     _check(_lib().tgaf_java_random_ints(seed, bound, count, _p(out, C.c_int32)))
     return out
+
+
+def encode_codes(kinds, values):
+    """Writes codes (0 unary, 1 gamma, k zeta_k, 64 a 64-bit integer); returns (bytes, bits)."""
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    v = np.ascontiguousarray(np.array(values, dtype=np.uint64))
+    cap = int(sum(64 if kk == 64 else 200 for kk in kinds) // 8 + 16)
+    out = np.zeros(cap, np.uint8)
+    nb = C.c_uint64()
+    _check(_lib().tgaf_encode_codes(_p(k, C.c_int32), _p(v, C.c_uint64), len(k), _p(out, C.c_uint8), cap,
+                                    C.byref(nb)))
+    return bytes(out[:(nb.value + 7) // 8]), nb.value

@@ -96,6 +96,12 @@ int tgaf_decode_table(int kind, int32_t* out65536);
  * scheme of BaselineRunner.java:392-396), count values into out. */
 int tgaf_java_random_ints(int64_t seed, int32_t bound, int64_t count, int32_t* out);
 
+/* Writes a sequence of codes MSB-first into out (cap bytes): kinds[i] is 0 for
+ * unary, 1 for gamma, k in [2, 63] for zeta_k, 64 for a 64-bit integer
+ * (the .timestamps header).  *nbits receives the number of bits written. */
+int tgaf_encode_codes(const int32_t* kinds, const uint64_t* values, int64_t n, uint8_t* out, int64_t cap,
+                      uint64_t* nbits);
+
 #ifdef __cplusplus
 }
 #endif

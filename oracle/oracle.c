@@ -629,3 +629,21 @@ int orc_pagerank_f32(orc* g, const orc_q* q, int32_t iters, double damping, doub
 
 int orc_hits_f64(orc* g, const orc_q* q, int32_t iters, double* hub, double* auth) { HITS_BODY(double) }
 int orc_hits_f32(orc* g, const orc_q* q, int32_t iters, float* hub, float* auth) { HITS_BODY(float) }
+
+/* Reads a sequence of codes (0 unary, 1 gamma, k zeta_k, 64 a 64-bit
+ * integer) from bit position `pos`; *end receives the position after them. */
+int orc_decode_codes(const uint8_t* buf, uint64_t nbytes, uint64_t pos, const int32_t* kinds, int64_t n,
+                     uint64_t* out, uint64_t* end) {
+  bitrd b = {buf, nbytes * 8, pos, 0};
+  for (int64_t i = 0; i < n; i++) {
+    int k = kinds[i];
+    if (k == 0) out[i] = rd_unary(&b);
+    else if (k == 1) out[i] = rd_gamma(&b);
+    else if (k >= 2 && k < 64) out[i] = rd_zeta(&b, k);
+    else if (k == 64) out[i] = rd_int(&b, 64);
+    else return fail("unknown code kind", k);
+  }
+  if (b.overrun) return fail("read past the end of the buffer", (long long)b.pos);
+  *end = b.pos;
+  return 0;
+}

@@ -86,12 +86,23 @@ def lib():
         L.orc_pagerank_f32.argtypes = [C.c_void_p, Q, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_float)]
         L.orc_hits_f64.argtypes = [C.c_void_p, Q, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_hits_f32.argtypes = [C.c_void_p, Q, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_decode_codes.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, p32, C.c_int64, pu64, pu64]
         _lib = L
     return _lib
 
 
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def decode_codes(buf: bytes, pos: int, kinds):
+    """Reads codes (0 unary, 1 gamma, k zeta_k, 64 a 64-bit integer) from bit `pos`."""
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    out = np.zeros(len(k), np.uint64)
+    end = C.c_uint64()
+    if lib().orc_decode_codes(buf, len(buf), pos, _p(k, C.c_int32), len(k), _p(out, C.c_uint64), C.byref(end)):
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+    return [int(v) for v in out], end.value
 
 
 def contact_hash(x: int, r: int, t: int) -> int:
