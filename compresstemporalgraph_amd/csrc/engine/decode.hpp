@@ -31,6 +31,10 @@ constexpr int32_t kInf = 0x7fffffff;
 constexpr int kMaxDepth = 3;          // maxrefcount (BVMultiGraph.java:163)
 constexpr int kWindow = 7;            // windowsize (BVMultiGraph.java:169)
 constexpr uint32_t kSpanBits = 1024;  // select blocks wider than this are stored exactly
+// Contacts between consecutive decode checkpoints of a long record, by the
+// record's reference-chain depth: deeper records cost more per contact (their
+// copies walk the referenced lists), so their chunks are shorter.
+__host__ __device__ constexpr int32_t chunk_stride(int depth) { return depth == 0 ? 256 : (depth == 1 ? 64 : 32); }
 
 // Error bits reported through the per-handle device error word.
 enum : int32_t {
@@ -51,14 +55,21 @@ struct EFDir {
   int32_t l;
 };
 
+struct Ckpt;
+
 struct Streams {
-  const uint64_t* g;  // graph stream, byte-swapped words (+2 zero words)
-  const uint64_t* t;  // timestamp stream, byte-swapped words (+2 zero words)
+  const uint64_t* g;  // graph stream, byte-swapped words (+3 zero words)
+  const uint64_t* t;  // timestamp stream, byte-swapped words (+3 zero words)
   uint64_t g_bits, t_bits;
   EFDir gi, ti;
   int32_t n;       // vertices; ids are [0, n)
   int32_t min_ts;  // .timestamps header value
   int32_t* err;    // sticky error word
+  // Decode checkpoints of long records: vinfo[x] >= 0 is the first
+  // checkpoint of x, checkpoint j covering contacts [j * stride, (j + 1) *
+  // stride) with the stride stored in the checkpoint.
+  const Ckpt* ck;
+  const int32_t* vinfo;
 };
 
 __device__ __forceinline__ void raise_err(const Streams& S, int32_t bit) { atomicOr(S.err, bit); }
@@ -154,23 +165,27 @@ __device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& po
   return v;
 }
 
-// Register bit buffer over one stream: words a = w[idx], b = w[idx + 1] are
-// held, `sh` is the bit offset into a.  Advancing past a loads one word.
+// Register bit buffer over one stream: words a = w[idx], b = w[idx + 1] and
+// c = w[idx + 2] are held, `sh` is the bit offset into a.  Advancing past a
+// loads the word after c, which is needed only 64 or more bits later, so the
+// load's latency overlaps the decoding of the codewords in between.  Stream
+// word arrays carry three zero words of padding; reads past `last` return 0.
 struct BitBuf {
   const uint64_t* w;
   uint64_t idx;
-  uint64_t a, b;
+  uint64_t a, b, c;
   uint32_t sh;
-  uint32_t last;  // index of the last readable word (the padding word)
+  uint32_t last;  // index of the last padding word
 
   __device__ __forceinline__ void init(const uint64_t* words, uint64_t nbits, uint64_t pos) {
     w = words;
-    last = (uint32_t)min((nbits >> 6) + 1, (uint64_t)0xffffffffu);
+    last = (uint32_t)min((((nbits >> 3) + 7) >> 3) + 2, (uint64_t)0xffffffffu);
     idx = pos >> 6;
-    if (idx >= last) idx = last - 1;
+    if (idx + 2 > last) idx = last - 2;
     sh = (uint32_t)pos & 63u;
     a = w[idx];
     b = w[idx + 1];
+    c = w[idx + 2];
   }
   __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
   __device__ __forceinline__ void skip(uint32_t n) {  // n <= 64
@@ -178,8 +193,9 @@ struct BitBuf {
     if (sh >= 64) {
       sh -= 64;
       a = b;
+      b = c;
       ++idx;
-      b = (idx < last) ? w[idx + 1] : 0;
+      c = (idx + 2 <= last) ? w[idx + 2] : 0;
     }
   }
   __device__ __forceinline__ uint64_t pos() const { return idx * 64 + sh; }
@@ -260,7 +276,8 @@ struct Level {
   int32_t hi, irem, ileft;                // intervals: head, values left incl. head, intervals not read
   int32_t hr, rleft;                      // residuals: head, residuals not read
   int32_t hc, cleft, bleft, bidx, brem;   // copies: head, copies left, blocks not read, block index, left in block
-  int32_t pad_[2];
+  int32_t kpos;                           // values this level has yielded
+  int32_t pad_;
 };
 
 template <int MAXD>
@@ -270,19 +287,64 @@ struct Dec {
 };
 
 // Saved decoder state at a chunk boundary of a long list (see engine.hip).
+//
+// Chunks hold a fixed number of contacts, so a run of contacts to one
+// neighbor may cross a chunk boundary.  Such a "split run" gets a slot (the
+// index of the first chunk it enters); `cont_*` names the split run a chunk
+// starts inside, `tail_*` the one it ends inside (slot -1: none).  Rules that
+// act once per distinct neighbor (first window contact of a pair) defer the
+// contacts of split runs to their slot and resolve them after the pass.
 struct Ckpt {
-  int32_t x;      // vertex
-  int32_t k0;     // first contact of the chunk
-  int32_t k1;     // one past its last contact
-  int32_t depth;  // levels in use - 1
-  uint64_t tpos;  // timestamp cursor
-  int32_t tprev;  // timestamp before contact k0
-  int32_t nchunk; // chunks of x (stored in every chunk of x)
+  int32_t x;          // vertex
+  int32_t k0;         // first contact of the chunk
+  int32_t k1;         // one past its last contact
+  int32_t depth;      // levels in use - 1
+  uint64_t tpos;      // timestamp cursor
+  int32_t tprev;      // timestamp before contact k0
+  int32_t nchunk;     // chunks of x (stored in every chunk of x)
+  int32_t cont_slot;  // split run continuing into this chunk, or -1
+  int32_t cont_r;     // its neighbor
+  int32_t tail_slot;  // split run continuing out of this chunk, or -1
+  int32_t tail_r;     // its neighbor
+  int32_t stride;     // contacts per chunk of x
+  int32_t pad_;
   Level lv[kMaxDepth + 1];
+};
+
+// Split-run context of the item being walked.
+struct Split {
+  int32_t cont_slot, cont_r, tail_slot, tail_r;
+  // Slot that contact (r) of this item is deferred to, or -1.
+  __device__ __forceinline__ int32_t slot_of(int32_t r) const {
+    if (r == cont_r && cont_slot >= 0) return cont_slot;
+    if (r == tail_r && tail_slot >= 0) return tail_slot;
+    return -1;
+  }
 };
 
 template <int L, int MAXD>
 __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D);
+
+// Skips toward value `target` of level LL by restoring the checkpoint of the
+// referenced record at or before it, when that record is long enough to have
+// checkpoints.  Returns how many values the jump passed over.
+template <int LL, int MAXD>
+__device__ __forceinline__ int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int32_t target) {
+  if (!S.vinfo) return 0;
+  const int32_t first = S.vinfo[D.lv[LL].x];
+  if (first < 0) return 0;
+  const int32_t stride = S.ck[first].stride;
+  int32_t j = target / stride;
+  const int32_t nch = S.ck[first].nchunk;
+  if (j >= nch) j = nch - 1;
+  const int32_t from = D.lv[LL].kpos;
+  if (j * stride <= from) return 0;
+  const Ckpt* c = S.ck + first + j;
+#pragma unroll
+  for (int l = 0; LL + l <= MAXD; ++l)
+    if (l <= c->depth) D.lv[LL + l] = c->lv[l];
+  return D.lv[LL].kpos - from;
+}
 
 // Pulls the next value of level L's copy source from level L+1.
 template <int L, int MAXD>
@@ -308,6 +370,8 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
         }
         continue;
       }
+      if ((v.bidx & 1) && v.brem > 32) v.brem -= dec_jump<L + 1, MAXD>(S, D, D.lv[L + 1].kpos + v.brem);
+      if (v.brem == 0) continue;
       const int32_t val = dec_next<L + 1, MAXD>(S, D);
       if (val == kInf) {
         raise_err(S, kErrBlocks);
@@ -327,8 +391,10 @@ template <int L, int MAXD>
 __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   Level& v = D.lv[L];
   const Rd g{S.g, S.g_bits};
-  const int32_t m = min(min(v.hm, v.hi), min(v.hr, v.hc));
+  // The deepest level has no copy source.
+  const int32_t m = (L < MAXD) ? min(min(v.hm, v.hi), min(v.hr, v.hc)) : min(min(v.hm, v.hi), v.hr);
   if (m == kInf) return kInf;
+  ++v.kpos;
   if (m == v.hr) {
     if (v.rleft > 0) {
       uint64_t z;
@@ -364,7 +430,7 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
     } else {
       ++v.hi;
     }
-  } else {
+  } else if constexpr (L < MAXD) {
     v.hc = dec_masked<L, MAXD>(S, D);
   }
   return m;
@@ -374,7 +440,7 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
 template <int MAXD>
 __device__ __forceinline__ int32_t dec_peek(const Dec<MAXD>& D) {
   const Level& v = D.lv[0];
-  return min(min(v.hm, v.hi), min(v.hr, v.hc));
+  return (MAXD > 0) ? min(min(v.hm, v.hi), min(v.hr, v.hc)) : min(min(v.hm, v.hi), v.hr);
 }
 
 // Parses the record of x at bit offset `gpos` into level L (and the levels of
@@ -389,6 +455,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   v.hm = v.hi = v.hr = v.hc = kInf;
   v.mrem = v.mleft = v.irem = v.ileft = v.rleft = 0;
   v.cleft = v.bleft = v.brem = 0;
+  v.kpos = 0;
   v.bidx = -1;
   v.mpos = v.ipos = v.rpos = v.bpos = gpos;
   uint64_t pos = gpos;
@@ -513,6 +580,8 @@ __device__ __forceinline__ void ckpt_load(const Streams& S, const Ckpt* c, Dec<M
 
 // Walks contacts [k, kend) of one work item (a whole record or a chunk) and
 // hands (x, r, t) to act.contact(); a false return stops the walk.
+// Walks contacts [k, kend) of one work item and hands (x, r, t, k) to
+// act.contact(); a false return stops the walk.
 template <int MAXD, class Act>
 __device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& tc, int32_t x, int32_t k, int32_t kend,
                                      Act& act) {
@@ -523,13 +592,13 @@ __device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& t
       raise_err(S, r == kInf ? kErrExhausted : kErrNeighbor);
       break;
     }
-    if (!act.contact(x, r, t)) break;
+    if (!act.contact(x, r, t, k)) break;
   }
 }
 
 // Runs one work item.  code >= 0: the whole record of vertex `code`;
-// code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked)
-// may veto the item; act.end(x, chunked) runs after the walk.
+// code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked,
+// split) may veto the item; act.end(x, chunked) runs after the walk.
 template <int MAXD, class Act>
 __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act) {
   Dec<MAXD> D;
@@ -538,7 +607,7 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
   const bool chunked = code < 0;
   if (!chunked) {
     x = code;
-    if (!act.begin(x, false)) return;
+    if (!act.begin(x, false, Split{-1, -1, -1, -1})) return;
     int32_t depth = 0;
     kend = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
     if (kend == 0) {
@@ -550,7 +619,7 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
   } else {
     const Ckpt* c = ck + (-code - 1);
     x = c->x;
-    if (!act.begin(x, true)) return;
+    if (!act.begin(x, true, Split{c->cont_slot, c->cont_r, c->tail_slot, c->tail_r})) return;
     k = c->k0;
     kend = c->k1;
     ckpt_load<MAXD>(S, c, D, tc);

@@ -7,9 +7,9 @@
 // decode.hpp).  At open the loader
 //   1. reads every record header on the GPU (outdegree d, reference r),
 //   2. derives each record's reference-chain depth on the host,
-//   3. decodes every record with d > kChunk once, sequentially, saving the
-//      decoder state every kChunk contacts at the next change of neighbor
-//      (so a neighbor's contacts never straddle two chunks), and
+//   3. decodes every record longer than its chunk stride once, sequentially,
+//      saving the decoder state every stride contacts (runs of one neighbor
+//      that cross a boundary are tracked as split runs, see Ckpt), and
 //   4. builds two item lists, one for flat records (no reference: a
 //      one-level decoder with few registers) and one for records with a
 //      reference chain (a four-level decoder), each ordered by decreasing
@@ -52,10 +52,18 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
-constexpr int32_t kChunk = 256;  // contacts per checkpointed chunk
 
-// vinfo[x] codes for vertices whose record is not chunked.
-constexpr int32_t kSmallFlat = -1, kSmallChain = -2, kEmpty = -3;
+// Work items are split into classes by the reference-chain depth of their
+// record (0: no reference ... 3), each decoded by a kernel holding exactly the
+// decoder levels it needs.
+constexpr int kClasses = kMaxDepth + 1;
+// vinfo[x]: whole record of depth d -> -1 - d; no contacts -> kEmpty;
+// chunked record -> index of its first checkpoint.
+constexpr int32_t kEmpty = -1 - kClasses;
+
+struct Queues {
+  int32_t* q[kClasses];
+};
 
 struct Window {
   int64_t ws, we;
@@ -112,8 +120,9 @@ __global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int
   }
 }
 
-// Decodes each long record once and saves a checkpoint every kChunk contacts,
-// moved forward to the next change of neighbor.
+// Decodes each long record once and saves a checkpoint every chunk_stride(MAXD)
+// contacts.  A run of one neighbor crossing a boundary gets a split-run slot:
+// the index of the first chunk it continues into (see Ckpt).
 template <int MAXD>
 __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* bigv, const int64_t* slot, int32_t nbig,
                                                    Ckpt* ck, int32_t* nchunk_out) {
@@ -121,6 +130,7 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
   if (i >= nbig) return;
   const int32_t x = bigv[i];
   Ckpt* out = ck + slot[i];
+  constexpr int32_t kStride = chunk_stride(MAXD);
   Dec<MAXD> D;
   TsCursor tc;
   int32_t depth = 0;
@@ -128,46 +138,59 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
   tc.b.init(S.t, S.t_bits, ef_get(S, S.ti, (uint64_t)x));
   tc.prev = S.min_ts;
   ckpt_save<MAXD>(out, D, depth, tc, x, 0);
-  int32_t nch = 1, target = kChunk, prev = -1;
+  out[0].cont_slot = -1;
+  out[0].cont_r = -1;
+  int32_t nch = 1, prev = -1, run_slot = -1;
   for (int32_t k = 0; k < d; ++k) {
-    if (k >= target) {
+    if (k > 0 && k % kStride == 0) {
       const int32_t nr = dec_peek<MAXD>(D);
-      if (nr != prev) {
-        out[nch - 1].k1 = k;
-        ckpt_save<MAXD>(out + nch, D, depth, tc, x, k);
-        ++nch;
-        target = k + kChunk;
-      }
+      const bool cont = nr == prev;
+      if (cont && run_slot < 0) run_slot = (int32_t)slot[i] + nch;
+      out[nch - 1].k1 = k;
+      out[nch - 1].tail_slot = cont ? run_slot : -1;
+      out[nch - 1].tail_r = prev;
+      ckpt_save<MAXD>(out + nch, D, depth, tc, x, k);
+      out[nch].cont_slot = cont ? run_slot : -1;
+      out[nch].cont_r = nr;
+      ++nch;
     }
-    prev = dec_next<0, MAXD>(S, D);
+    const int32_t r = dec_next<0, MAXD>(S, D);
     ts_next(S, tc);
+    if (r != prev) run_slot = -1;
+    prev = r;
   }
   out[nch - 1].k1 = d;
-  for (int32_t j = 0; j < nch; ++j) out[j].nchunk = nch;
+  out[nch - 1].tail_slot = -1;
+  out[nch - 1].tail_r = -1;
+  for (int32_t j = 0; j < nch; ++j) {
+    out[j].nchunk = nch;
+    out[j].stride = kStride;
+  }
   nchunk_out[i] = nch;
 }
 
 // Expands a vertex set (an explicit list, or the range [base, base + count))
-// into flat and chain item queues.  keep_empty keeps records without contacts
-// (graph coloring colors them).  Long records also go to bigq when given.
+// into per-class item queues; cnt[c] counts class c.  keep_empty keeps
+// records without contacts (graph coloring colors them) in class 0.  Chunked
+// records also go to bigq (counted in cnt[kClasses]) when given.
 __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t base, uint32_t count,
-                                                   const int32_t* vinfo, const Ckpt* ck, int32_t* flatq,
-                                                   int32_t* chainq, uint32_t* cnt, int32_t* bigq, int keep_empty) {
+                                                   const int32_t* vinfo, const Ckpt* ck, Queues Q, uint32_t* cnt,
+                                                   int32_t* bigq, int keep_empty) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const int32_t x = front ? front[i] : base + (int32_t)i;
     const int32_t info = vinfo[x];
-    if (info == kSmallFlat || (info == kEmpty && keep_empty)) {
-      push(flatq, &cnt[0], x);
-    } else if (info == kSmallChain) {
-      push(chainq, &cnt[1], x);
-    } else if (info >= 0) {
+    if (info == kEmpty) {
+      if (keep_empty) push(Q.q[0], &cnt[0], x);
+    } else if (info < 0) {
+      const int32_t cls = -1 - info;
+      push(Q.q[cls], &cnt[cls], x);
+    } else {
       const Ckpt* c = ck + info;
       const int32_t nch = c->nchunk;
-      const bool chain = c->depth > 0;
-      const uint32_t at = atomicAdd(&cnt[chain ? 1 : 0], (uint32_t)nch);
-      int32_t* q = chain ? chainq : flatq;
-      for (int32_t j = 0; j < nch; ++j) q[at + j] = -(info + j) - 1;
-      if (bigq) push(bigq, &cnt[2], x);
+      const int32_t cls = c->depth;
+      const uint32_t at = atomicAdd(&cnt[cls], (uint32_t)nch);
+      for (int32_t j = 0; j < nch; ++j) Q.q[cls][at + j] = -(info + j) - 1;
+      if (bigq) push(bigq, &cnt[kClasses], x);
     }
   }
 }
@@ -176,8 +199,8 @@ __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t
 
 struct RaAct {
   uint64_t cs = 0, cnt = 0;
-  __device__ bool begin(int32_t, bool) { return true; }
-  __device__ bool contact(int32_t x, int32_t r, int32_t t) {
+  __device__ bool begin(int32_t, bool, Split) { return true; }
+  __device__ bool contact(int32_t x, int32_t r, int32_t t, int32_t) {
     cs += contact_hash(x, r, t);
     ++cnt;
     return true;
@@ -204,12 +227,12 @@ struct DcAct {
   int32_t* centr;
   int32_t deg = 0;
   int64_t avg = 0;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split) {
     deg = 0;
     avg = 0;
     return W.active(x);
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
     if (W.in(t) && W.active(r)) {
       ++deg;
       avg += ((int64_t)t - avg) / deg;  // truncating, as Java's long division
@@ -266,8 +289,8 @@ struct BfsAct {
   int32_t* next;
   uint32_t* nsize;
   int32_t lvl;
-  __device__ bool begin(int32_t x, bool) { return W.active(x); }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool begin(int32_t x, bool, Split) { return W.active(x); }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
     if (W.in(t) && W.active(r) && level[r] == kInf && atomicCAS(&level[r], kInf, lvl + 1) == kInf)
       push(next, nsize, r);
     return true;
@@ -291,12 +314,12 @@ struct CcAct {
   int32_t* next;
   uint32_t* nsize;
   int32_t lx = 0;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split) {
     if (!W.active(x)) return false;
     lx = label[x];
     return true;
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
     if (W.in(t) && W.active(r) && lx < label[r]) {
       const int32_t old = atomicMin(&label[r], lx);
       if (lx < old && atomicMax(&stamp[r], round) < round) push(next, nsize, r);
@@ -327,7 +350,7 @@ struct GcAct {
   uint32_t* nsize;
   int32_t v = 0, pv = 0;
   bool blk = false;
-  __device__ bool begin(int32_t x, bool chunked) {
+  __device__ bool begin(int32_t x, bool chunked, Split) {
     if (!W.active(x) || color[x] != kInf) return false;
     if (chunked && blocked[x]) return false;  // another chunk already found a blocker
     v = x;
@@ -335,7 +358,7 @@ struct GcAct {
     blk = false;
     return true;
   }
-  __device__ bool contact(int32_t, int32_t u, int32_t t) {
+  __device__ bool contact(int32_t, int32_t u, int32_t t, int32_t) {
     if (W.in(t) && W.active(u)) {
       const int32_t pu = prio[u];
       if ((pu > pv || (pu == pv && u < v)) && color[u] >= round) {
@@ -379,6 +402,19 @@ __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint3
   }
 }
 
+// Rules applied once per distinct neighbor use the pair's first window
+// contact.  Contacts of a run split across chunks are deferred: each chunk
+// posts its first window contact of the run to the run's slot as
+// (contact index << 32 | t) with an atomic minimum, and a fix-up kernel
+// applies the rule once per slot after the pass.
+__device__ __forceinline__ bool defer(const Split& sp, uint64_t* runkey, int32_t r, int32_t t, int32_t k,
+                                      const Window& W) {
+  const int32_t s = sp.slot_of(r);
+  if (s < 0) return false;
+  if (W.in(t)) atomicMin((unsigned long long*)&runkey[s], ((uint64_t)(uint32_t)k << 32) | (uint32_t)t);
+  return true;
+}
+
 // Earliest arrival (bfs.cu:1452-1471 rule): first window contact per pair.
 struct EaAct {
   Window W;
@@ -387,20 +423,27 @@ struct EaAct {
   int32_t round;
   int32_t* next;
   uint32_t* nsize;
+  uint64_t* runkey;
+  Split sp{-1, -1, -1, -1};
   int32_t ax = 0, last = -1;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split s) {
     if (!W.active(x)) return false;
     ax = arr[x];
     last = -1;
+    sp = s;
     return true;
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
-    if (!W.in(t) || r == last) return true;
-    last = r;
+  __device__ void relax(int32_t r, int32_t t) {
     if (W.active(r) && ax <= t && t < arr[r]) {
       const int32_t old = atomicMin(&arr[r], t);
       if (t < old && atomicMax(&stamp[r], round) < round) push(next, nsize, r);
     }
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    if (defer(sp, runkey, r, t, k, W)) return true;
+    if (!W.in(t) || r == last) return true;
+    last = r;
+    relax(r, t);
     return true;
   }
   __device__ void end(int32_t, bool) {}
@@ -408,9 +451,35 @@ struct EaAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ea(Streams S, const Ckpt* ck, Items I, Window W, int32_t* arr,
-                                               int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize) {
-  EaAct a{W, arr, stamp, round, next, nsize};
+                                               int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize,
+                                               uint64_t* runkey) {
+  EaAct a{W, arr, stamp, round, next, nsize, runkey};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Resolved split runs: the slot's pair (x, r) and its first window contact t.
+struct SplitRun {
+  int32_t x, r, t;
+};
+
+__device__ __forceinline__ bool split_run(const Ckpt* ck, uint64_t* runkey, int32_t s, SplitRun& out) {
+  const uint64_t key = runkey[s];
+  if (key == ~0ull) return false;
+  runkey[s] = ~0ull;  // ready for the next pass
+  out = SplitRun{ck[s].x, ck[s].cont_r, (int32_t)(uint32_t)key};
+  return true;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ea_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
+                                                   int32_t* arr, int32_t* stamp, int32_t round, int32_t* next,
+                                                   uint32_t* nsize) {
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nck; s += gridDim.x * blockDim.x) {
+    SplitRun q;
+    if (!split_run(ck, runkey, s, q)) continue;
+    EaAct a{W, arr, stamp, round, next, nsize, runkey};
+    a.ax = arr[q.x];
+    a.relax(q.r, q.t);
+  }
 }
 
 // Window contact count (PageRank divisor).
@@ -418,11 +487,11 @@ struct WdegAct {
   Window W;
   int32_t* du;
   int32_t c = 0;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split) {
     c = 0;
     return W.active(x);
   }
-  __device__ bool contact(int32_t, int32_t, int32_t t) {
+  __device__ bool contact(int32_t, int32_t, int32_t t, int32_t) {
     c += W.in(t) ? 1 : 0;
     return true;
   }
@@ -441,27 +510,37 @@ __global__ void __launch_bounds__(kBlock) k_wdeg(Streams S, const Ckpt* ck, Item
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
+__device__ __forceinline__ float pr_coef(float damping, const float* rank, const int32_t* du, int32_t x) {
+  return damping * rank[x] / (float)du[x];
+}
+
+__device__ __forceinline__ double pr_term(float coef, int32_t t) {
+  const float w = 1.0f - expf(-0.1f * (float)t);
+  return (double)(coef * w);
+}
+
 struct PrAct {
   Window W;
   const float* rank;
   const int32_t* du;
   float damping;
   double* acc;
+  uint64_t* runkey;
+  Split sp{-1, -1, -1, -1};
   float coef = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split s) {
     if (!W.active(x) || du[x] == 0) return false;
-    coef = damping * rank[x] / (float)du[x];
+    coef = pr_coef(damping, rank, du, x);
     last = -1;
+    sp = s;
     return true;
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    if (defer(sp, runkey, r, t, k, W)) return true;
     if (!W.in(t) || r == last) return true;
     last = r;
-    if (W.active(r)) {
-      const float w = 1.0f - expf(-0.1f * (float)t);
-      atomicAdd(&acc[r], (double)(coef * w));
-    }
+    if (W.active(r)) atomicAdd(&acc[r], pr_term(coef, t));
     return true;
   }
   __device__ void end(int32_t, bool) {}
@@ -469,23 +548,36 @@ struct PrAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_pr_push(Streams S, const Ckpt* ck, Items I, Window W, const float* rank,
-                                                    const int32_t* du, float damping, double* acc) {
-  PrAct a{W, rank, du, damping, acc};
+                                                    const int32_t* du, float damping, double* acc, uint64_t* runkey) {
+  PrAct a{W, rank, du, damping, acc, runkey};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pr_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
+                                                   const float* rank, const int32_t* du, float damping, double* acc) {
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nck; s += gridDim.x * blockDim.x) {
+    SplitRun q;
+    if (!split_run(ck, runkey, s, q)) continue;
+    if (W.active(q.r)) atomicAdd(&acc[q.r], pr_term(pr_coef(damping, rank, du, q.x), q.t));
+  }
 }
 
 struct HubAct {
   Window W;
   const float* auth;
   double* acc;
+  uint64_t* runkey;
+  Split sp{-1, -1, -1, -1};
   double sum = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split s) {
     sum = 0;
     last = -1;
+    sp = s;
     return W.active(x);
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    if (defer(sp, runkey, r, t, k, W)) return true;
     if (!W.in(t) || r == last) return true;
     last = r;
     if (W.active(r)) sum += (double)auth[r];
@@ -501,8 +593,8 @@ struct HubAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_hits_pull(Streams S, const Ckpt* ck, Items I, Window W, const float* auth,
-                                                      double* acc) {
-  HubAct a{W, auth, acc};
+                                                      double* acc, uint64_t* runkey) {
+  HubAct a{W, auth, acc, runkey};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
@@ -510,15 +602,19 @@ struct AuthAct {
   Window W;
   const float* hub;
   double* acc;
+  uint64_t* runkey;
+  Split sp{-1, -1, -1, -1};
   double h = 0;
   int32_t last = -1;
-  __device__ bool begin(int32_t x, bool) {
+  __device__ bool begin(int32_t x, bool, Split s) {
     if (!W.active(x)) return false;
     h = (double)hub[x];
     last = -1;
+    sp = s;
     return true;
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    if (defer(sp, runkey, r, t, k, W)) return true;
     if (!W.in(t) || r == last) return true;
     last = r;
     if (W.active(r)) atomicAdd(&acc[r], h);
@@ -529,9 +625,23 @@ struct AuthAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_hits_push(Streams S, const Ckpt* ck, Items I, Window W, const float* hub,
-                                                      double* acc) {
-  AuthAct a{W, hub, acc};
+                                                      double* acc, uint64_t* runkey) {
+  AuthAct a{W, hub, acc, runkey};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// mode 0: hub[x] += auth[r]; mode 1: auth[r] += hub[x] (per resolved split run).
+__global__ void __launch_bounds__(kBlock) k_hits_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
+                                                     const float* other, double* acc, int mode) {
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nck; s += gridDim.x * blockDim.x) {
+    SplitRun q;
+    if (!split_run(ck, runkey, s, q)) continue;
+    if (!W.active(q.r) || !W.active(q.x)) continue;
+    if (mode == 0)
+      atomicAdd(&acc[q.x], (double)other[q.r]);
+    else
+      atomicAdd(&acc[q.r], (double)other[q.x]);
+  }
 }
 
 // ------------------------------------------------------ elementwise, sums ---
@@ -623,9 +733,9 @@ bool read_file(const std::string& path, std::vector<uint8_t>& out) {
 }
 
 // Bytes -> 64-bit words whose most significant byte is the first stream byte,
-// plus two zero words of padding.
+// plus three zero words of padding (BitBuf reads up to two words ahead).
 std::vector<uint64_t> be_words(const std::vector<uint8_t>& b) {
-  std::vector<uint64_t> w((b.size() + 7) / 8 + 2, 0);
+  std::vector<uint64_t> w((b.size() + 7) / 8 + 3, 0);
   for (size_t i = 0; i < b.size(); ++i) w[i >> 3] |= (uint64_t)b[i] << (56 - 8 * (i & 7));
   return w;
 }
@@ -675,6 +785,11 @@ void build_select(const std::vector<uint64_t>& up, size_t words, uint64_t& ones,
 struct tga_graph {
   int device = 0;
   hipStream_t stream = nullptr;
+  // The classes of one pass run concurrently on their own streams (default
+  // priority), forked from and joined back into `stream` with events.
+  hipStream_t cstream[kClasses] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kClasses] = {};
   int64_t n = 0;
   int32_t ab = 0, ae = 0;
   int32_t min_ts = 0;
@@ -685,17 +800,17 @@ struct tga_graph {
   // schedule
   Ckpt* ck = nullptr;
   int64_t nck = 0;
-  int32_t* vinfo = nullptr;  // per vertex: kSmallFlat / kSmallChain / kEmpty / first checkpoint
+  uint64_t* runkey = nullptr;  // per checkpoint: split-run first window contact (~0: none)
+  int32_t* vinfo = nullptr;  // per vertex item code (see kEmpty)
   int32_t* vdeg = nullptr;   // per vertex outdegree
-  int32_t* items_flat = nullptr;
-  int32_t* items_chain = nullptr;
-  uint32_t n_flat = 0, n_chain = 0;
-  int32_t* big_flat = nullptr;  // long records, by class
-  int32_t* big_chain = nullptr;
-  int32_t n_big_flat = 0, n_big_chain = 0;
+  int32_t* items[kClasses] = {};  // static item lists by class
+  uint32_t n_items[kClasses] = {};
+  int32_t* big[kClasses] = {};  // chunked records by class
+  int32_t n_big[kClasses] = {};
   // scratch (n-sized unless stated)
   int32_t *q0 = nullptr, *q1 = nullptr, *i0 = nullptr, *i1 = nullptr, *i2 = nullptr;
-  int32_t *xq_flat = nullptr, *xq_chain = nullptr, *xq_big = nullptr;  // expanded item queues
+  Queues xq{};                // expanded item queues by class
+  int32_t* xq_big = nullptr;  // chunked records of the expanded set
   uint32_t* cnt = nullptr;  // 8 counters
   unsigned long long* u64s = nullptr;  // 4 values
   double* acc = nullptr;
@@ -793,19 +908,34 @@ struct tga_graph {
     HIP_TRY(hipStreamSynchronize(stream));
     return TGA_OK;
   }
-  Items flat_items() const { return Items{items_flat, n_flat}; }
-  Items chain_items() const { return Items{items_chain, n_chain}; }
-  // Expands a vertex set into xq_flat / xq_chain (and xq_big); returns the counts.
-  int expand(const int32_t* front, int32_t base, uint32_t count, bool keep_empty, bool want_big, uint32_t hc[3]) {
-    HIP_TRY(hipMemsetAsync(cnt, 0, 3 * sizeof(uint32_t), stream));
-    k_expand<<<grid(count), kBlock, 0, stream>>>(front, base, count, vinfo, ck, xq_flat, xq_chain, cnt,
+  // Expands a vertex set into the class queues (and xq_big); hc[c] receives
+  // the count of class c and hc[kClasses] the number of chunked records.
+  int expand(const int32_t* front, int32_t base, uint32_t count, bool keep_empty, bool want_big,
+             uint32_t hc[kClasses + 1]) {
+    HIP_TRY(hipMemsetAsync(cnt, 0, (kClasses + 1) * sizeof(uint32_t), stream));
+    k_expand<<<grid(count), kBlock, 0, stream>>>(front, base, count, vinfo, ck, xq, cnt,
                                                  want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
     HIP_TRY(hipGetLastError());
-    return read_counts(cnt, hc, 3);
+    return read_counts(cnt, hc, kClasses + 1);
+  }
+  int create_streams() {
+    HIP_TRY(hipStreamCreate(&stream));
+    HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (int c = 0; c < kClasses; ++c) {
+      HIP_TRY(hipStreamCreate(&cstream[c]));
+      HIP_TRY(hipEventCreateWithFlags(&ev_join[c], hipEventDisableTiming));
+    }
+    return TGA_OK;
   }
   ~tga_graph() {
+    if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+    for (int c = 0; c < kClasses; ++c) {
+      if (ev_join[c]) (void)hipEventDestroy(ev_join[c]);
+      if (cstream[c]) (void)hipStreamDestroy(cstream[c]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -847,85 +977,99 @@ static int build_schedule(tga_graph* g) {
   g->info.contacts = contacts;
   g->info.max_degree = maxd;
 
-  // Long records and their checkpoint slots (at most ceil(d / kChunk) chunks).
-  std::vector<int32_t> bigf, bigc;
-  std::vector<int64_t> slotf, slotc;
+  // Long records and their checkpoint slots (ceil(d / stride) chunks each).
+  std::vector<int32_t> bigv[kClasses];
+  std::vector<int64_t> slotv[kClasses];
   int64_t slots = 0;
   for (int64_t x = 0; x < n; ++x) {
-    if (deg[(size_t)x] <= kChunk) continue;
-    auto& v = depth[(size_t)x] ? bigc : bigf;
-    auto& s = depth[(size_t)x] ? slotc : slotf;
-    v.push_back((int32_t)x);
-    s.push_back(slots);
-    slots += (deg[(size_t)x] + kChunk - 1) / kChunk;
+    const int c = depth[(size_t)x];
+    if (deg[(size_t)x] <= chunk_stride(c)) continue;
+    bigv[c].push_back((int32_t)x);
+    slotv[c].push_back(slots);
+    slots += (deg[(size_t)x] + chunk_stride(c) - 1) / chunk_stride(c);
   }
   g->nck = slots;
-  if ((rc = g->alloc(g->ck, (size_t)std::max<int64_t>(slots, 1)))) return rc;
-  HIP_TRY(hipMemsetAsync(g->ck, 0, sizeof(Ckpt) * (size_t)std::max<int64_t>(slots, 1), g->stream));
-  const int32_t *d_bigf = nullptr, *d_bigc = nullptr;
-  const int64_t *d_slotf = nullptr, *d_slotc = nullptr;
-  if ((rc = g->upload_const(d_bigf, bigf)) || (rc = g->upload_const(d_bigc, bigc)) ||
-      (rc = g->upload_const(d_slotf, slotf)) || (rc = g->upload_const(d_slotc, slotc)))
-    return rc;
-  g->big_flat = const_cast<int32_t*>(d_bigf);
-  g->big_chain = const_cast<int32_t*>(d_bigc);
-  g->n_big_flat = (int32_t)bigf.size();
-  g->n_big_chain = (int32_t)bigc.size();
+  const size_t nslots = (size_t)std::max<int64_t>(slots, 1);
+  if ((rc = g->alloc(g->ck, nslots))) return rc;
+  HIP_TRY(hipMemsetAsync(g->ck, 0, sizeof(Ckpt) * nslots, g->stream));
+  if ((rc = g->alloc(g->runkey, nslots))) return rc;
+  HIP_TRY(hipMemsetAsync(g->runkey, 0xff, sizeof(uint64_t) * nslots, g->stream));
+  // Item code of every vertex.  Uploaded before the checkpoints are built so
+  // that a record referencing a long record can jump through the latter's
+  // checkpoints (classes are built in order of depth, a record's references
+  // having smaller depth).
+  std::vector<int32_t> vinfo((size_t)n);
+  for (int64_t x = 0; x < n; ++x) vinfo[(size_t)x] = deg[(size_t)x] == 0 ? kEmpty : -1 - depth[(size_t)x];
+  for (int c = 0; c < kClasses; ++c)
+    for (size_t i = 0; i < bigv[c].size(); ++i) vinfo[(size_t)bigv[c][i]] = (int32_t)slotv[c][i];
+  if ((rc = g->upload(g->vinfo, vinfo))) return rc;
+  g->S.ck = g->ck;
+  g->S.vinfo = g->vinfo;
+  size_t nbig_all = 0;
+  for (int c = 0; c < kClasses; ++c) nbig_all += bigv[c].size();
   int32_t* d_nch = nullptr;
-  if ((rc = g->alloc(d_nch, bigf.size() + bigc.size()))) return rc;
-  if (!bigf.empty())
-    k_ckpt_build<0><<<(int)((bigf.size() + 63) / 64), 64, 0, g->stream>>>(S, d_bigf, d_slotf, (int32_t)bigf.size(),
-                                                                          g->ck, d_nch);
-  if (!bigc.empty())
-    k_ckpt_build<kMaxDepth><<<(int)((bigc.size() + 63) / 64), 64, 0, g->stream>>>(
-        S, d_bigc, d_slotc, (int32_t)bigc.size(), g->ck, d_nch + bigf.size());
-  HIP_TRY(hipGetLastError());
-  std::vector<int32_t> nch_host(bigf.size() + bigc.size());
+  if ((rc = g->alloc(d_nch, nbig_all))) return rc;
+  size_t nch_at[kClasses];
+  {
+    size_t at = 0;
+    for (int c = 0; c < kClasses; ++c) {
+      nch_at[c] = at;
+      at += bigv[c].size();
+      const int64_t* d_slot = nullptr;
+      if ((rc = g->upload(g->big[c], bigv[c])) || (rc = g->upload_const(d_slot, slotv[c]))) return rc;
+      g->n_big[c] = (int32_t)bigv[c].size();
+      if (bigv[c].empty()) continue;
+      const int blocks = (int)((bigv[c].size() + 63) / 64);
+      const int32_t nb = (int32_t)bigv[c].size();
+      if (c == 0) k_ckpt_build<0><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
+      if (c == 1) k_ckpt_build<1><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
+      if (c == 2) k_ckpt_build<2><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
+      if (c == 3) k_ckpt_build<3><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  std::vector<int32_t> nch_host(nbig_all);
   if (!nch_host.empty())
     HIP_TRY(hipMemcpyAsync(nch_host.data(), d_nch, nch_host.size() * 4, hipMemcpyDeviceToHost, g->stream));
   if ((rc = g->check_err())) return rc;
 
-  // Chunk counts of every long record (stored in each of its checkpoints).
-  std::vector<int32_t> vinfo((size_t)n);
-  for (int64_t x = 0; x < n; ++x)
-    vinfo[(size_t)x] = deg[(size_t)x] == 0 ? kEmpty : (depth[(size_t)x] ? kSmallChain : kSmallFlat);
-  std::vector<int32_t> nchunk((size_t)std::max<int64_t>(slots, 1), 0);
-  for (size_t i = 0; i < bigf.size(); ++i) nchunk[(size_t)slotf[i]] = nch_host[i];
-  for (size_t i = 0; i < bigc.size(); ++i) nchunk[(size_t)slotc[i]] = nch_host[bigf.size() + i];
-  std::vector<int32_t> itf, itc;
-  auto chunks_of = [&](const std::vector<int32_t>& big, const std::vector<int64_t>& slot,
-                       std::vector<int32_t>& out) -> int {
-    for (size_t i = 0; i < big.size(); ++i) {
-      const int32_t nch = nchunk[(size_t)slot[i]];
-      if (nch < 1 || nch > (deg[(size_t)big[i]] + kChunk - 1) / kChunk)
-        return fail(TGA_EFORMAT, "checkpoint build failed at vertex " + std::to_string(big[i]));
-      vinfo[(size_t)big[i]] = (int32_t)slot[i];
-      for (int32_t j = 0; j < nch; ++j) out.push_back(-(int32_t)(slot[i] + j) - 1);
+  // Item code of every vertex, and the static item lists: chunks first, then
+  // whole records longest first (within a length class by id, for index
+  // locality).
+  std::vector<int32_t> nchunk(nslots, 0);
+  std::vector<int32_t> items[kClasses];
+  int64_t total_items = 0;
+  for (int c = 0; c < kClasses; ++c) {
+    for (size_t i = 0; i < bigv[c].size(); ++i) {
+      const int32_t x = bigv[c][i];
+      const int32_t nch = nch_host[nch_at[c] + i];
+      if (nch != (deg[(size_t)x] + chunk_stride(c) - 1) / chunk_stride(c))
+        return fail(TGA_EFORMAT, "checkpoint build failed at vertex " + std::to_string(x));
+      nchunk[(size_t)slotv[c][i]] = nch;
+      for (int32_t j = 0; j < nch; ++j) items[c].push_back(-(int32_t)(slotv[c][i] + j) - 1);
     }
-    return TGA_OK;
-  };
-  if ((rc = chunks_of(bigf, slotf, itf)) || (rc = chunks_of(bigc, slotc, itc))) return rc;
-  // Whole records, longest first; within a length class by id (index locality).
-  std::vector<int32_t> small[2];
-  for (int64_t x = 0; x < n; ++x)
-    if (deg[(size_t)x] > 0 && deg[(size_t)x] <= kChunk) small[depth[(size_t)x] ? 1 : 0].push_back((int32_t)x);
-  for (auto& v : small)
-    std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
-      return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
-    });
-  itf.insert(itf.end(), small[0].begin(), small[0].end());
-  itc.insert(itc.end(), small[1].begin(), small[1].end());
-  g->info.chunks = (int64_t)(itf.size() + itc.size());
+  }
+  {
+    std::vector<int32_t> small[kClasses];
+    for (int64_t x = 0; x < n; ++x)
+      if (deg[(size_t)x] > 0 && deg[(size_t)x] <= chunk_stride(depth[(size_t)x]))
+        small[depth[(size_t)x]].push_back((int32_t)x);
+    for (int c = 0; c < kClasses; ++c) {
+      std::stable_sort(small[c].begin(), small[c].end(), [&](int32_t a, int32_t b) {
+        return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
+      });
+      items[c].insert(items[c].end(), small[c].begin(), small[c].end());
+      g->n_items[c] = (uint32_t)items[c].size();
+      total_items += (int64_t)items[c].size();
+      if ((rc = g->upload(g->items[c], items[c]))) return rc;
+    }
+  }
+  g->info.chunks = total_items;
   g->h_items.assign((size_t)n, 0);
   for (int64_t x = 0; x < n; ++x) {
     const int32_t v = vinfo[(size_t)x];
     g->h_items[(size_t)x] = v == kEmpty ? 0 : (v < 0 ? 1 : nchunk[(size_t)v]);
   }
-  g->n_flat = (uint32_t)itf.size();
-  g->n_chain = (uint32_t)itc.size();
-  if ((rc = g->upload(g->items_flat, itf)) || (rc = g->upload(g->items_chain, itc)) ||
-      (rc = g->upload(g->vinfo, vinfo)))
-    return rc;
   return TGA_OK;
 }
 
@@ -989,7 +1133,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipSetDevice failed"));
   g->device = device;
-  if (hipStreamCreate(&g->stream) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipStreamCreate failed"));
+  if ((rc = g->create_streams())) return bail(rc);
   g->n = n;
   g->ab = (int32_t)std::max<int64_t>(0, desc ? desc->active_begin : 0);
   g->ae = (int32_t)((desc && desc->active_end > 0) ? std::min<int64_t>(desc->active_end, n) : n);
@@ -1019,8 +1163,9 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
 
   const size_t qn = (size_t)n + (size_t)g->nck;
   if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
-      (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq_flat, qn)) ||
-      (rc = g->alloc(g->xq_chain, qn)) || (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
+      (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq.q[0], qn)) ||
+      (rc = g->alloc(g->xq.q[1], qn)) || (rc = g->alloc(g->xq.q[2], qn)) || (rc = g->alloc(g->xq.q[3], qn)) ||
+      (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
       (rc = g->alloc(g->u64s, 4)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
       (rc = g->alloc(g->scal, 4)) || (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
     return bail(rc);
@@ -1070,26 +1215,38 @@ extern "C" int tga_info(const tga_graph* g, tga_graph_info* info) {
 
 static bool bad_window(tga_window w) { return w.start > w.end; }
 
-// Launches kernel K<0> over flat items and K<kMaxDepth> over chain items.
-#define LAUNCH_BOTH(K, IF, IC, ...)                                                               \
-  do {                                                                                            \
-    const Items if_ = (IF), ic_ = (IC);                                                           \
-    if (if_.count > 0) {                                                                          \
-      g->kbegin();                                                                                \
-      K<0><<<g->grid(if_.count), kBlock, 0, g->stream>>>(g->S, g->ck, if_, __VA_ARGS__);          \
-      g->kend();                                                                                  \
-    }                                                                                             \
-    if (ic_.count > 0) {                                                                          \
-      g->kbegin();                                                                                \
-      K<kMaxDepth><<<g->grid(ic_.count), kBlock, 0, g->stream>>>(g->S, g->ck, ic_, __VA_ARGS__);  \
-      g->kend();                                                                                  \
-    }                                                                                             \
-    HIP_TRY(hipGetLastError());                                                                   \
+// Launches kernel K<c> over the items of every class c with work, each class
+// on its own stream, and joins them back into the handle's stream.  One
+// profiled interval covers the whole pass.
+#define LAUNCH_CLASSES(K, ITEMS, ...)                                                       \
+  do {                                                                                      \
+    g->kbegin();                                                                            \
+    HIP_TRY(hipEventRecord(g->ev_fork, g->stream));                                         \
+    for (int c_ = 0; c_ < kClasses; ++c_) {                                                 \
+      const Items it_ = ITEMS(c_);                                                          \
+      if (it_.count == 0) continue;                                                         \
+      hipStream_t cs_ = g->cstream[c_];                                                     \
+      HIP_TRY(hipStreamWaitEvent(cs_, g->ev_fork, 0));                                      \
+      const int grid_ = g->grid(it_.count);                                                 \
+      if (c_ == 0) K<0><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      if (c_ == 1) K<1><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      if (c_ == 2) K<2><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      if (c_ == 3) K<3><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      HIP_TRY(hipEventRecord(g->ev_join[c_], cs_));                                         \
+      HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_join[c_], 0));                            \
+    }                                                                                       \
+    g->kend();                                                                              \
+    HIP_TRY(hipGetLastError());                                                             \
   } while (0)
 
-#define LAUNCH_STATIC(K, ...) LAUNCH_BOTH(K, g->flat_items(), g->chain_items(), __VA_ARGS__)
-#define LAUNCH_QUEUED(K, HC, ...) \
-  LAUNCH_BOTH(K, (Items{g->xq_flat, (HC)[0]}), (Items{g->xq_chain, (HC)[1]}), __VA_ARGS__)
+#define STATIC_ITEMS(c) (Items{g->items[c], g->n_items[c]})
+#define QUEUED_ITEMS(c) (Items{g->xq.q[c], hc_[c]})
+#define LAUNCH_STATIC(K, ...) LAUNCH_CLASSES(K, STATIC_ITEMS, __VA_ARGS__)
+#define LAUNCH_QUEUED(K, HC, ...)         \
+  do {                                    \
+    const uint32_t* hc_ = (HC);           \
+    LAUNCH_CLASSES(K, QUEUED_ITEMS, __VA_ARGS__); \
+  } while (0)
 
 extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
                                    int64_t* contacts) {
@@ -1110,7 +1267,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
         items += g->h_items[(size_t)verts[i1++]];
       HIP_TRY(hipMemcpyAsync(g->q0, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
                              g->stream));
-      uint32_t hc[3];
+      uint32_t hc[kClasses + 1];
       int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), false, false, hc);
       if (rc) return rc;
       LAUNCH_QUEUED(k_ra, hc, g->u64s);
@@ -1119,7 +1276,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   } else if (g->ab == 0 && g->ae == g->n) {
     LAUNCH_STATIC(k_ra, g->u64s);
   } else {
-    uint32_t hc[3];
+    uint32_t hc[kClasses + 1];
     int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), false, false, hc);
     if (rc) return rc;
     LAUNCH_QUEUED(k_ra, hc, g->u64s);
@@ -1146,11 +1303,14 @@ extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count,
   }
   LAUNCH_STATIC(k_dc, W, g->i0, centr);
   if (centr) {
-    if (g->n_big_flat)
-      k_dc_long<0><<<(g->n_big_flat + 63) / 64, 64, 0, g->stream>>>(g->S, g->big_flat, g->n_big_flat, W, centr);
-    if (g->n_big_chain)
-      k_dc_long<kMaxDepth><<<(g->n_big_chain + 63) / 64, 64, 0, g->stream>>>(g->S, g->big_chain, g->n_big_chain, W,
-                                                                              centr);
+    for (int c = 0; c < kClasses; ++c) {
+      if (!g->n_big[c]) continue;
+      const int blocks = (g->n_big[c] + 63) / 64;
+      if (c == 0) k_dc_long<0><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
+      if (c == 1) k_dc_long<1><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
+      if (c == 2) k_dc_long<2><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
+      if (c == 3) k_dc_long<3><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
+    }
     HIP_TRY(hipGetLastError());
   }
   int rc = g->out_copy(count, g->i0, (size_t)g->n * 4, flags);
@@ -1170,12 +1330,12 @@ extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, 
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t lvl = 0; fsize > 0; ++lvl) {
-    uint32_t hc[3];
+    uint32_t hc[kClasses + 1];
     int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_bfs, hc, W, g->i0, nxt, g->cnt + 4, lvl);
-    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_bfs, hc, W, g->i0, nxt, g->cnt + 6, lvl);
+    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
     std::swap(cur, nxt);
   }
   int rc = g->out_copy(level, g->i0, (size_t)g->n * 4, flags);
@@ -1196,12 +1356,17 @@ extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t round = 0; fsize > 0; ++round) {
-    uint32_t hc[3];
+    uint32_t hc[kClasses + 1];
     int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_ea, hc, W, g->i0, g->i1, round, nxt, g->cnt + 4);
-    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_ea, hc, W, g->i0, g->i1, round, nxt, g->cnt + 6, g->runkey);
+    if (g->nck > 0) {
+      k_ea_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->i0, g->i1, round,
+                                                          nxt, g->cnt + 6);
+      HIP_TRY(hipGetLastError());
+    }
+    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
     std::swap(cur, nxt);
   }
   int rc = g->out_copy(arrival, g->i0, (size_t)g->n * 4, flags);
@@ -1217,12 +1382,12 @@ extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* lab
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t fsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; fsize > 0; ++round) {
-    uint32_t hc[3];
+    uint32_t hc[kClasses + 1];
     int rc = g->expand(cur, g->ab, fsize, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round, nxt, g->cnt + 4);
-    if ((rc = g->read_counts(g->cnt + 4, &fsize, 1))) return rc;
+    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round, nxt, g->cnt + 6);
+    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
     cur = nxt;
     nxt = (nxt == g->q0) ? g->q1 : g->q0;
   }
@@ -1277,17 +1442,17 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t wsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; round < rounds && wsize > 0; ++round) {
-    uint32_t hc[3];
+    uint32_t hc[kClasses + 1];
     int rc = g->expand(cur, g->ab, wsize, true, true, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 4, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round, nxt, g->cnt + 4);
-    if (hc[2] > 0) {
-      k_gc_finish<<<g->grid(hc[2]), kBlock, 0, g->stream>>>(g->xq_big, hc[2], W, g->i0, g->i2, round, nxt,
-                                                             g->cnt + 4);
+    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round, nxt, g->cnt + 6);
+    if (hc[kClasses] > 0) {
+      k_gc_finish<<<g->grid(hc[kClasses]), kBlock, 0, g->stream>>>(g->xq_big, hc[kClasses], W, g->i0, g->i2, round,
+                                                                   nxt, g->cnt + 6);
       HIP_TRY(hipGetLastError());
     }
-    if ((rc = g->read_counts(g->cnt + 4, &wsize, 1))) return rc;
+    if ((rc = g->read_counts(g->cnt + 6, &wsize, 1))) return rc;
     cur = nxt;
     nxt = (nxt == g->q0) ? g->q1 : g->q0;
   }
@@ -1328,7 +1493,10 @@ extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float dam
   int32_t it = 0;
   for (; it < iters; ++it) {
     k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
-    LAUNCH_STATIC(k_pr_push, W, g->f0, du, damping, g->acc);
+    LAUNCH_STATIC(k_pr_push, W, g->f0, du, damping, g->acc, g->runkey);
+    if (g->nck > 0)
+      k_pr_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, du, damping,
+                                                          g->acc);
     int rc = reduce(g, 0, g->acc, nullptr, nullptr, g->scal, false);
     if (!rc && tol > 0) rc = reduce(g, 2, g->acc, g->f0, g->scal, g->scal + 1, false);
     if (rc) return rc;
@@ -1358,12 +1526,16 @@ extern "C" int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, f
   k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f1, 1.0f, 0.0f);  // auth
   for (int32_t it = 0; it < iters; ++it) {
     k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
-    LAUNCH_STATIC(k_hits_pull, W, g->f1, g->acc);
+    LAUNCH_STATIC(k_hits_pull, W, g->f1, g->acc, g->runkey);
+    if (g->nck > 0)
+      k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f1, g->acc, 0);
     int rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
     k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f0);
     k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
-    LAUNCH_STATIC(k_hits_push, W, g->f0, g->acc);
+    LAUNCH_STATIC(k_hits_push, W, g->f0, g->acc, g->runkey);
+    if (g->nck > 0)
+      k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, g->acc, 1);
     rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
     k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f1);
