@@ -73,11 +73,26 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per decode launch from rocprof FETCH_SIZE + WRITE_SIZE")
+    ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
+
+    if args.calibrate_copy is not None:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        a = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {nbytes} bytes three times")
+        return 0
 
     from compresstemporalgraph_amd import datasets
     from compresstemporalgraph_amd.engine import TemporalGraph

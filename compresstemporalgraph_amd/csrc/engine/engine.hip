@@ -28,6 +28,7 @@
 #include <fstream>
 #include <numeric>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/tga.h"
@@ -53,16 +54,22 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kBlock = 256;
 
-// Work items are split into classes by the reference-chain depth of their
-// record (0: no reference ... 3), each decoded by a kernel holding exactly the
-// decoder levels it needs.
+// Records fall into classes by their reference-chain depth (0: no reference
+// ... 3); the class fixes the chunk stride of long records.  A pass launches
+// two groups of work items: flat records (class 0) with the one-level
+// decoder, and chain records (classes 1-3) with the full-depth decoder.  Two
+// concurrent launches plus the handle's stream stay within the hardware
+// queues a process gets by default (4), so no launch of a pass waits behind
+// another on a shared queue.
 constexpr int kClasses = kMaxDepth + 1;
+constexpr int kGroups = 2;
+__host__ __device__ constexpr int group_of(int depth) { return depth > 0 ? 1 : 0; }
 // vinfo[x]: whole record of depth d -> -1 - d; no contacts -> kEmpty;
 // chunked record -> index of its first checkpoint.
 constexpr int32_t kEmpty = -1 - kClasses;
 
 struct Queues {
-  int32_t* q[kClasses];
+  int32_t* q[kGroups];
 };
 
 struct Window {
@@ -170,9 +177,9 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
 }
 
 // Expands a vertex set (an explicit list, or the range [base, base + count))
-// into per-class item queues; cnt[c] counts class c.  keep_empty keeps
-// records without contacts (graph coloring colors them) in class 0.  Chunked
-// records also go to bigq (counted in cnt[kClasses]) when given.
+// into per-group item queues; cnt[g] counts group g.  keep_empty keeps
+// records without contacts (graph coloring colors them) in group 0.  Chunked
+// records also go to bigq (counted in cnt[kGroups]) when given.
 __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t base, uint32_t count,
                                                    const int32_t* vinfo, const Ckpt* ck, Queues Q, uint32_t* cnt,
                                                    int32_t* bigq, int keep_empty) {
@@ -182,15 +189,15 @@ __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t
     if (info == kEmpty) {
       if (keep_empty) push(Q.q[0], &cnt[0], x);
     } else if (info < 0) {
-      const int32_t cls = -1 - info;
-      push(Q.q[cls], &cnt[cls], x);
+      const int grp = group_of(-1 - info);
+      push(Q.q[grp], &cnt[grp], x);
     } else {
       const Ckpt* c = ck + info;
       const int32_t nch = c->nchunk;
-      const int32_t cls = c->depth;
-      const uint32_t at = atomicAdd(&cnt[cls], (uint32_t)nch);
-      for (int32_t j = 0; j < nch; ++j) Q.q[cls][at + j] = -(info + j) - 1;
-      if (bigq) push(bigq, &cnt[kClasses], x);
+      const int grp = group_of(c->depth);
+      const uint32_t at = atomicAdd(&cnt[grp], (uint32_t)nch);
+      for (int32_t j = 0; j < nch; ++j) Q.q[grp][at + j] = -(info + j) - 1;
+      if (bigq) push(bigq, &cnt[kGroups], x);
     }
   }
 }
@@ -785,11 +792,11 @@ void build_select(const std::vector<uint64_t>& up, size_t words, uint64_t& ones,
 struct tga_graph {
   int device = 0;
   hipStream_t stream = nullptr;
-  // The classes of one pass run concurrently on their own streams (default
+  // The groups of one pass run concurrently on their own streams (default
   // priority), forked from and joined back into `stream` with events.
-  hipStream_t cstream[kClasses] = {};
+  hipStream_t cstream[kGroups] = {};
   hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[kClasses] = {};
+  hipEvent_t ev_join[kGroups] = {};
   int64_t n = 0;
   int32_t ab = 0, ae = 0;
   int32_t min_ts = 0;
@@ -803,13 +810,13 @@ struct tga_graph {
   uint64_t* runkey = nullptr;  // per checkpoint: split-run first window contact (~0: none)
   int32_t* vinfo = nullptr;  // per vertex item code (see kEmpty)
   int32_t* vdeg = nullptr;   // per vertex outdegree
-  int32_t* items[kClasses] = {};  // static item lists by class
-  uint32_t n_items[kClasses] = {};
+  int32_t* items[kGroups] = {};  // static item lists by group
+  uint32_t n_items[kGroups] = {};
   int32_t* big[kClasses] = {};  // chunked records by class
   int32_t n_big[kClasses] = {};
   // scratch (n-sized unless stated)
   int32_t *q0 = nullptr, *q1 = nullptr, *i0 = nullptr, *i1 = nullptr, *i2 = nullptr;
-  Queues xq{};                // expanded item queues by class
+  Queues xq{};                // expanded item queues by group
   int32_t* xq_big = nullptr;  // chunked records of the expanded set
   uint32_t* cnt = nullptr;  // 8 counters
   unsigned long long* u64s = nullptr;  // 4 values
@@ -908,20 +915,20 @@ struct tga_graph {
     HIP_TRY(hipStreamSynchronize(stream));
     return TGA_OK;
   }
-  // Expands a vertex set into the class queues (and xq_big); hc[c] receives
-  // the count of class c and hc[kClasses] the number of chunked records.
+  // Expands a vertex set into the group queues (and xq_big); hc[g] receives
+  // the count of group g and hc[kGroups] the number of chunked records.
   int expand(const int32_t* front, int32_t base, uint32_t count, bool keep_empty, bool want_big,
-             uint32_t hc[kClasses + 1]) {
-    HIP_TRY(hipMemsetAsync(cnt, 0, (kClasses + 1) * sizeof(uint32_t), stream));
+             uint32_t hc[kGroups + 1]) {
+    HIP_TRY(hipMemsetAsync(cnt, 0, (kGroups + 1) * sizeof(uint32_t), stream));
     k_expand<<<grid(count), kBlock, 0, stream>>>(front, base, count, vinfo, ck, xq, cnt,
                                                  want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
     HIP_TRY(hipGetLastError());
-    return read_counts(cnt, hc, kClasses + 1);
+    return read_counts(cnt, hc, kGroups + 1);
   }
   int create_streams() {
     HIP_TRY(hipStreamCreate(&stream));
     HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    for (int c = 0; c < kClasses; ++c) {
+    for (int c = 0; c < kGroups; ++c) {
       HIP_TRY(hipStreamCreate(&cstream[c]));
       HIP_TRY(hipEventCreateWithFlags(&ev_join[c], hipEventDisableTiming));
     }
@@ -931,7 +938,7 @@ struct tga_graph {
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
-    for (int c = 0; c < kClasses; ++c) {
+    for (int c = 0; c < kGroups; ++c) {
       if (ev_join[c]) (void)hipEventDestroy(ev_join[c]);
       if (cstream[c]) (void)hipStreamDestroy(cstream[c]);
     }
@@ -1033,36 +1040,59 @@ static int build_schedule(tga_graph* g) {
     HIP_TRY(hipMemcpyAsync(nch_host.data(), d_nch, nch_host.size() * 4, hipMemcpyDeviceToHost, g->stream));
   if ((rc = g->check_err())) return rc;
 
-  // Item code of every vertex, and the static item lists: chunks first, then
-  // whole records longest first (within a length class by id, for index
-  // locality).
+  // Static item lists.  Flat group: chunks first, then whole records longest
+  // first (within a length class by id, for index locality).  Chain group:
+  // chunks and whole records together, ordered by an estimate of their decode
+  // latency (contacts times a per-level factor, deeper records pulling each
+  // copied value through more levels), longest first.
   std::vector<int32_t> nchunk(nslots, 0);
-  std::vector<int32_t> items[kClasses];
-  int64_t total_items = 0;
+  std::vector<int32_t> items[kGroups];
+  std::vector<std::pair<int64_t, int32_t>> chain;  // (cost, code)
+  auto chain_cost = [](int64_t contacts, int depth) { return contacts * (1 + 2 * depth); };
   for (int c = 0; c < kClasses; ++c) {
+    const int32_t stride = chunk_stride(c);
     for (size_t i = 0; i < bigv[c].size(); ++i) {
       const int32_t x = bigv[c][i];
       const int32_t nch = nch_host[nch_at[c] + i];
-      if (nch != (deg[(size_t)x] + chunk_stride(c) - 1) / chunk_stride(c))
+      if (nch != (deg[(size_t)x] + stride - 1) / stride)
         return fail(TGA_EFORMAT, "checkpoint build failed at vertex " + std::to_string(x));
       nchunk[(size_t)slotv[c][i]] = nch;
-      for (int32_t j = 0; j < nch; ++j) items[c].push_back(-(int32_t)(slotv[c][i] + j) - 1);
+      for (int32_t j = 0; j < nch; ++j) {
+        const int32_t code = -(int32_t)(slotv[c][i] + j) - 1;
+        if (c == 0) {
+          items[0].push_back(code);
+        } else {
+          const int64_t len = std::min<int64_t>(stride, deg[(size_t)x] - (int64_t)j * stride);
+          chain.emplace_back(chain_cost(len, c), code);
+        }
+      }
     }
   }
   {
-    std::vector<int32_t> small[kClasses];
-    for (int64_t x = 0; x < n; ++x)
-      if (deg[(size_t)x] > 0 && deg[(size_t)x] <= chunk_stride(depth[(size_t)x]))
-        small[depth[(size_t)x]].push_back((int32_t)x);
-    for (int c = 0; c < kClasses; ++c) {
-      std::stable_sort(small[c].begin(), small[c].end(), [&](int32_t a, int32_t b) {
-        return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
-      });
-      items[c].insert(items[c].end(), small[c].begin(), small[c].end());
-      g->n_items[c] = (uint32_t)items[c].size();
-      total_items += (int64_t)items[c].size();
-      if ((rc = g->upload(g->items[c], items[c]))) return rc;
+    std::vector<int32_t> small;
+    for (int64_t x = 0; x < n; ++x) {
+      const int c = depth[(size_t)x];
+      if (deg[(size_t)x] <= 0 || deg[(size_t)x] > chunk_stride(c)) continue;
+      if (c == 0)
+        small.push_back((int32_t)x);
+      else
+        chain.emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
     }
+    std::stable_sort(small.begin(), small.end(), [&](int32_t a, int32_t b) {
+      return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
+    });
+    items[0].insert(items[0].end(), small.begin(), small.end());
+    std::stable_sort(chain.begin(), chain.end(),
+                     [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
+                       return a.first > b.first;
+                     });
+    for (const auto& e : chain) items[1].push_back(e.second);
+  }
+  int64_t total_items = 0;
+  for (int c = 0; c < kGroups; ++c) {
+    g->n_items[c] = (uint32_t)items[c].size();
+    total_items += (int64_t)items[c].size();
+    if ((rc = g->upload(g->items[c], items[c]))) return rc;
   }
   g->info.chunks = total_items;
   g->h_items.assign((size_t)n, 0);
@@ -1164,7 +1194,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   const size_t qn = (size_t)n + (size_t)g->nck;
   if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
       (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq.q[0], qn)) ||
-      (rc = g->alloc(g->xq.q[1], qn)) || (rc = g->alloc(g->xq.q[2], qn)) || (rc = g->alloc(g->xq.q[3], qn)) ||
+      (rc = g->alloc(g->xq.q[1], qn)) ||
       (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
       (rc = g->alloc(g->u64s, 4)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
       (rc = g->alloc(g->scal, 4)) || (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
@@ -1215,23 +1245,21 @@ extern "C" int tga_info(const tga_graph* g, tga_graph_info* info) {
 
 static bool bad_window(tga_window w) { return w.start > w.end; }
 
-// Launches kernel K<c> over the items of every class c with work, each class
-// on its own stream, and joins them back into the handle's stream.  One
-// profiled interval covers the whole pass.
+// Launches kernel K over the items of both groups (K<0> for flat records,
+// K<kMaxDepth> for chain records), each group on its own stream, and joins
+// them back into the handle's stream.  One profiled interval covers the pass.
 #define LAUNCH_CLASSES(K, ITEMS, ...)                                                       \
   do {                                                                                      \
     g->kbegin();                                                                            \
     HIP_TRY(hipEventRecord(g->ev_fork, g->stream));                                         \
-    for (int c_ = 0; c_ < kClasses; ++c_) {                                                 \
+    for (int c_ = kGroups - 1; c_ >= 0; --c_) {                                             \
       const Items it_ = ITEMS(c_);                                                          \
       if (it_.count == 0) continue;                                                         \
       hipStream_t cs_ = g->cstream[c_];                                                     \
       HIP_TRY(hipStreamWaitEvent(cs_, g->ev_fork, 0));                                      \
       const int grid_ = g->grid(it_.count);                                                 \
       if (c_ == 0) K<0><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
-      if (c_ == 1) K<1><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
-      if (c_ == 2) K<2><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
-      if (c_ == 3) K<3><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      if (c_ == 1) K<kMaxDepth><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);  \
       HIP_TRY(hipEventRecord(g->ev_join[c_], cs_));                                         \
       HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_join[c_], 0));                            \
     }                                                                                       \
@@ -1267,7 +1295,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
         items += g->h_items[(size_t)verts[i1++]];
       HIP_TRY(hipMemcpyAsync(g->q0, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
                              g->stream));
-      uint32_t hc[kClasses + 1];
+      uint32_t hc[kGroups + 1];
       int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), false, false, hc);
       if (rc) return rc;
       LAUNCH_QUEUED(k_ra, hc, g->u64s);
@@ -1276,7 +1304,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   } else if (g->ab == 0 && g->ae == g->n) {
     LAUNCH_STATIC(k_ra, g->u64s);
   } else {
-    uint32_t hc[kClasses + 1];
+    uint32_t hc[kGroups + 1];
     int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), false, false, hc);
     if (rc) return rc;
     LAUNCH_QUEUED(k_ra, hc, g->u64s);
@@ -1330,7 +1358,7 @@ extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, 
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t lvl = 0; fsize > 0; ++lvl) {
-    uint32_t hc[kClasses + 1];
+    uint32_t hc[kGroups + 1];
     int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
@@ -1356,7 +1384,7 @@ extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int
   int32_t *cur = g->q0, *nxt = g->q1;
   uint32_t fsize = 1;
   for (int32_t round = 0; fsize > 0; ++round) {
-    uint32_t hc[kClasses + 1];
+    uint32_t hc[kGroups + 1];
     int rc = g->expand(cur, 0, fsize, false, false, hc);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
@@ -1382,7 +1410,7 @@ extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* lab
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t fsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; fsize > 0; ++round) {
-    uint32_t hc[kClasses + 1];
+    uint32_t hc[kGroups + 1];
     int rc = g->expand(cur, g->ab, fsize, false, false, hc);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
@@ -1442,13 +1470,13 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
   int32_t *cur = nullptr, *nxt = g->q0;
   uint32_t wsize = (uint32_t)(g->ae - g->ab);
   for (int32_t round = 0; round < rounds && wsize > 0; ++round) {
-    uint32_t hc[kClasses + 1];
+    uint32_t hc[kGroups + 1];
     int rc = g->expand(cur, g->ab, wsize, true, true, hc);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
     LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round, nxt, g->cnt + 6);
-    if (hc[kClasses] > 0) {
-      k_gc_finish<<<g->grid(hc[kClasses]), kBlock, 0, g->stream>>>(g->xq_big, hc[kClasses], W, g->i0, g->i2, round,
+    if (hc[kGroups] > 0) {
+      k_gc_finish<<<g->grid(hc[kGroups]), kBlock, 0, g->stream>>>(g->xq_big, hc[kGroups], W, g->i0, g->i2, round,
                                                                    nxt, g->cnt + 6);
       HIP_TRY(hipGetLastError());
     }
