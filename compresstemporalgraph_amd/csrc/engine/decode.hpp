@@ -17,10 +17,11 @@
 // Codes are decoded arithmetically from a 64-bit window with a count of
 // leading zeros instead of the reference's 16-bit lookup tables
 // (bfs.cu:202-256, :590-639); every codeword this format produces is at most
-// 51 bits long (zeta_2 of an int2nat'ed int32 difference).  The two hot
-// sub-streams (timestamps, and the residuals of the top-level list) are read
-// through a register bit buffer that loads one 64-bit word per 64 bits
-// consumed; the rarer sub-streams use two-word peeks.
+// 51 bits long (zeta_2 of an int2nat'ed int32 difference).  The sequential
+// sub-streams (timestamps, record headers, and the residuals and multiples of
+// every level) are read through register bit buffers that load one 64-bit
+// word per 64 bits consumed, ahead of use; the rare ones (intervals, copy
+// blocks) use two-word peeks.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -61,6 +62,7 @@ struct Streams {
   const uint64_t* g;  // graph stream, byte-swapped words (+3 zero words)
   const uint64_t* t;  // timestamp stream, byte-swapped words (+3 zero words)
   uint64_t g_bits, t_bits;
+  uint32_t g_last, t_last;  // index of the last padding word of each stream
   EFDir gi, ti;
   int32_t n;       // vertices; ids are [0, n)
   int32_t min_ts;  // .timestamps header value
@@ -165,30 +167,35 @@ __device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& po
   return v;
 }
 
-// Register bit buffer over one stream: words a = w[idx], b = w[idx + 1] and
-// c = w[idx + 2] are held, `sh` is the bit offset into a.  Advancing past a
-// loads the word after c, which is needed only 64 or more bits later, so the
-// load's latency overlaps the decoding of the codewords in between.  Stream
-// word arrays carry three zero words of padding; reads past `last` return 0.
+// Register bit buffer over one sequentially read sub-stream: words
+// a = w[idx], b = w[idx + 1] and c = w[idx + 2] are held, `sh` is the bit
+// offset into a.  Advancing past a loads the word after c, which is needed
+// only 64 or more bits later, so the load's latency overlaps the decoding of
+// the codewords in between.  The stream pointer and the index of its last
+// padding word are passed in (they are the same for every buffer over one
+// stream); stream word arrays carry three zero words of padding, and reads
+// past `last` return 0 (every code then fails its length check).
 struct BitBuf {
-  const uint64_t* w;
-  uint64_t idx;
   uint64_t a, b, c;
-  uint32_t sh;
-  uint32_t last;  // index of the last padding word
+  uint32_t idx;  // word index of a
+  uint32_t sh;   // bit offset into a
 
-  __device__ __forceinline__ void init(const uint64_t* words, uint64_t nbits, uint64_t pos) {
-    w = words;
-    last = (uint32_t)min((((nbits >> 3) + 7) >> 3) + 2, (uint64_t)0xffffffffu);
-    idx = pos >> 6;
-    if (idx + 2 > last) idx = last - 2;
+  __device__ __forceinline__ void init(const uint64_t* w, uint32_t last, uint64_t pos) {
+    uint64_t i = pos >> 6;
+    if (i + 2 > last) i = last - 2;
+    idx = (uint32_t)i;
     sh = (uint32_t)pos & 63u;
     a = w[idx];
     b = w[idx + 1];
     c = w[idx + 2];
   }
+  // A buffer that will not be read (its position is recorded as 0).
+  __device__ __forceinline__ void clear() {
+    a = b = c = 0;
+    idx = sh = 0;
+  }
   __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
-  __device__ __forceinline__ void skip(uint32_t n) {  // n <= 64
+  __device__ __forceinline__ void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
     sh += n;
     if (sh >= 64) {
       sh -= 64;
@@ -198,14 +205,41 @@ struct BitBuf {
       c = (idx + 2 <= last) ? w[idx + 2] : 0;
     }
   }
-  __device__ __forceinline__ uint64_t pos() const { return idx * 64 + sh; }
+  __device__ __forceinline__ uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
 };
 
+// Buffered readers of the graph stream (g_*) and the timestamp stream (t_*).
+__device__ __forceinline__ void g_init(const Streams& S, BitBuf& b, uint64_t pos) {
+  if (pos >= S.g_bits) raise_err(S, kErrOverrun);
+  b.init(S.g, S.g_last, pos);
+}
+__device__ __forceinline__ uint64_t g_gamma(const Streams& S, BitBuf& b) {
+  uint32_t len;
+  const uint64_t v = dec_gamma(S, b.peek(), len);
+  b.skip(len, S.g, S.g_last);
+  return v;
+}
+__device__ __forceinline__ uint32_t g_unary(const Streams& S, BitBuf& b) {
+  uint32_t len;
+  const uint32_t v = dec_unary(S, b.peek(), len);
+  b.skip(len, S.g, S.g_last);
+  return v;
+}
 template <int K>
-__device__ __forceinline__ uint64_t bb_zeta(const Streams& S, BitBuf& b) {
+__device__ __forceinline__ uint64_t g_zeta(const Streams& S, BitBuf& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<K>(S, b.peek(), len);
-  b.skip(len);
+  b.skip(len, S.g, S.g_last);
+  return v;
+}
+__device__ __forceinline__ void t_init(const Streams& S, BitBuf& b, uint64_t pos) {
+  if (pos >= S.t_bits) raise_err(S, kErrOverrun);
+  b.init(S.t, S.t_last, pos);
+}
+__device__ __forceinline__ uint64_t t_zeta2(const Streams& S, BitBuf& b) {
+  uint32_t len;
+  const uint64_t v = dec_zeta<2>(S, b.peek(), len);
+  b.skip(len, S.t, S.t_last);
   return v;
 }
 
@@ -280,10 +314,15 @@ struct Level {
   int32_t pad_;
 };
 
+// Residuals and multiples of every level are read through bit buffers (their
+// Level cursors rpos / mpos are stale while decoding and are written back
+// only into checkpoints); intervals and copy blocks, which are rare, are read
+// directly.
 template <int MAXD>
 struct Dec {
   Level lv[MAXD + 1];
-  BitBuf rb;  // residual cursor of level 0 (lv[0].rpos is stale while decoding)
+  BitBuf rb[MAXD + 1];  // residuals
+  BitBuf mb[MAXD + 1];  // multiples (and the record header while it is parsed)
 };
 
 // Saved decoder state at a chunk boundary of a long list (see engine.hip).
@@ -325,6 +364,21 @@ struct Split {
 template <int L, int MAXD>
 __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D);
 
+// Restores one level from a saved copy: the buffers of the sub-streams that
+// still have codes to read are refilled at their saved positions.
+__device__ __forceinline__ void lv_restore(const Streams& S, Level& v, BitBuf& rb, BitBuf& mb, const Level& src) {
+  v = src;
+  if (v.rleft > 0) g_init(S, rb, v.rpos); else rb.clear();
+  if (v.mleft > 0) g_init(S, mb, v.mpos); else mb.clear();
+}
+
+// Copies one level, with its sub-stream positions, into a checkpoint.
+__device__ __forceinline__ void lv_save(Level& dst, const Level& v, const BitBuf& rb, const BitBuf& mb) {
+  dst = v;
+  dst.rpos = rb.pos();
+  dst.mpos = mb.pos();
+}
+
 // Skips toward value `target` of level LL by restoring the checkpoint of the
 // referenced record at or before it, when that record is long enough to have
 // checkpoints.  Returns how many values the jump passed over.
@@ -340,9 +394,10 @@ __device__ __forceinline__ int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int3
   const int32_t from = D.lv[LL].kpos;
   if (j * stride <= from) return 0;
   const Ckpt* c = S.ck + first + j;
+  const int32_t depth = c->depth;
 #pragma unroll
   for (int l = 0; LL + l <= MAXD; ++l)
-    if (l <= c->depth) D.lv[LL + l] = c->lv[l];
+    if (l <= depth) lv_restore(S, D.lv[LL + l], D.rb[LL + l], D.mb[LL + l], c->lv[l]);
   return D.lv[LL].kpos - from;
 }
 
@@ -397,13 +452,7 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   ++v.kpos;
   if (m == v.hr) {
     if (v.rleft > 0) {
-      uint64_t z;
-      if constexpr (L == 0) {
-        z = bb_zeta<3>(S, D.rb);
-      } else {
-        z = rd_zeta<3>(S, g, v.rpos);
-      }
-      v.hr += (int32_t)z + 1;
+      v.hr += (int32_t)g_zeta<3>(S, D.rb[L]) + 1;
       --v.rleft;
     } else {
       v.hr = kInf;
@@ -411,8 +460,8 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   } else if (m == v.hm) {
     if (--v.mrem == 0) {
       if (v.mleft > 0) {
-        v.hm += (int32_t)rd_gamma(S, g, v.mpos) + 1;
-        v.mrem = (int32_t)rd_gamma(S, g, v.mpos) + 2;
+        v.hm += (int32_t)g_gamma(S, D.mb[L]) + 1;
+        v.mrem = (int32_t)g_gamma(S, D.mb[L]) + 2;
         --v.mleft;
       } else {
         v.hm = kInf;
@@ -450,7 +499,10 @@ template <int L, int MAXD>
 __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos,
                                             int32_t& depth) {
   Level& v = D.lv[L];
-  const Rd g{S.g, S.g_bits};
+  // The header is parsed through the residual buffer, which ends up at the
+  // first residual; when the record has multiples, the multiples buffer is
+  // forked from it at the first run.
+  BitBuf& h = D.rb[L];
   v.x = x;
   v.hm = v.hi = v.hr = v.hc = kInf;
   v.mrem = v.mleft = v.irem = v.ileft = v.rleft = 0;
@@ -458,11 +510,12 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   v.kpos = 0;
   v.bidx = -1;
   v.mpos = v.ipos = v.rpos = v.bpos = gpos;
-  uint64_t pos = gpos;
-  const int32_t d = (int32_t)rd_gamma(S, g, pos);
+  D.mb[L].clear();
+  g_init(S, h, gpos);
+  const int32_t d = (int32_t)g_gamma(S, h);
   if (d <= 0) return 0;
   if (L > depth) depth = L;
-  const uint32_t r = rd_unary(S, g, pos);
+  const uint32_t r = g_unary(S, h);
   int32_t copied = 0;
   if (r > 0) {
     if constexpr (L >= MAXD) {
@@ -471,15 +524,15 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
       if (r > (uint32_t)kWindow || (int32_t)r > x) {
         raise_err(S, kErrDepth);
       } else {
-        const int32_t bc = (int32_t)rd_gamma(S, g, pos);
-        v.bpos = pos;
+        const int32_t bc = (int32_t)g_gamma(S, h);
+        v.bpos = h.pos();
         int32_t total = 0, even = 0;
         for (int32_t i = 0; i < bc; ++i) {
-          if (pos >= S.g_bits) {  // malformed: stop at the end of the stream
+          if (h.pos() >= S.g_bits) {  // malformed: stop at the end of the stream
             raise_err(S, kErrOverrun);
             break;
           }
-          const int32_t b = (int32_t)rd_gamma(S, g, pos) + (i ? 1 : 0);
+          const int32_t b = (int32_t)g_gamma(S, h) + (i ? 1 : 0);
           total += b;
           if ((i & 1) == 0) even += b;
         }
@@ -498,34 +551,34 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   }
   int32_t extra = d - copied;
   if (extra > 0) {  // multiples
-    const int32_t mc = (int32_t)rd_gamma(S, g, pos);
+    const int32_t mc = (int32_t)g_gamma(S, h);
     if (mc > 0) {
-      uint64_t first = pos;
+      D.mb[L] = h;
       for (int32_t i = 0; i < mc; ++i) {
         if (extra <= 0) {  // more runs than contacts: malformed
           raise_err(S, kErrBlocks);
           break;
         }
-        rd_gamma(S, g, pos);
-        extra -= (int32_t)rd_gamma(S, g, pos) + 2;
+        g_gamma(S, h);
+        extra -= (int32_t)g_gamma(S, h) + 2;
       }
-      v.hm = (int32_t)(nat2int(rd_gamma(S, g, first)) + x);
-      v.mrem = (int32_t)rd_gamma(S, g, first) + 2;
+      v.hm = (int32_t)(nat2int(g_gamma(S, D.mb[L])) + x);
+      v.mrem = (int32_t)g_gamma(S, D.mb[L]) + 2;
       v.mleft = mc - 1;
-      v.mpos = first;
     }
   }
   if (extra > 0) {  // intervals
-    const int32_t ic = (int32_t)rd_gamma(S, g, pos);
+    const int32_t ic = (int32_t)g_gamma(S, h);
     if (ic > 0) {
-      uint64_t first = pos;
+      const Rd g{S.g, S.g_bits};
+      uint64_t first = h.pos();
       for (int32_t i = 0; i < ic; ++i) {
         if (extra <= 0) {  // more intervals than contacts: malformed
           raise_err(S, kErrBlocks);
           break;
         }
-        rd_gamma(S, g, pos);
-        extra -= (int32_t)rd_gamma(S, g, pos) + 4;
+        g_gamma(S, h);
+        extra -= (int32_t)g_gamma(S, h) + 4;
       }
       v.hi = (int32_t)(nat2int(rd_gamma(S, g, first)) + x);
       v.irem = (int32_t)rd_gamma(S, g, first) + 4;
@@ -534,11 +587,9 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
     }
   }
   if (extra > 0) {  // residuals
-    v.rpos = pos;
-    v.hr = (int32_t)(nat2int(rd_zeta<3>(S, g, v.rpos)) + x);
+    v.hr = (int32_t)(nat2int(g_zeta<3>(S, h)) + x);
     v.rleft = extra - 1;
   }
-  if constexpr (L == 0) D.rb.init(S.g, S.g_bits, v.rpos);
   return d;
 }
 
@@ -550,7 +601,7 @@ struct TsCursor {
 };
 
 __device__ __forceinline__ int32_t ts_next(const Streams& S, TsCursor& c) {
-  c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(bb_zeta<2>(S, c.b)));
+  c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(t_zeta2(S, c.b)));
   return c.prev;
 }
 
@@ -564,22 +615,19 @@ __device__ __forceinline__ void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t d
   c->tpos = tc.b.pos();
   c->tprev = tc.prev;
 #pragma unroll
-  for (int l = 0; l <= MAXD; ++l) c->lv[l] = D.lv[l];
-  c->lv[0].rpos = D.rb.pos();
+  for (int l = 0; l <= MAXD; ++l) lv_save(c->lv[l], D.lv[l], D.rb[l], D.mb[l]);
 }
 
 template <int MAXD>
 __device__ __forceinline__ void ckpt_load(const Streams& S, const Ckpt* c, Dec<MAXD>& D, TsCursor& tc) {
+  const int32_t depth = c->depth;
 #pragma unroll
   for (int l = 0; l <= MAXD; ++l)
-    if (l <= c->depth) D.lv[l] = c->lv[l];
-  D.rb.init(S.g, S.g_bits, D.lv[0].rpos);
-  tc.b.init(S.t, S.t_bits, c->tpos);
+    if (l <= depth) lv_restore(S, D.lv[l], D.rb[l], D.mb[l], c->lv[l]);
+  t_init(S, tc.b, c->tpos);
   tc.prev = c->tprev;
 }
 
-// Walks contacts [k, kend) of one work item (a whole record or a chunk) and
-// hands (x, r, t) to act.contact(); a false return stops the walk.
 // Walks contacts [k, kend) of one work item and hands (x, r, t, k) to
 // act.contact(); a false return stops the walk.
 template <int MAXD, class Act>
@@ -614,7 +662,7 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
       act.end(x, false);
       return;
     }
-    tc.b.init(S.t, S.t_bits, ef_get(S, S.ti, (uint64_t)x));
+    t_init(S, tc.b, ef_get(S, S.ti, (uint64_t)x));
     tc.prev = S.min_ts;
   } else {
     const Ckpt* c = ck + (-code - 1);

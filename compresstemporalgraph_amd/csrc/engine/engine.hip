@@ -10,13 +10,13 @@
 //   3. decodes every record longer than its chunk stride once, sequentially,
 //      saving the decoder state every stride contacts (runs of one neighbor
 //      that cross a boundary are tracked as split runs, see Ckpt), and
-//   4. builds two item lists, one for flat records (no reference: a
-//      one-level decoder with few registers) and one for records with a
-//      reference chain (a four-level decoder), each ordered by decreasing
-//      cost so waves get items of similar length.
-// Full passes (RA, DC, PR, HITS) launch over the two lists.  Frontier
-// algorithms (BFS, CC, GC, earliest arrival) expand their vertex frontier into
-// flat/chain item queues each round; the host reads a few 4-byte counts per
+//   4. builds three item lists: flat records (no reference: a one-level
+//      decoder with few registers), depth-1 records (two levels) and deeper
+//      records (four levels), each ordered by decreasing cost so waves get
+//      items of similar length.
+// Full passes (RA, DC, PR, HITS) launch over the three lists concurrently.
+// Frontier algorithms (BFS, CC, GC, earliest arrival) expand their vertex
+// frontier into per-group item queues each round; the host reads a few 4-byte counts per
 // round to size the next launches (the reference copies the queue size back
 // per level too, bfs.cu:2028).
 #include <hip/hip_runtime.h>
@@ -56,14 +56,14 @@ constexpr int kBlock = 256;
 
 // Records fall into classes by their reference-chain depth (0: no reference
 // ... 3); the class fixes the chunk stride of long records.  A pass launches
-// two groups of work items: flat records (class 0) with the one-level
-// decoder, and chain records (classes 1-3) with the full-depth decoder.  Two
-// concurrent launches plus the handle's stream stay within the hardware
-// queues a process gets by default (4), so no launch of a pass waits behind
-// another on a shared queue.
+// three groups of work items, each with a decoder of the levels it needs:
+// flat records (class 0, one level), depth-1 records (two levels) and deeper
+// ones (four levels, few records).  Three concurrent launches plus the
+// handle's stream stay within the hardware queues a process gets by default
+// (4), so no launch of a pass waits behind another on a shared queue.
 constexpr int kClasses = kMaxDepth + 1;
-constexpr int kGroups = 2;
-__host__ __device__ constexpr int group_of(int depth) { return depth > 0 ? 1 : 0; }
+constexpr int kGroups = 3;
+__host__ __device__ constexpr int group_of(int depth) { return depth < 2 ? depth : 2; }
 // vinfo[x]: whole record of depth d -> -1 - d; no contacts -> kEmpty;
 // chunked record -> index of its first checkpoint.
 constexpr int32_t kEmpty = -1 - kClasses;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
   TsCursor tc;
   int32_t depth = 0;
   const int32_t d = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
-  tc.b.init(S.t, S.t_bits, ef_get(S, S.ti, (uint64_t)x));
+  t_init(S, tc.b, ef_get(S, S.ti, (uint64_t)x));
   tc.prev = S.min_ts;
   ckpt_save<MAXD>(out, D, depth, tc, x, 0);
   out[0].cont_slot = -1;
@@ -1041,13 +1041,13 @@ static int build_schedule(tga_graph* g) {
   if ((rc = g->check_err())) return rc;
 
   // Static item lists.  Flat group: chunks first, then whole records longest
-  // first (within a length class by id, for index locality).  Chain group:
+  // first (within a length class by id, for index locality).  Chain groups:
   // chunks and whole records together, ordered by an estimate of their decode
   // latency (contacts times a per-level factor, deeper records pulling each
   // copied value through more levels), longest first.
   std::vector<int32_t> nchunk(nslots, 0);
   std::vector<int32_t> items[kGroups];
-  std::vector<std::pair<int64_t, int32_t>> chain;  // (cost, code)
+  std::vector<std::pair<int64_t, int32_t>> chain[kGroups];  // (cost, code)
   auto chain_cost = [](int64_t contacts, int depth) { return contacts * (1 + 2 * depth); };
   for (int c = 0; c < kClasses; ++c) {
     const int32_t stride = chunk_stride(c);
@@ -1063,7 +1063,7 @@ static int build_schedule(tga_graph* g) {
           items[0].push_back(code);
         } else {
           const int64_t len = std::min<int64_t>(stride, deg[(size_t)x] - (int64_t)j * stride);
-          chain.emplace_back(chain_cost(len, c), code);
+          chain[group_of(c)].emplace_back(chain_cost(len, c), code);
         }
       }
     }
@@ -1076,17 +1076,19 @@ static int build_schedule(tga_graph* g) {
       if (c == 0)
         small.push_back((int32_t)x);
       else
-        chain.emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
+        chain[group_of(c)].emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
     }
     std::stable_sort(small.begin(), small.end(), [&](int32_t a, int32_t b) {
       return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
     });
     items[0].insert(items[0].end(), small.begin(), small.end());
-    std::stable_sort(chain.begin(), chain.end(),
-                     [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
-                       return a.first > b.first;
-                     });
-    for (const auto& e : chain) items[1].push_back(e.second);
+    for (int c = 1; c < kGroups; ++c) {
+      std::stable_sort(chain[c].begin(), chain[c].end(),
+                       [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
+                         return a.first > b.first;
+                       });
+      for (const auto& e : chain[c]) items[c].push_back(e.second);
+    }
   }
   int64_t total_items = 0;
   for (int c = 0; c < kGroups; ++c) {
@@ -1179,6 +1181,11 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
     return bail(rc);
   S.g_bits = (uint64_t)graph.size() * 8;
   S.t_bits = (uint64_t)ts.size() * 8;
+  // Bit buffers index stream words with 32 bits (streams up to 32 GiB).
+  if (gw.size() >= 0xffffffffull || tw.size() >= 0xffffffffull)
+    return bail(fail(TGA_ERANGE, "stream longer than 2^32 words"));
+  S.g_last = (uint32_t)(gw.size() - 1);
+  S.t_last = (uint32_t)(tw.size() - 1);
   S.gi.upper_words = (gup.size() + 7) / 8;
   S.ti.upper_words = (tup.size() + 7) / 8;
   S.gi.l = lg;
@@ -1194,7 +1201,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   const size_t qn = (size_t)n + (size_t)g->nck;
   if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
       (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq.q[0], qn)) ||
-      (rc = g->alloc(g->xq.q[1], qn)) ||
+      (rc = g->alloc(g->xq.q[1], qn)) || (rc = g->alloc(g->xq.q[2], qn)) ||
       (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
       (rc = g->alloc(g->u64s, 4)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
       (rc = g->alloc(g->scal, 4)) || (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
@@ -1245,9 +1252,10 @@ extern "C" int tga_info(const tga_graph* g, tga_graph_info* info) {
 
 static bool bad_window(tga_window w) { return w.start > w.end; }
 
-// Launches kernel K over the items of both groups (K<0> for flat records,
-// K<kMaxDepth> for chain records), each group on its own stream, and joins
-// them back into the handle's stream.  One profiled interval covers the pass.
+// Launches kernel K over the items of every group (K<0> for flat records,
+// K<1> for depth-1 records, K<kMaxDepth> for deeper ones), each group on its
+// own stream, and joins them back into the handle's stream.  One profiled
+// interval covers the pass.
 #define LAUNCH_CLASSES(K, ITEMS, ...)                                                       \
   do {                                                                                      \
     g->kbegin();                                                                            \
@@ -1259,7 +1267,8 @@ static bool bad_window(tga_window w) { return w.start > w.end; }
       HIP_TRY(hipStreamWaitEvent(cs_, g->ev_fork, 0));                                      \
       const int grid_ = g->grid(it_.count);                                                 \
       if (c_ == 0) K<0><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
-      if (c_ == 1) K<kMaxDepth><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);  \
+      if (c_ == 1) K<1><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);          \
+      if (c_ == 2) K<kMaxDepth><<<grid_, kBlock, 0, cs_>>>(g->S, g->ck, it_, __VA_ARGS__);  \
       HIP_TRY(hipEventRecord(g->ev_join[c_], cs_));                                         \
       HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_join[c_], 0));                            \
     }                                                                                       \
