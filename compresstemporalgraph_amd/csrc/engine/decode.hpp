@@ -416,6 +416,9 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
         if (v.bleft > 0) {
           v.brem = (int32_t)rd_gamma(S, g, v.bpos) + (v.bidx ? 1 : 0);
           --v.bleft;
+          // A long skip block tries, once, to jump through the referenced
+          // record's checkpoints.
+          if ((v.bidx & 1) && v.brem > 32) v.brem -= dec_jump<L + 1, MAXD>(S, D, D.lv[L + 1].kpos + v.brem);
         } else if ((v.bidx & 1) == 0) {
           v.brem = kInf;  // implicit last block: copy the rest
         } else {
@@ -425,8 +428,6 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
         }
         continue;
       }
-      if ((v.bidx & 1) && v.brem > 32) v.brem -= dec_jump<L + 1, MAXD>(S, D, D.lv[L + 1].kpos + v.brem);
-      if (v.brem == 0) continue;
       const int32_t val = dec_next<L + 1, MAXD>(S, D);
       if (val == kInf) {
         raise_err(S, kErrBlocks);
