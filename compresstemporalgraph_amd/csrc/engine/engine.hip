@@ -102,6 +102,9 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Appends v to a device queue.
 __device__ __forceinline__ void push(int32_t* q, uint32_t* qn, int32_t v) { q[atomicAdd(qn, 1u)] = v; }
 
+// Accumulator pairs of the neighbor-access checksum and count.
+constexpr int kSumSlots = 256;
+
 // An item list and its length.
 struct Items {
   const int32_t* code;
@@ -221,9 +224,12 @@ __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items 
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
   const unsigned long long cs = wave_sum((unsigned long long)a.cs);
   const unsigned long long cnt = wave_sum((unsigned long long)a.cnt);
+  // Spread over kSumSlots accumulator pairs: tens of thousands of waves
+  // adding into one address would serialize on it.
   if ((threadIdx.x & 63) == 0 && cnt) {
-    atomicAdd(&out[0], cs);
-    atomicAdd(&out[1], cnt);
+    unsigned long long* o = out + 2 * (blockIdx.x & (kSumSlots - 1));
+    atomicAdd(&o[0], cs);
+    atomicAdd(&o[1], cnt);
   }
 }
 
@@ -819,7 +825,7 @@ struct tga_graph {
   Queues xq{};                // expanded item queues by group
   int32_t* xq_big = nullptr;  // chunked records of the expanded set
   uint32_t* cnt = nullptr;  // 8 counters
-  unsigned long long* u64s = nullptr;  // 4 values
+  unsigned long long* u64s = nullptr;  // 2 * kSumSlots values
   double* acc = nullptr;
   double* partial = nullptr;  // kRedBlocks
   double* scal = nullptr;     // 4 scalars
@@ -1203,7 +1209,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
       (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq.q[0], qn)) ||
       (rc = g->alloc(g->xq.q[1], qn)) || (rc = g->alloc(g->xq.q[2], qn)) ||
       (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
-      (rc = g->alloc(g->u64s, 4)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
+      (rc = g->alloc(g->u64s, 2 * kSumSlots)) || (rc = g->alloc(g->acc, (size_t)n)) || (rc = g->alloc(g->partial, kRedBlocks)) ||
       (rc = g->alloc(g->scal, 4)) || (rc = g->alloc(g->f0, (size_t)n)) || (rc = g->alloc(g->f1, (size_t)n)))
     return bail(rc);
 
@@ -1289,7 +1295,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
                                    int64_t* contacts) {
   if (!g || (nverts > 0 && !verts) || nverts < 0) return fail(TGA_EINVAL, "tga_neighbor_access: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
-  HIP_TRY(hipMemsetAsync(g->u64s, 0, 2 * sizeof(unsigned long long), g->stream));
+  HIP_TRY(hipMemsetAsync(g->u64s, 0, 2 * kSumSlots * sizeof(unsigned long long), g->stream));
   if (nverts > 0) {
     // Listed vertices: expanded into flat/chain items like a frontier, in
     // batches whose expansion fits the item queues (a list may repeat a long
@@ -1318,12 +1324,17 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
     if (rc) return rc;
     LAUNCH_QUEUED(k_ra, hc, g->u64s);
   }
-  unsigned long long hv[2];
+  unsigned long long hv[2 * kSumSlots];
   HIP_TRY(hipMemcpyAsync(hv, g->u64s, sizeof hv, hipMemcpyDeviceToHost, g->stream));
   int rc = g->check_err();
   if (rc) return rc;
-  if (checksum) *checksum = hv[0];
-  if (contacts) *contacts = (int64_t)hv[1];
+  unsigned long long cs = 0, cnt = 0;
+  for (int i = 0; i < kSumSlots; ++i) {
+    cs += hv[2 * i];
+    cnt += hv[2 * i + 1];
+  }
+  if (checksum) *checksum = cs;
+  if (contacts) *contacts = (int64_t)cnt;
   return TGA_OK;
 }
 
