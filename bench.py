@@ -7,12 +7,14 @@ vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks (``chunglu21``),
 generated and encoded by the format kit before the timed region.
 
 Prints one JSON line with the driver's contract fields plus:
-  roofline      the decode kernel against MI355X HBM bandwidth (8 TB/s):
-                algorithmic bytes per launch (|.graph| + |.timestamps| + the
+  roofline      the decode pass against MI355X HBM bandwidth (8 TB/s):
+                algorithmic bytes per pass (|.graph| + |.timestamps| + the
                 Elias-Fano index bits (n+1)(l_g+2 + l_t+2)/8) divided by the
-                average kernel duration measured with HIP events on the
-                engine's stream; `traffic` (rocprof FETCH_SIZE + WRITE_SIZE per
-                launch) is filled from --traffic-bytes when given, else null.
+                average pass duration measured with HIP events on the engine's
+                stream (the pass is the group kernels launched concurrently);
+                `traffic` is the rocprof FETCH_SIZE + WRITE_SIZE per pass, scaled
+                by a copy-kernel calibration, from profiles/ when that
+                measurement was taken on the current engine sources (else null).
   cpu_baseline  the CPU oracle's neighbor access (single thread, the build's
                 C restatement of the reference reader, kind "port") timed on
                 this host over a bounded vertex sample.
@@ -34,6 +36,33 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ra_chunglu21.traffic.json")
+
+
+def engine_source_digest() -> str:
+    """SHA-256 over the engine's HIP sources (identifies the measured build)."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "compresstemporalgraph_amd", "csrc", "engine")
+    for name in sorted(os.listdir(d)):
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per pass from the committed rocprof FETCH_SIZE / WRITE_SIZE
+    measurement, when it was taken on this workload with these engine sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != workload or rec.get("engine_sha256") != engine_source_digest():
+        return None
+    return rec.get("bytes_per_pass")
 
 
 def cpu_baseline(ds_dir: str, name: str, n: int, budget_s: float) -> dict:
@@ -72,7 +101,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="HBM bytes per decode launch from rocprof FETCH_SIZE + WRITE_SIZE")
+                    help="HBM bytes per pass from rocprof FETCH_SIZE + WRITE_SIZE (default: the committed "
+                         "measurement in profiles/ when it matches the engine sources)")
     ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
@@ -127,15 +157,16 @@ def main() -> int:
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = contacts * args.steps / elapsed
-    # One pass = the flat-record launch plus the chain-record launch.
+    # One pass = the concurrent launches of the three item groups (flat
+    # records, depth-1 chains, deeper chains), timed as one interval.
     avg_kernel_s = kernel_ms / args.steps / 1e3
     index_bits = (n + 1) * (info["l_graph"] + 2 + info["l_timestamps"] + 2)
     alg_bytes = info["graph_bytes"] + info["timestamps_bytes"] + index_bits / 8.0
     achieved = alg_bytes / avg_kernel_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": args.traffic_bytes,
-                "kernel": "k_ra (flat + chain launches of one pass)",
+                "traffic": args.traffic_bytes if args.traffic_bytes is not None else measured_traffic(args.workload),
+                "kernel": "k_ra (the three concurrent group launches of one pass)",
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
     g.close()

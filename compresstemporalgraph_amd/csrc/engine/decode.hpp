@@ -35,7 +35,7 @@ constexpr uint32_t kSpanBits = 1024;  // select blocks wider than this are store
 // Contacts between consecutive decode checkpoints of a long record, by the
 // record's reference-chain depth: deeper records cost more per contact (their
 // copies walk the referenced lists), so their chunks are shorter.
-__host__ __device__ constexpr int32_t chunk_stride(int depth) { return depth == 0 ? 256 : (depth == 1 ? 64 : 32); }
+__host__ __device__ constexpr int32_t chunk_stride(int depth) { return depth == 0 ? 128 : (depth == 1 ? 16 : 8); }
 
 // Error bits reported through the per-handle device error word.
 enum : int32_t {
