@@ -59,8 +59,8 @@ struct EFDir {
 struct Ckpt;
 
 struct Streams {
-  const uint64_t* g;  // graph stream, byte-swapped words (+3 zero words)
-  const uint64_t* t;  // timestamp stream, byte-swapped words (+3 zero words)
+  const uint64_t* g;  // graph stream, byte-swapped words (+5 zero words)
+  const uint64_t* t;  // timestamp stream, byte-swapped words (+5 zero words)
   uint64_t g_bits, t_bits;
   uint32_t g_last, t_last;  // index of the last padding word of each stream
   EFDir gi, ti;
@@ -173,7 +173,7 @@ __device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& po
 // only 64 or more bits later, so the load's latency overlaps the decoding of
 // the codewords in between.  The stream pointer and the index of its last
 // padding word are passed in (they are the same for every buffer over one
-// stream); stream word arrays carry three zero words of padding, and reads
+// stream); stream word arrays carry five zero words of padding, and reads
 // past `last` return 0 (every code then fails its length check).
 struct BitBuf {
   uint64_t a, b, c;
@@ -208,35 +208,107 @@ struct BitBuf {
   __device__ __forceinline__ uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
 };
 
+// Pipelined bit buffer for the sub-streams read about one codeword per
+// contact (timestamps, and the top-level residuals and multiples).  Words
+// a, b, c = w[idx..idx+2] are always held, and up to two look-ahead words
+// d, e = w[idx+3], w[idx+4] (`nv` of them).  Once per contact the walk calls
+// step() before decoding: it merges the word requested by the previous step
+// and requests the next missing look-ahead word.  Every wait for loaded data
+// then falls at the top of a contact, on loads issued a contact earlier: the
+// wave's memory counter is shared by its lanes and drains in order, so a load
+// consumed in the contact that issues it would stall the whole wave on every
+// lane's refills.  A shift with no look-ahead word left (more than one word
+// consumed in one contact) loads the missing word itself.
+struct PBuf {
+  uint64_t a, b, c, d, e;
+  uint64_t pf;    // word pfi, requested by the last step
+  uint32_t idx;   // word index of a
+  uint32_t sh;    // bit offset into a
+  uint32_t pfi;   // index of the word in pf (~0: none)
+  uint32_t nv;    // look-ahead words held (0..2)
+
+  __device__ __forceinline__ void init(const uint64_t* w, uint32_t last, uint64_t pos) {
+    uint64_t i = pos >> 6;
+    if (i + 4 > last) i = last - 4;
+    idx = (uint32_t)i;
+    sh = (uint32_t)pos & 63u;
+    a = w[idx];
+    b = w[idx + 1];
+    c = w[idx + 2];
+    d = w[idx + 3];
+    e = w[idx + 4];
+    nv = 2;
+    pf = 0;
+    pfi = ~0u;
+  }
+  __device__ __forceinline__ void clear() {
+    a = b = c = d = e = pf = 0;
+    idx = sh = nv = 0;
+    pfi = ~0u;
+  }
+  __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
+  __device__ __forceinline__ void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
+    sh += n;
+    if (sh >= 64) {
+      sh -= 64;
+      a = b;
+      b = c;
+      if (nv) {
+        c = d;
+        d = e;
+        --nv;
+      } else {
+        c = w[min(idx + 3, last)];
+      }
+      ++idx;
+    }
+  }
+  __device__ __forceinline__ void step(const uint64_t* w, uint32_t last) {
+    // Merge, branch-free: pf fills the first missing look-ahead slot.
+    const uint32_t m = (pfi == idx + 3 + nv);
+    d = (m & (nv == 0)) ? pf : d;
+    e = (m & (nv == 1)) ? pf : e;
+    nv += m;
+    pfi = nv < 2 ? idx + 3 + nv : ~0u;
+    if (nv < 2) pf = w[min(pfi, last)];
+  }
+  __device__ __forceinline__ uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
+};
+
 // Buffered readers of the graph stream (g_*) and the timestamp stream (t_*).
-__device__ __forceinline__ void g_init(const Streams& S, BitBuf& b, uint64_t pos) {
+template <class B>
+__device__ __forceinline__ void g_init(const Streams& S, B& b, uint64_t pos) {
   if (pos >= S.g_bits) raise_err(S, kErrOverrun);
   b.init(S.g, S.g_last, pos);
 }
-__device__ __forceinline__ uint64_t g_gamma(const Streams& S, BitBuf& b) {
+template <class B>
+__device__ __forceinline__ uint64_t g_gamma(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_gamma(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
-__device__ __forceinline__ uint32_t g_unary(const Streams& S, BitBuf& b) {
+template <class B>
+__device__ __forceinline__ uint32_t g_unary(const Streams& S, B& b) {
   uint32_t len;
   const uint32_t v = dec_unary(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
-template <int K>
-__device__ __forceinline__ uint64_t g_zeta(const Streams& S, BitBuf& b) {
+template <int K, class B>
+__device__ __forceinline__ uint64_t g_zeta(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<K>(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
-__device__ __forceinline__ void t_init(const Streams& S, BitBuf& b, uint64_t pos) {
+template <class B>
+__device__ __forceinline__ void t_init(const Streams& S, B& b, uint64_t pos) {
   if (pos >= S.t_bits) raise_err(S, kErrOverrun);
   b.init(S.t, S.t_last, pos);
 }
-__device__ __forceinline__ uint64_t t_zeta2(const Streams& S, BitBuf& b) {
+template <class B>
+__device__ __forceinline__ uint64_t t_zeta2(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<2>(S, b.peek(), len);
   b.skip(len, S.t, S.t_last);
@@ -317,13 +389,40 @@ struct Level {
 // Residuals and multiples of every level are read through bit buffers (their
 // Level cursors rpos / mpos are stale while decoding and are written back
 // only into checkpoints); intervals and copy blocks, which are rare, are read
-// directly.
+// directly.  The top level yields one value per contact and uses pipelined
+// buffers; the levels of a reference chain yield values in bursts and refill
+// on demand.
 template <int MAXD>
 struct Dec {
   Level lv[MAXD + 1];
-  BitBuf rb[MAXD + 1];  // residuals
-  BitBuf mb[MAXD + 1];  // multiples (and the record header while it is parsed)
+  PBuf rb0, mb0;                         // level 0: residuals, multiples (and the header while parsed)
+  BitBuf rb[MAXD > 0 ? MAXD : 1];        // level L >= 1 at [L - 1]
+  BitBuf mb[MAXD > 0 ? MAXD : 1];
 };
+
+template <int L, int MAXD>
+__device__ __forceinline__ auto& rbuf(Dec<MAXD>& D) {
+  if constexpr (L == 0) return D.rb0; else return D.rb[L - 1];
+}
+template <int L, int MAXD>
+__device__ __forceinline__ auto& mbuf(Dec<MAXD>& D) {
+  if constexpr (L == 0) return D.mb0; else return D.mb[L - 1];
+}
+template <int L, int MAXD>
+__device__ __forceinline__ const auto& rbuf(const Dec<MAXD>& D) {
+  if constexpr (L == 0) return D.rb0; else return D.rb[L - 1];
+}
+template <int L, int MAXD>
+__device__ __forceinline__ const auto& mbuf(const Dec<MAXD>& D) {
+  if constexpr (L == 0) return D.mb0; else return D.mb[L - 1];
+}
+
+// Advances the pipelined buffers of the top level by one contact.
+template <int MAXD>
+__device__ __forceinline__ void dec_step(const Streams& S, Dec<MAXD>& D) {
+  D.rb0.step(S.g, S.g_last);
+  D.mb0.step(S.g, S.g_last);
+}
 
 // Saved decoder state at a chunk boundary of a long list (see engine.hip).
 //
@@ -366,17 +465,33 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D);
 
 // Restores one level from a saved copy: the buffers of the sub-streams that
 // still have codes to read are refilled at their saved positions.
-__device__ __forceinline__ void lv_restore(const Streams& S, Level& v, BitBuf& rb, BitBuf& mb, const Level& src) {
+template <class RB, class MB>
+__device__ __forceinline__ void lv_restore(const Streams& S, Level& v, RB& rb, MB& mb, const Level& src) {
   v = src;
   if (v.rleft > 0) g_init(S, rb, v.rpos); else rb.clear();
   if (v.mleft > 0) g_init(S, mb, v.mpos); else mb.clear();
 }
 
 // Copies one level, with its sub-stream positions, into a checkpoint.
-__device__ __forceinline__ void lv_save(Level& dst, const Level& v, const BitBuf& rb, const BitBuf& mb) {
+template <class RB, class MB>
+__device__ __forceinline__ void lv_save(Level& dst, const Level& v, const RB& rb, const MB& mb) {
   dst = v;
   dst.rpos = rb.pos();
   dst.mpos = mb.pos();
+}
+
+// Restores levels L..MAXD of D from saved levels src[L - LL ..] (those within
+// the saved chain depth).
+template <int LL, int L, int MAXD>
+__device__ __forceinline__ void restore_levels(const Streams& S, Dec<MAXD>& D, const Level* src, int32_t depth) {
+  if (L - LL <= depth) lv_restore(S, D.lv[L], rbuf<L>(D), mbuf<L>(D), src[L - LL]);
+  if constexpr (L < MAXD) restore_levels<LL, L + 1, MAXD>(S, D, src, depth);
+}
+
+template <int L, int MAXD>
+__device__ __forceinline__ void save_levels(Level* dst, const Dec<MAXD>& D) {
+  lv_save(dst[L], D.lv[L], rbuf<L>(D), mbuf<L>(D));
+  if constexpr (L < MAXD) save_levels<L + 1, MAXD>(dst, D);
 }
 
 // Skips toward value `target` of level LL by restoring the checkpoint of the
@@ -394,10 +509,7 @@ __device__ __forceinline__ int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int3
   const int32_t from = D.lv[LL].kpos;
   if (j * stride <= from) return 0;
   const Ckpt* c = S.ck + first + j;
-  const int32_t depth = c->depth;
-#pragma unroll
-  for (int l = 0; LL + l <= MAXD; ++l)
-    if (l <= depth) lv_restore(S, D.lv[LL + l], D.rb[LL + l], D.mb[LL + l], c->lv[l]);
+  restore_levels<LL, LL, MAXD>(S, D, c->lv, c->depth);
   return D.lv[LL].kpos - from;
 }
 
@@ -453,7 +565,7 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   ++v.kpos;
   if (m == v.hr) {
     if (v.rleft > 0) {
-      v.hr += (int32_t)g_zeta<3>(S, D.rb[L]) + 1;
+      v.hr += (int32_t)g_zeta<3>(S, rbuf<L>(D)) + 1;
       --v.rleft;
     } else {
       v.hr = kInf;
@@ -461,8 +573,8 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   } else if (m == v.hm) {
     if (--v.mrem == 0) {
       if (v.mleft > 0) {
-        v.hm += (int32_t)g_gamma(S, D.mb[L]) + 1;
-        v.mrem = (int32_t)g_gamma(S, D.mb[L]) + 2;
+        v.hm += (int32_t)g_gamma(S, mbuf<L>(D)) + 1;
+        v.mrem = (int32_t)g_gamma(S, mbuf<L>(D)) + 2;
         --v.mleft;
       } else {
         v.hm = kInf;
@@ -503,7 +615,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   // The header is parsed through the residual buffer, which ends up at the
   // first residual; when the record has multiples, the multiples buffer is
   // forked from it at the first run.
-  BitBuf& h = D.rb[L];
+  auto& h = rbuf<L>(D);
   v.x = x;
   v.hm = v.hi = v.hr = v.hc = kInf;
   v.mrem = v.mleft = v.irem = v.ileft = v.rleft = 0;
@@ -511,7 +623,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   v.kpos = 0;
   v.bidx = -1;
   v.mpos = v.ipos = v.rpos = v.bpos = gpos;
-  D.mb[L].clear();
+  mbuf<L>(D).clear();
   g_init(S, h, gpos);
   const int32_t d = (int32_t)g_gamma(S, h);
   if (d <= 0) return 0;
@@ -554,7 +666,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
   if (extra > 0) {  // multiples
     const int32_t mc = (int32_t)g_gamma(S, h);
     if (mc > 0) {
-      D.mb[L] = h;
+      mbuf<L>(D) = h;
       for (int32_t i = 0; i < mc; ++i) {
         if (extra <= 0) {  // more runs than contacts: malformed
           raise_err(S, kErrBlocks);
@@ -563,8 +675,8 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
         g_gamma(S, h);
         extra -= (int32_t)g_gamma(S, h) + 2;
       }
-      v.hm = (int32_t)(nat2int(g_gamma(S, D.mb[L])) + x);
-      v.mrem = (int32_t)g_gamma(S, D.mb[L]) + 2;
+      v.hm = (int32_t)(nat2int(g_gamma(S, mbuf<L>(D))) + x);
+      v.mrem = (int32_t)g_gamma(S, mbuf<L>(D)) + 2;
       v.mleft = mc - 1;
     }
   }
@@ -597,7 +709,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
 // Timestamp cursor: zeta_2 of int2nat deltas, reset to the header value per
 // vertex (EvolvingMultiGraph.java:131-144; bfs.cu:1444, :1597).
 struct TsCursor {
-  BitBuf b;
+  PBuf b;
   int32_t prev;
 };
 
@@ -615,16 +727,12 @@ __device__ __forceinline__ void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t d
   c->depth = depth;
   c->tpos = tc.b.pos();
   c->tprev = tc.prev;
-#pragma unroll
-  for (int l = 0; l <= MAXD; ++l) lv_save(c->lv[l], D.lv[l], D.rb[l], D.mb[l]);
+  save_levels<0, MAXD>(c->lv, D);
 }
 
 template <int MAXD>
 __device__ __forceinline__ void ckpt_load(const Streams& S, const Ckpt* c, Dec<MAXD>& D, TsCursor& tc) {
-  const int32_t depth = c->depth;
-#pragma unroll
-  for (int l = 0; l <= MAXD; ++l)
-    if (l <= depth) lv_restore(S, D.lv[l], D.rb[l], D.mb[l], c->lv[l]);
+  restore_levels<0, 0, MAXD>(S, D, c->lv, c->depth);
   t_init(S, tc.b, c->tpos);
   tc.prev = c->tprev;
 }
@@ -635,6 +743,8 @@ template <int MAXD, class Act>
 __device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& tc, int32_t x, int32_t k, int32_t kend,
                                      Act& act) {
   for (; k < kend; ++k) {
+    tc.b.step(S.t, S.t_last);
+    dec_step(S, D);
     const int32_t r = dec_next<0, MAXD>(S, D);
     const int32_t t = ts_next(S, tc);
     if ((uint32_t)r >= (uint32_t)S.n) {

@@ -747,8 +747,11 @@ bool read_file(const std::string& path, std::vector<uint8_t>& out) {
 
 // Bytes -> 64-bit words whose most significant byte is the first stream byte,
 // plus three zero words of padding (BitBuf reads up to two words ahead).
+// Stream words plus kPadWords zero words (bit buffers read up to five words
+// from any position inside the stream).
+constexpr size_t kPadWords = 5;
 std::vector<uint64_t> be_words(const std::vector<uint8_t>& b) {
-  std::vector<uint64_t> w((b.size() + 7) / 8 + 3, 0);
+  std::vector<uint64_t> w((b.size() + 7) / 8 + kPadWords, 0);
   for (size_t i = 0; i < b.size(); ++i) w[i >> 3] |= (uint64_t)b[i] << (56 - 8 * (i & 7));
   return w;
 }
