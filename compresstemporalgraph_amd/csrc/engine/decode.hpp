@@ -241,9 +241,12 @@ struct PBuf {
     pf = 0;
     pfi = ~0u;
   }
+  // A buffer that will not be read: its look-ahead counts as full, so steps
+  // request nothing.
   __device__ __forceinline__ void clear() {
     a = b = c = d = e = pf = 0;
-    idx = sh = nv = 0;
+    idx = sh = 0;
+    nv = 2;
     pfi = ~0u;
   }
   __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
