@@ -15,10 +15,12 @@
 //      records (four levels), each ordered by decreasing cost so waves get
 //      items of similar length.
 // Full passes (RA, DC, PR, HITS) launch over the three lists concurrently.
-// Frontier algorithms (BFS, CC, GC, earliest arrival) expand their vertex
-// frontier into per-group item queues each round; the host reads a few 4-byte counts per
-// round to size the next launches (the reference copies the queue size back
-// per level too, bfs.cu:2028).
+// Frontier algorithms (BFS, CC, GC, earliest arrival) mark vertices in
+// per-vertex arrays (level, change stamp, color) and select each round's
+// vertices by scanning the active range with a predicate (k_expand), which
+// writes the round's item queues without per-vertex atomics; the host reads
+// the few 4-byte queue counts to size the launches (the reference copies its
+// queue size back per level too, bfs.cu:2028).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -99,8 +101,6 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Appends v to a device queue.
-__device__ __forceinline__ void push(int32_t* q, uint32_t* qn, int32_t v) { q[atomicAdd(qn, 1u)] = v; }
 
 // Accumulator pairs of the neighbor-access checksum and count.
 constexpr int kSumSlots = 256;
@@ -179,29 +179,139 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
   nchunk_out[i] = nch;
 }
 
-// Expands a vertex set (an explicit list, or the range [base, base + count))
-// into per-group item queues; cnt[g] counts group g.  keep_empty keeps
-// records without contacts (graph coloring colors them) in group 0.  Chunked
-// records also go to bigq (counted in cnt[kGroups]) when given.
-__global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t base, uint32_t count,
+// Selection of the vertices a round expands (evaluated on device arrays).
+enum : int32_t {
+  kSelAll = 0,        // every vertex of the range or list
+  kSelEq = 1,         // a[v] == val (BFS level, round stamp of a changed vertex)
+  kSelUncolored = 2,  // a[v] == kInf (graph coloring's worklist)
+};
+struct Sel {
+  int32_t kind;
+  const int32_t* a;
+  int32_t val;
+  __device__ __forceinline__ bool operator()(int32_t v) const {
+    if (kind == kSelAll) return true;
+    const int32_t x = a[v];
+    return kind == kSelEq ? x == val : x == kInf;
+  }
+};
+
+constexpr int kExpandBlocks = 1024;  // grid of k_expand (one queue reservation per group per block)
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix of v over the block's threads (in thread order) for
+// kGroups + 1 counters at once; tot receives the block totals.  All threads
+// of the block must call it.
+__device__ __forceinline__ void block_excl_scan(const uint32_t (&v)[kGroups + 1], uint32_t (&pre)[kGroups + 1],
+                                                uint32_t (&tot)[kGroups + 1]) {
+  __shared__ uint32_t wsum[kBlock / 64][kGroups + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t inc[kGroups + 1];
+#pragma unroll
+  for (int g = 0; g <= kGroups; ++g) {
+    inc[g] = wave_incl_scan(v[g]);
+    if (lane == 63) wsum[wave][g] = inc[g];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g <= kGroups; ++g) {
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wave) before += wsum[w][g];
+      all += wsum[w][g];
+    }
+    pre[g] = before + inc[g] - v[g];
+    tot[g] = all;
+  }
+  __syncthreads();
+}
+
+// Expands the selected vertices of an explicit list, or of the range
+// [base, base + count), into per-group item queues: cnt[g] receives the count
+// of group g, and, when bigq is given, chunked records are also listed in bigq
+// (counted in cnt[kGroups]).  keep_empty keeps records without contacts
+// (graph coloring colors them) as group-0 items.  Each block takes one
+// contiguous slice: it counts its items, reserves queue space with one atomic
+// per group, then writes them in vertex order.
+__global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t base, uint32_t count, Sel sel,
                                                    const int32_t* vinfo, const Ckpt* ck, Queues Q, uint32_t* cnt,
                                                    int32_t* bigq, int keep_empty) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    const int32_t x = front ? front[i] : base + (int32_t)i;
-    const int32_t info = vinfo[x];
+  __shared__ uint32_t at[kGroups + 1];
+  const uint32_t per = (count + gridDim.x - 1) / gridDim.x;
+  const uint32_t slice = (per + kBlock - 1) / kBlock * kBlock;
+  const uint32_t lo = min(count, blockIdx.x * slice), hi = min(count, lo + slice);
+  // (vertex, group, items) of element i of the slice; items 0: not expanded.
+  auto classify = [&](uint32_t i, int32_t& x, int32_t& info, int& grp, uint32_t& items) {
+    x = front ? front[i] : base + (int32_t)i;
+    items = 0;
+    grp = 0;
+    info = kEmpty;
+    if (!sel(x)) return;
+    info = vinfo[x];
     if (info == kEmpty) {
-      if (keep_empty) push(Q.q[0], &cnt[0], x);
+      items = keep_empty ? 1u : 0u;
     } else if (info < 0) {
-      const int grp = group_of(-1 - info);
-      push(Q.q[grp], &cnt[grp], x);
+      grp = group_of(-1 - info);
+      items = 1;
     } else {
-      const Ckpt* c = ck + info;
-      const int32_t nch = c->nchunk;
-      const int grp = group_of(c->depth);
-      const uint32_t at = atomicAdd(&cnt[grp], (uint32_t)nch);
-      for (int32_t j = 0; j < nch; ++j) Q.q[grp][at + j] = -(info + j) - 1;
-      if (bigq) push(bigq, &cnt[kGroups], x);
+      grp = group_of(ck[info].depth);
+      items = (uint32_t)ck[info].nchunk;
     }
+  };
+  uint32_t own[kGroups + 1] = {};
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    int32_t x, info;
+    int grp;
+    uint32_t items;
+    classify(i, x, info, grp, items);
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) own[g] += g == grp ? items : 0u;
+    if (items && info >= 0) ++own[kGroups];
+  }
+  uint32_t pre[kGroups + 1], tot[kGroups + 1];
+  block_excl_scan(own, pre, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) at[g] = tot[g] ? atomicAdd(&cnt[g], tot[g]) : 0;
+    at[kGroups] = (bigq && tot[kGroups]) ? atomicAdd(&cnt[kGroups], tot[kGroups]) : 0;
+  }
+  __syncthreads();
+  uint32_t run[kGroups + 1];
+#pragma unroll
+  for (int g = 0; g <= kGroups; ++g) run[g] = at[g];
+  for (uint32_t t0 = lo; t0 < hi; t0 += kBlock) {
+    const uint32_t i = t0 + threadIdx.x;
+    int32_t x = 0, info = kEmpty;
+    int grp = 0;
+    uint32_t items = 0;
+    if (i < hi) classify(i, x, info, grp, items);
+    uint32_t v[kGroups + 1];
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) v[g] = g == grp ? items : 0u;
+    v[kGroups] = (items && info >= 0) ? 1u : 0u;
+    block_excl_scan(v, pre, tot);
+    if (items) {
+      // (static indices only: a dynamically indexed register array spills)
+      int32_t* q = grp == 0 ? Q.q[0] + run[0] + pre[0] : (grp == 1 ? Q.q[1] + run[1] + pre[1] : Q.q[2] + run[2] + pre[2]);
+      if (info < 0) {
+        q[0] = x;
+      } else {
+        for (uint32_t j = 0; j < items; ++j) q[j] = -(info + (int32_t)j) - 1;
+        if (bigq) bigq[run[kGroups] + pre[kGroups]] = x;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g <= kGroups; ++g) run[g] += tot[g];
   }
 }
 
@@ -296,16 +406,16 @@ __global__ void __launch_bounds__(64) k_dc_long(Streams S, const int32_t* bigv, 
   centr[x] = W.active(x) ? dc_formula(a.deg, a.avg, W.ws, W.we) : dc_formula(0, 0, W.ws, W.we);
 }
 
+// BFS level step: every unreached active neighbor over a window contact gets
+// level lvl + 1 (all writers of a pass store the same value, so plain stores
+// suffice); the next level is selected by its level value.
 struct BfsAct {
   Window W;
   int32_t* level;
-  int32_t* next;
-  uint32_t* nsize;
   int32_t lvl;
   __device__ bool begin(int32_t x, bool, Split) { return W.active(x); }
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
-    if (W.in(t) && W.active(r) && level[r] == kInf && atomicCAS(&level[r], kInf, lvl + 1) == kInf)
-      push(next, nsize, r);
+    if (W.in(t) && W.active(r) && level[r] == kInf) level[r] = lvl + 1;
     return true;
   }
   __device__ void end(int32_t, bool) {}
@@ -313,19 +423,18 @@ struct BfsAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_bfs(Streams S, const Ckpt* ck, Items I, Window W, int32_t* level,
-                                                int32_t* next, uint32_t* nsize, int32_t lvl) {
-  BfsAct a{W, level, next, nsize, lvl};
+                                                int32_t lvl) {
+  BfsAct a{W, level, lvl};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
-// Min-label propagation: a vertex whose label drops is queued once per round.
+// Min-label propagation: a vertex whose label drops in round r is stamped r
+// and expanded in round r + 1.
 struct CcAct {
   Window W;
   int32_t* label;
   int32_t* stamp;
   int32_t round;
-  int32_t* next;
-  uint32_t* nsize;
   int32_t lx = 0;
   __device__ bool begin(int32_t x, bool, Split) {
     if (!W.active(x)) return false;
@@ -335,7 +444,7 @@ struct CcAct {
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
     if (W.in(t) && W.active(r) && lx < label[r]) {
       const int32_t old = atomicMin(&label[r], lx);
-      if (lx < old && atomicMax(&stamp[r], round) < round) push(next, nsize, r);
+      if (lx < old) stamp[r] = round;
     }
     return true;
   }
@@ -344,23 +453,22 @@ struct CcAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* label,
-                                               int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize) {
-  CcAct a{W, label, stamp, round, next, nsize};
+                                               int32_t* stamp, int32_t round) {
+  CcAct a{W, label, stamp, round};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
 // uncolored at the start of the round (its color is kInf, or `round` if it was
 // colored earlier in this same round).  Chunks of a long record report to
-// blocked[x]; k_gc_finish decides those vertices.
+// blocked[x]; k_gc_finish decides those vertices.  Blocked vertices stay
+// uncolored and are selected again next round.
 struct GcAct {
   Window W;
   const int32_t* prio;
   int32_t* color;
   int32_t* blocked;
   int32_t round;
-  int32_t* next;
-  uint32_t* nsize;
   int32_t v = 0, pv = 0;
   bool blk = false;
   __device__ bool begin(int32_t x, bool chunked, Split) {
@@ -386,32 +494,26 @@ struct GcAct {
       if (blk) atomicOr(&blocked[x], 1);
     } else if (!blk) {
       color[x] = round;
-    } else {
-      push(next, nsize, x);
     }
   }
 };
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_gc(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
-                                               int32_t* color, int32_t* blocked, int32_t round, int32_t* next,
-                                               uint32_t* nsize) {
-  GcAct a{W, prio, color, blocked, round, next, nsize};
+                                               int32_t* color, int32_t* blocked, int32_t round) {
+  GcAct a{W, prio, color, blocked, round};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
 __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint32_t nbig, Window W, int32_t* color,
-                                                      int32_t* blocked, int32_t round, int32_t* next,
-                                                      uint32_t* nsize) {
+                                                      int32_t* blocked, int32_t round) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += gridDim.x * blockDim.x) {
     const int32_t x = bigq[i];
     if (!W.active(x) || color[x] != kInf) continue;
-    if (blocked[x]) {
+    if (blocked[x])
       blocked[x] = 0;
-      push(next, nsize, x);
-    } else {
+    else
       color[x] = round;
-    }
   }
 }
 
@@ -434,8 +536,6 @@ struct EaAct {
   int32_t* arr;
   int32_t* stamp;
   int32_t round;
-  int32_t* next;
-  uint32_t* nsize;
   uint64_t* runkey;
   Split sp{-1, -1, -1, -1};
   int32_t ax = 0, last = -1;
@@ -449,7 +549,7 @@ struct EaAct {
   __device__ void relax(int32_t r, int32_t t) {
     if (W.active(r) && ax <= t && t < arr[r]) {
       const int32_t old = atomicMin(&arr[r], t);
-      if (t < old && atomicMax(&stamp[r], round) < round) push(next, nsize, r);
+      if (t < old) stamp[r] = round;
     }
   }
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
@@ -464,9 +564,8 @@ struct EaAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ea(Streams S, const Ckpt* ck, Items I, Window W, int32_t* arr,
-                                               int32_t* stamp, int32_t round, int32_t* next, uint32_t* nsize,
-                                               uint64_t* runkey) {
-  EaAct a{W, arr, stamp, round, next, nsize, runkey};
+                                               int32_t* stamp, int32_t round, uint64_t* runkey) {
+  EaAct a{W, arr, stamp, round, runkey};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
@@ -484,12 +583,11 @@ __device__ __forceinline__ bool split_run(const Ckpt* ck, uint64_t* runkey, int3
 }
 
 __global__ void __launch_bounds__(kBlock) k_ea_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
-                                                   int32_t* arr, int32_t* stamp, int32_t round, int32_t* next,
-                                                   uint32_t* nsize) {
+                                                   int32_t* arr, int32_t* stamp, int32_t round) {
   for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nck; s += gridDim.x * blockDim.x) {
     SplitRun q;
     if (!split_run(ck, runkey, s, q)) continue;
-    EaAct a{W, arr, stamp, round, next, nsize, runkey};
+    EaAct a{W, arr, stamp, round, runkey};
     a.ax = arr[q.x];
     a.relax(q.r, q.t);
   }
@@ -824,7 +922,7 @@ struct tga_graph {
   int32_t* big[kClasses] = {};  // chunked records by class
   int32_t n_big[kClasses] = {};
   // scratch (n-sized unless stated)
-  int32_t *q0 = nullptr, *q1 = nullptr, *i0 = nullptr, *i1 = nullptr, *i2 = nullptr;
+  int32_t *q0 = nullptr, *i0 = nullptr, *i1 = nullptr, *i2 = nullptr;
   Queues xq{};                // expanded item queues by group
   int32_t* xq_big = nullptr;  // chunked records of the expanded set
   uint32_t* cnt = nullptr;  // 8 counters
@@ -924,14 +1022,18 @@ struct tga_graph {
     HIP_TRY(hipStreamSynchronize(stream));
     return TGA_OK;
   }
-  // Expands a vertex set into the group queues (and xq_big); hc[g] receives
+  // Expands the selected vertices of a list (front) or of the range
+  // [base, base + count) into the group queues (and xq_big); hc[g] receives
   // the count of group g and hc[kGroups] the number of chunked records.
-  int expand(const int32_t* front, int32_t base, uint32_t count, bool keep_empty, bool want_big,
+  int expand(const int32_t* front, int32_t base, uint32_t count, Sel sel, bool keep_empty, bool want_big,
              uint32_t hc[kGroups + 1]) {
     HIP_TRY(hipMemsetAsync(cnt, 0, (kGroups + 1) * sizeof(uint32_t), stream));
-    k_expand<<<grid(count), kBlock, 0, stream>>>(front, base, count, vinfo, ck, xq, cnt,
-                                                 want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
-    HIP_TRY(hipGetLastError());
+    if (count > 0) {
+      const int blocks = (int)std::min<int64_t>(kExpandBlocks, (count + kBlock - 1) / kBlock);
+      k_expand<<<blocks, kBlock, 0, stream>>>(front, base, count, sel, vinfo, ck, xq, cnt,
+                                              want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
+      HIP_TRY(hipGetLastError());
+    }
     return read_counts(cnt, hc, kGroups + 1);
   }
   int create_streams() {
@@ -1208,7 +1310,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   if ((rc = build_schedule(g))) return bail(rc);
 
   const size_t qn = (size_t)n + (size_t)g->nck;
-  if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->q1, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
+  if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
       (rc = g->alloc(g->i1, (size_t)n)) || (rc = g->alloc(g->i2, (size_t)n)) || (rc = g->alloc(g->xq.q[0], qn)) ||
       (rc = g->alloc(g->xq.q[1], qn)) || (rc = g->alloc(g->xq.q[2], qn)) ||
       (rc = g->alloc(g->xq_big, (size_t)n)) || (rc = g->alloc(g->cnt, 8)) ||
@@ -1314,7 +1416,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
       HIP_TRY(hipMemcpyAsync(g->q0, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
                              g->stream));
       uint32_t hc[kGroups + 1];
-      int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), false, false, hc);
+      int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), Sel{kSelAll, nullptr, 0}, false, false, hc);
       if (rc) return rc;
       LAUNCH_QUEUED(k_ra, hc, g->u64s);
       i0 = i1;
@@ -1323,7 +1425,7 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
     LAUNCH_STATIC(k_ra, g->u64s);
   } else {
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), false, false, hc);
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelAll, nullptr, 0}, false, false, hc);
     if (rc) return rc;
     LAUNCH_QUEUED(k_ra, hc, g->u64s);
   }
@@ -1370,6 +1472,12 @@ extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count,
   return g->check_err();
 }
 
+static uint32_t total_items(const uint32_t (&hc)[kGroups + 1]) {
+  uint32_t t = 0;
+  for (int c = 0; c < kGroups; ++c) t += hc[c];
+  return t;
+}
+
 extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, uint32_t flags) {
   if (!g || !level || bad_window(w) || src < 0 || src >= g->n) return fail(TGA_EINVAL, "tga_bfs: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
@@ -1377,17 +1485,12 @@ extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, 
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
   const int32_t zero = 0;
   HIP_TRY(hipMemcpyAsync(g->i0 + src, &zero, 4, hipMemcpyHostToDevice, g->stream));
-  HIP_TRY(hipMemcpyAsync(g->q0, &src, 4, hipMemcpyHostToDevice, g->stream));
-  int32_t *cur = g->q0, *nxt = g->q1;
-  uint32_t fsize = 1;
-  for (int32_t lvl = 0; fsize > 0; ++lvl) {
+  for (int32_t lvl = 0;; ++lvl) {
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(cur, 0, fsize, false, false, hc);
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelEq, g->i0, lvl}, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_bfs, hc, W, g->i0, nxt, g->cnt + 6, lvl);
-    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
-    std::swap(cur, nxt);
+    if (total_items(hc) == 0) break;
+    LAUNCH_QUEUED(k_bfs, hc, W, g->i0, lvl);
   }
   int rc = g->out_copy(level, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
@@ -1400,25 +1503,21 @@ extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int
   HIP_TRY(hipSetDevice(g->device));
   const Window W = g->window(w);
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
-  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i1, g->n, -1);
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i1, g->n, kInf);
   const int32_t a0 = (int32_t)std::max<int64_t>(0, std::min<int64_t>(w.start, INT32_MAX));
+  const int32_t start_stamp = -1;  // round r expands the vertices stamped r - 1
   HIP_TRY(hipMemcpyAsync(g->i0 + src, &a0, 4, hipMemcpyHostToDevice, g->stream));
-  HIP_TRY(hipMemcpyAsync(g->q0, &src, 4, hipMemcpyHostToDevice, g->stream));
-  int32_t *cur = g->q0, *nxt = g->q1;
-  uint32_t fsize = 1;
-  for (int32_t round = 0; fsize > 0; ++round) {
+  HIP_TRY(hipMemcpyAsync(g->i1 + src, &start_stamp, 4, hipMemcpyHostToDevice, g->stream));
+  for (int32_t round = 0;; ++round) {
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(cur, 0, fsize, false, false, hc);
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelEq, g->i1, round - 1}, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_ea, hc, W, g->i0, g->i1, round, nxt, g->cnt + 6, g->runkey);
+    if (total_items(hc) == 0) break;
+    LAUNCH_QUEUED(k_ea, hc, W, g->i0, g->i1, round, g->runkey);
     if (g->nck > 0) {
-      k_ea_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->i0, g->i1, round,
-                                                          nxt, g->cnt + 6);
+      k_ea_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->i0, g->i1, round);
       HIP_TRY(hipGetLastError());
     }
-    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
-    std::swap(cur, nxt);
   }
   int rc = g->out_copy(arrival, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
@@ -1430,17 +1529,12 @@ extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* lab
   HIP_TRY(hipSetDevice(g->device));
   const Window W = g->window(w);
   k_init_labels<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->i0, g->i1);
-  int32_t *cur = nullptr, *nxt = g->q0;
-  uint32_t fsize = (uint32_t)(g->ae - g->ab);
-  for (int32_t round = 0; fsize > 0; ++round) {
+  for (int32_t round = 0;; ++round) {
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(cur, g->ab, fsize, false, false, hc);
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelEq, g->i1, round - 1}, false, false, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round, nxt, g->cnt + 6);
-    if ((rc = g->read_counts(g->cnt + 6, &fsize, 1))) return rc;
-    cur = nxt;
-    nxt = (nxt == g->q0) ? g->q1 : g->q0;
+    if (total_items(hc) == 0) break;
+    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round);
   }
   int rc = g->out_copy(label, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
@@ -1490,22 +1584,18 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
   }
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i2, g->n, 0);
-  int32_t *cur = nullptr, *nxt = g->q0;
-  uint32_t wsize = (uint32_t)(g->ae - g->ab);
-  for (int32_t round = 0; round < rounds && wsize > 0; ++round) {
+  for (int32_t round = 0; round < rounds; ++round) {
+    // Worklist: the active vertices still uncolored (vertices without
+    // contacts included: they are colored in the first round).
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(cur, g->ab, wsize, true, true, hc);
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelUncolored, g->i0, 0}, true, true, hc);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(g->cnt + 6, 0, sizeof(uint32_t), g->stream));
-    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round, nxt, g->cnt + 6);
+    if (total_items(hc) == 0) break;
+    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round);
     if (hc[kGroups] > 0) {
-      k_gc_finish<<<g->grid(hc[kGroups]), kBlock, 0, g->stream>>>(g->xq_big, hc[kGroups], W, g->i0, g->i2, round,
-                                                                   nxt, g->cnt + 6);
+      k_gc_finish<<<g->grid(hc[kGroups]), kBlock, 0, g->stream>>>(g->xq_big, hc[kGroups], W, g->i0, g->i2, round);
       HIP_TRY(hipGetLastError());
     }
-    if ((rc = g->read_counts(g->cnt + 6, &wsize, 1))) return rc;
-    cur = nxt;
-    nxt = (nxt == g->q0) ? g->q1 : g->q0;
   }
   int rc = g->out_copy(color, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
