@@ -65,6 +65,39 @@ def measured_traffic(workload: str):
     return rec.get("bytes_per_pass")
 
 
+def workload_times(g, t_lo: int, t_hi: int, sources: int = 8) -> dict:
+    """Milliseconds per call of the seven workloads over windows covering
+    10%, 50% and 100% of the timestamp span (the at-scale plain
+    configuration): BFS from `sources` seeded sources (mean), CC, PR (20
+    iterations, FP32), HITS (20 iterations, FP32), DC, GC and RA; plus the
+    earliest-arrival rule.  Each call is timed once after one warm call."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    span = t_hi - t_lo
+    windows = {"100%": (t_lo, t_hi), "50%": (t_lo + span // 4, t_lo + span // 4 + span // 2),
+               "10%": (t_lo + span // 2, t_lo + span // 2 + span // 10)}
+    srcs = [int(v) for v in rng.integers(0, g.info["n"], sources)]
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        f()
+        return round((time.perf_counter() - t0) * 1e3, 3)
+
+    out = {}
+    for name, w in windows.items():
+        row = {"RA": timed(lambda: g.neighbor_access()),
+               "DC": timed(lambda: g.degree_centrality(w)),
+               "BFS": round(sum(timed(lambda s=s: g.bfs(s, w)) for s in srcs) / len(srcs), 3),
+               "EA": timed(lambda: g.earliest_arrival(srcs[0], w)),
+               "CC": timed(lambda: g.connected_components(w)),
+               "GC": timed(lambda: g.graph_coloring(w, rounds=512)),
+               "PR": timed(lambda: g.pagerank(w, iters=20)),
+               "HITS": timed(lambda: g.hits(w, iters=20))}
+        out[name] = row
+    return out
+
+
 def cpu_baseline(ds_dir: str, name: str, n: int, budget_s: float) -> dict:
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -100,6 +133,7 @@ def main() -> int:
     ap.add_argument("--data-root", default=None)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the seven-workload timing table")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per pass from rocprof FETCH_SIZE + WRITE_SIZE (default: the committed "
                          "measurement in profiles/ when it matches the engine sources)")
@@ -169,6 +203,11 @@ def main() -> int:
                 "kernel": "k_ra (the three concurrent group launches of one pass)",
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
+    workloads = None
+    if not args.no_workloads:
+        t0 = time.perf_counter()
+        workloads = workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
+        log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
     cpu = None
@@ -196,6 +235,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "workloads_ms": workloads,
     }
     print(json.dumps(line), flush=True)
     return 0

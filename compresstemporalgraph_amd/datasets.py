@@ -78,6 +78,8 @@ def write_dataset(root: str, name: str, src, dst, t, n_min: int = 0, raw: bool =
         "active_end": st.nodes,
         "min_timestamp": st.min_timestamp,
         "contacts": st.arcs,
+        "t_min": int(np.min(t)) if len(t) else 0,
+        "t_max": int(np.max(t)) if len(t) else 0,
         "stats": asdict(st),
     }
     with open(os.path.join(d, name + ".dataset.json"), "w") as f:
