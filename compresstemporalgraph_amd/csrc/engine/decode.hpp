@@ -72,6 +72,11 @@ struct Streams {
   // stride) with the stride stored in the checkpoint.
   const Ckpt* ck;
   const int32_t* vinfo;
+  // Record start table: vpos[2x] and vpos[2x + 1] are the bit offsets of
+  // vertex x in .graph and .timestamps (the Elias-Fano getOffset of both
+  // indexes, bfs.cu:543-565, evaluated once at load), so that starting a
+  // record costs one 16-byte load instead of two select walks.
+  const uint64_t* vpos;
 };
 
 __device__ __forceinline__ void raise_err(const Streams& S, int32_t bit) { atomicOr(S.err, bit); }
@@ -365,6 +370,13 @@ __device__ __forceinline__ uint64_t ef_get(const Streams& S, const EFDir& d, uin
   return (hi << d.l) | lo;
 }
 
+// Start offsets of the record of x in both streams.
+__device__ __forceinline__ void rec_pos(const Streams& S, int32_t x, uint64_t& gpos, uint64_t& tpos) {
+  const ulonglong2 p = reinterpret_cast<const ulonglong2*>(S.vpos)[x];
+  gpos = p.x;
+  tpos = p.y;
+}
+
 // ---------------------------------------------------------------------------
 // Successor-list state machine.
 //
@@ -653,7 +665,7 @@ __device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int3
           if ((i & 1) == 0) even += b;
         }
         const int32_t y = x - (int32_t)r;
-        const int32_t dref = dec_init<L + 1, MAXD>(S, D, y, ef_get(S, S.gi, (uint64_t)y), depth);
+        const int32_t dref = dec_init<L + 1, MAXD>(S, D, y, S.vpos[2 * (size_t)y], depth);
         copied = even + ((bc & 1) ? 0 : dref - total);
         if (copied < 0 || copied > d) {
           raise_err(S, kErrBlocks);
@@ -771,12 +783,14 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
     x = code;
     if (!act.begin(x, false, Split{-1, -1, -1, -1})) return;
     int32_t depth = 0;
-    kend = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
+    uint64_t gpos, tpos;
+    rec_pos(S, x, gpos, tpos);
+    kend = dec_init<0, MAXD>(S, D, x, gpos, depth);
     if (kend == 0) {
       act.end(x, false);
       return;
     }
-    t_init(S, tc.b, ef_get(S, S.ti, (uint64_t)x));
+    t_init(S, tc.b, tpos);
     tc.prev = S.min_ts;
   } else {
     const Ckpt* c = ck + (-code - 1);

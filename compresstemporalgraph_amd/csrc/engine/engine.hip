@@ -117,16 +117,21 @@ struct Items {
 // ------------------------------------------------------- schedule kernels ---
 
 // Record header of every vertex: outdegree and reference distance.
-__global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int8_t* ref) {
+// Also fills the record start table (Streams::vpos) from the two
+// Elias-Fano indexes.
+__global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int8_t* ref, uint64_t* vpos) {
   const Rd g{S.g, S.g_bits};
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < S.n; x += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t pos = ef_get(S, S.gi, (uint64_t)x);
+    const uint64_t gpos = ef_get(S, S.gi, (uint64_t)x);
+    const uint64_t tpos = ef_get(S, S.ti, (uint64_t)x);
+    uint64_t pos = gpos;
     const int32_t d = (int32_t)rd_gamma(S, g, pos);
     int32_t r = 0;
     if (d > 0) r = (int32_t)rd_unary(S, g, pos);
-    ef_get(S, S.ti, (uint64_t)x);  // validates the timestamp index entry
     deg[x] = d;
     ref[x] = (int8_t)min(r, 127);
+    vpos[2 * x] = gpos;
+    vpos[2 * x + 1] = tpos;
   }
 }
 
@@ -144,8 +149,10 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
   Dec<MAXD> D;
   TsCursor tc;
   int32_t depth = 0;
-  const int32_t d = dec_init<0, MAXD>(S, D, x, ef_get(S, S.gi, (uint64_t)x), depth);
-  t_init(S, tc.b, ef_get(S, S.ti, (uint64_t)x));
+  uint64_t gpos, tpos;
+  rec_pos(S, x, gpos, tpos);
+  const int32_t d = dec_init<0, MAXD>(S, D, x, gpos, depth);
+  t_init(S, tc.b, tpos);
   tc.prev = S.min_ts;
   ckpt_save<MAXD>(out, D, depth, tc, x, 0);
   out[0].cont_slot = -1;
@@ -1063,14 +1070,17 @@ extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
 static int build_schedule(tga_graph* g) {
   const int64_t n = g->n;
-  const Streams& S = g->S;
+  Streams& S = g->S;
   int32_t* d_deg = nullptr;
   int8_t* d_ref = nullptr;
+  uint64_t* d_vpos = nullptr;
   int rc = g->alloc(d_deg, (size_t)n);
   if (!rc) rc = g->alloc(d_ref, (size_t)n);
+  if (!rc) rc = g->alloc(d_vpos, 2 * (size_t)n);
   if (rc) return rc;
-  k_headers<<<g->grid(n), kBlock, 0, g->stream>>>(S, d_deg, d_ref);
+  k_headers<<<g->grid(n), kBlock, 0, g->stream>>>(S, d_deg, d_ref, d_vpos);
   HIP_TRY(hipGetLastError());
+  S.vpos = d_vpos;
   std::vector<int32_t> deg((size_t)n);
   std::vector<int8_t> ref((size_t)n);
   HIP_TRY(hipMemcpyAsync(deg.data(), d_deg, (size_t)n * 4, hipMemcpyDeviceToHost, g->stream));
