@@ -191,13 +191,16 @@ enum : int32_t {
   kSelAll = 0,        // every vertex of the range or list
   kSelEq = 1,         // a[v] == val (BFS level, round stamp of a changed vertex)
   kSelUncolored = 2,  // a[v] == kInf (graph coloring's worklist)
+  kSelMask = 3,       // m[v] != 0 (batched BFS: some query has v in its frontier)
 };
 struct Sel {
   int32_t kind;
   const int32_t* a;
   int32_t val;
+  const uint64_t* m = nullptr;
   __device__ __forceinline__ bool operator()(int32_t v) const {
     if (kind == kSelAll) return true;
+    if (kind == kSelMask) return m[v] != 0;
     const int32_t x = a[v];
     return kind == kSelEq ? x == val : x == kInf;
   }
@@ -521,6 +524,99 @@ __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint3
       blocked[x] = 0;
     else
       color[x] = round;
+  }
+}
+
+// ---------------------------------------------------------- batched BFS ---
+//
+// Up to 64 BFS queries, each with its own source and window, share one
+// decode pass per level as bit lanes: bit q of front[v] / vis[v] / next[v]
+// belongs to query q.  The queries containing a timestamp t are found by a
+// binary search over the sorted window boundaries: boundary list b (at most
+// kMaxBounds, ascending), and mask[i] = the queries whose window contains
+// [b[i-1], b[i]) (mask[0] below b[0], mask[nb] from b[nb-1] on).
+constexpr int kMaxQueries = 64;
+constexpr int kMaxBounds = 2 * kMaxQueries;
+
+struct WinTable {
+  const int64_t* bound;
+  const uint64_t* mask;
+  int32_t nb;
+};
+
+struct BfsbAct {
+  Window W;
+  const uint64_t* front;
+  const uint64_t* vis;
+  uint64_t* next;
+  const int64_t* bound;  // LDS copies
+  const uint64_t* mask;
+  int32_t nb;
+  uint64_t fx = 0;
+  __device__ bool begin(int32_t x, bool, Split) {
+    if (!W.active(x)) return false;
+    fx = front[x];
+    return fx != 0;
+  }
+  __device__ __forceinline__ uint64_t winmask(int32_t t) const {
+    int lo = 0, hi = nb;  // first boundary > t
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (bound[mid] <= (int64_t)t) lo = mid + 1; else hi = mid;
+    }
+    return mask[lo];
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
+    if (!W.active(r)) return true;
+    const uint64_t m = fx & winmask(t);
+    if (!m) return true;
+    const uint64_t nv = m & ~vis[r];
+    if (nv && (nv & ~next[r])) atomicOr((unsigned long long*)&next[r], (unsigned long long)nv);
+    return true;
+  }
+  __device__ void end(int32_t, bool) {}
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_bfsb(Streams S, const Ckpt* ck, Items I, Window W, const uint64_t* front,
+                                                 const uint64_t* vis, uint64_t* next, WinTable tab) {
+  __shared__ int64_t bound[kMaxBounds];
+  __shared__ uint64_t mask[kMaxBounds + 1];
+  for (int i = threadIdx.x; i <= tab.nb; i += blockDim.x) {
+    if (i < tab.nb) bound[i] = tab.bound[i];
+    mask[i] = tab.mask[i];
+  }
+  __syncthreads();
+  BfsbAct a{W, front, vis, next, bound, mask, tab.nb};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Ends level `lvl` of a batched BFS: the bits newly set in next[v] become the
+// frontier, join vis[v], and give their queries level lvl + 1 at v (levels
+// is nq rows of n).
+__global__ void __launch_bounds__(kBlock) k_bfsb_finish(int32_t n, uint64_t* front, uint64_t* vis, uint64_t* next,
+                                                        int32_t* levels, int32_t lvl) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t nx = next[v] & ~vis[v];
+    next[v] = 0;
+    front[v] = nx;
+    if (!nx) continue;
+    vis[v] |= nx;
+    for (uint64_t b = nx; b; b &= b - 1) {
+      const int64_t q = __ffsll((unsigned long long)b) - 1;
+      levels[q * n + v] = lvl + 1;
+    }
+  }
+}
+
+// Starts a batched BFS: query q's source gets level 0 and joins the frontier.
+__global__ void k_bfsb_start(int32_t n, const int32_t* src, int32_t nq, uint64_t* front, uint64_t* vis,
+                             int32_t* levels) {
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    const int32_t s = src[q];
+    atomicOr((unsigned long long*)&front[s], 1ull << q);
+    atomicOr((unsigned long long*)&vis[s], 1ull << q);
+    levels[(int64_t)q * n + s] = 0;
   }
 }
 
@@ -939,6 +1035,12 @@ struct tga_graph {
   double* scal = nullptr;     // 4 scalars
   float *f0 = nullptr, *f1 = nullptr;
   int32_t* prio = nullptr;  // default GC priorities, built on first use
+  // batched BFS state, allocated on first use
+  uint64_t *bq_front = nullptr, *bq_vis = nullptr, *bq_next = nullptr, *bq_mask = nullptr;
+  int64_t* bq_bound = nullptr;
+  int32_t* bq_src = nullptr;
+  int32_t* bq_levels = nullptr;  // nq rows of n when the caller's output is on the host
+  int32_t bq_levels_rows = 0;
   int64_t device_bytes = 0;
   std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
@@ -1504,6 +1606,69 @@ extern "C" int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, 
   }
   int rc = g->out_copy(level, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
+  return g->check_err();
+}
+
+extern "C" int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int32_t* src, int32_t nq, int32_t* levels,
+                             uint32_t flags) {
+  if (!g || !windows || !src || !levels || nq < 1 || nq > kMaxQueries)
+    return fail(TGA_EINVAL, "tga_bfs_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q]) || src[q] < 0 || src[q] >= g->n)
+      return fail(TGA_EINVAL, "tga_bfs_batch: bad window or source of query " + std::to_string(q));
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  int rc;
+  if (!g->bq_front) {
+    if ((rc = g->alloc(g->bq_front, (size_t)n)) || (rc = g->alloc(g->bq_vis, (size_t)n)) ||
+        (rc = g->alloc(g->bq_next, (size_t)n)) || (rc = g->alloc(g->bq_bound, kMaxBounds)) ||
+        (rc = g->alloc(g->bq_mask, kMaxBounds + 1)) || (rc = g->alloc(g->bq_src, kMaxQueries)))
+      return rc;
+  }
+  int32_t* lv = levels;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if (g->bq_levels_rows < nq) {
+      if ((rc = g->alloc(g->bq_levels, (size_t)nq * (size_t)n))) return rc;
+      g->bq_levels_rows = nq;
+    }
+    lv = g->bq_levels;
+  }
+  // Window table: sorted boundaries (each window [s, e] contributes s and
+  // e + 1) and the query mask of every elementary interval between them.
+  std::vector<int64_t> b;
+  for (int32_t q = 0; q < nq; ++q) {
+    b.push_back(windows[q].start);
+    b.push_back(windows[q].end + 1);
+  }
+  std::sort(b.begin(), b.end());
+  b.erase(std::unique(b.begin(), b.end()), b.end());
+  std::vector<uint64_t> m(b.size() + 1, 0);
+  for (size_t i = 1; i <= b.size(); ++i)  // interval [b[i-1], b[i]) (b[nb] = +inf)
+    for (int32_t q = 0; q < nq; ++q)
+      if (windows[q].start <= b[i - 1] && b[i - 1] <= windows[q].end) m[i] |= 1ull << q;
+  HIP_TRY(hipMemcpyAsync(g->bq_bound, b.data(), b.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(g->bq_mask, m.data(), m.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(g->bq_src, src, (size_t)nq * 4, hipMemcpyHostToDevice, g->stream));
+  const WinTable tab{g->bq_bound, g->bq_mask, (int32_t)b.size()};
+  k_fill<int32_t><<<g->grid((int64_t)nq * n), kBlock, 0, g->stream>>>(lv, (int64_t)nq * n, kInf);
+  k_fill<uint64_t><<<g->grid(n), kBlock, 0, g->stream>>>(g->bq_front, n, 0);
+  k_fill<uint64_t><<<g->grid(n), kBlock, 0, g->stream>>>(g->bq_vis, n, 0);
+  k_fill<uint64_t><<<g->grid(n), kBlock, 0, g->stream>>>(g->bq_next, n, 0);
+  k_bfsb_start<<<1, kMaxQueries, 0, g->stream>>>((int32_t)n, g->bq_src, nq, g->bq_front, g->bq_vis, lv);
+  HIP_TRY(hipGetLastError());
+  const Window W = g->window(tga_window{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END});
+  for (int32_t lvl = 0;; ++lvl) {
+    uint32_t hc[kGroups + 1];
+    Sel sel{kSelMask, nullptr, 0};
+    sel.m = g->bq_front;
+    if ((rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), sel, false, false, hc))) return rc;
+    if (total_items(hc) == 0) break;
+    LAUNCH_QUEUED(k_bfsb, hc, W, g->bq_front, g->bq_vis, g->bq_next, tab);
+    k_bfsb_finish<<<g->grid(n), kBlock, 0, g->stream>>>((int32_t)n, g->bq_front, g->bq_vis, g->bq_next, lv, lvl);
+    HIP_TRY(hipGetLastError());
+  }
+  if (!(flags & TGA_OUT_DEVICE))
+    HIP_TRY(hipMemcpyAsync(levels, lv, (size_t)nq * (size_t)n * 4, hipMemcpyDeviceToHost, g->stream));
   return g->check_err();
 }
 

@@ -69,6 +69,7 @@ def lib():
         L.tga_neighbor_access.argtypes = [vp, p32, C.c_int64, pu64, p64]
         L.tga_degree_centrality.argtypes = [vp, Window, vp, vp, C.c_uint32]
         L.tga_bfs.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
+        L.tga_bfs_batch.argtypes = [vp, C.POINTER(Window), p32, C.c_int32, vp, C.c_uint32]
         L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
         L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
@@ -184,6 +185,21 @@ class TemporalGraph:
         o = _Out(self.n, np.int32, out)
         _check(lib().tga_bfs(self.h, _win(window), src, o.ptr, o.flags))
         return o.arr
+
+    def bfs_batch(self, srcs, windows=None, out=None):
+        """Batched BFS: up to 64 (source, window) queries in one call.  Returns
+        an (nq, n) int32 array whose row q equals bfs(srcs[q], windows[q]); `out`
+        may be a caller's (nq, n) device tensor instead."""
+        srcs = np.ascontiguousarray(srcs, np.int32)
+        nq = len(srcs)
+        wins = (Window * nq)(*[_win(w) for w in (windows if windows is not None else [None] * nq)])
+        if out is None:
+            arr = np.empty((nq, self.n), np.int32)
+            ptr, flags = arr.ctypes.data_as(C.c_void_p), 0
+        else:
+            arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
+        _check(lib().tga_bfs_batch(self.h, wins, srcs.ctypes.data_as(C.POINTER(C.c_int32)), nq, ptr, flags))
+        return arr
 
     def earliest_arrival(self, src: int, window=None, out=None):
         o = _Out(self.n, np.int32, out)

@@ -115,6 +115,15 @@ int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* c
  * simpleBfs + host level loop, bfs.cu:1657-1692, :1999-2038). */
 int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, uint32_t flags);
 
+/* Batched BFS: nq (1..64) independent queries, query q from src[q] over
+ * windows[q], decoded together -- one pass over the frontier union per level,
+ * the queries carried as bit lanes.  levels receives nq rows of n: row q is
+ * exactly what tga_bfs(g, windows[q], src[q], row, flags) returns.  No
+ * counterpart in the reference (one query per executable run); it is the
+ * batched form of bfs (simpleBfs, bfs.cu:1657-1692) for windowed query mixes. */
+int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int32_t* src, int32_t nq, int32_t* levels,
+                  uint32_t flags);
+
 /* Earliest arrival (the reference GPU BFS rule, bfs.cu:1452-1471): arrival[src]
  * = max(ws, 0); the first window contact (x, r, t) of each pair relaxes r when
  * arrival[x] <= t < arrival[r]; INT32_MAX when unreached. */

@@ -79,6 +79,36 @@ def test_bfs(pair):
             np.testing.assert_array_equal(g.bfs(s, w), ds.oracle().bfs(s, window=w))
 
 
+def test_bfs_batch(pair):
+    """64 queries with random sources and windows (full, ranges, single
+    instants, empty) in one batched call: every row equals the oracle's BFS."""
+    ds, g = pair
+    rng = np.random.default_rng(2024)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    srcs, wins = [], []
+    for q in range(64):
+        srcs.append(int(ds.src[rng.integers(0, len(ds.src))]) if q % 5 else int(rng.integers(0, ds.n)))
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))  # no contact inside
+    srcs[7] = srcs[3]  # a repeated source
+    got = g.bfs_batch(srcs, wins)
+    o = ds.oracle()
+    for q in range(64):
+        np.testing.assert_array_equal(got[q], o.bfs(srcs[q], window=wins[q]), err_msg=f"query {q}")
+    # a smaller batch on the same handle reuses its state
+    got2 = g.bfs_batch(srcs[:5], wins[:5])
+    np.testing.assert_array_equal(got2, got[:5])
+
+
 def test_earliest_arrival(pair):
     ds, g = pair
     for w in windows_for(ds)[:2]:
