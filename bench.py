@@ -98,6 +98,42 @@ def workload_times(g, t_lo: int, t_hi: int, sources: int = 8) -> dict:
     return out
 
 
+def batched_bfs(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 BFS queries with random sources and windows whose lengths are
+    log-uniform in [2^10, 2^24] and starts uniform (the batching mix's BFS
+    share): one tga_bfs_batch call against nq plain tga_bfs calls, outputs on
+    the device in both cases; every batched row is compared with its plain
+    result."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(5)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    wins = [(int(a), int(a + b)) for a, b in zip(starts, lens)]
+    srcs = rng.integers(0, n, nq).astype(np.int32)
+    out = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+    plain = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+    g.bfs_batch(srcs, wins, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.bfs_batch(srcs, wins, out=out)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    g.bfs(int(srcs[0]), wins[0], out=plain[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(nq):
+        g.bfs(int(srcs[q]), wins[q], out=plain[q])
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    same = bool(torch.equal(out, plain))
+    return {"queries": nq, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+            "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1), "speedup": round(tp / tb, 2),
+            "identical": same,
+            "mix": "BFS, random sources, windows log-uniform in [2^10, 2^24] with uniform starts"}
+
+
 def cpu_baseline(ds_dir: str, name: str, n: int, budget_s: float) -> dict:
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -203,6 +239,12 @@ def main() -> int:
                 "kernel": "k_ra (the three concurrent group launches of one pass)",
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
+    batched = None
+    if not args.no_workloads:
+        batched = batched_bfs(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
+        log(f"[bench] batched BFS: {batched}")
+        if not batched["identical"]:
+            raise SystemExit("batched BFS results differ from the plain results")
     workloads = None
     if not args.no_workloads:
         t0 = time.perf_counter()
@@ -236,6 +278,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
+        "batched_bfs": batched,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -591,21 +591,52 @@ __global__ void __launch_bounds__(kBlock) k_bfsb(Streams S, const Ckpt* ck, Item
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
-// Ends level `lvl` of a batched BFS: the bits newly set in next[v] become the
-// frontier, join vis[v], and give their queries level lvl + 1 at v (levels
-// is nq rows of n).
+// Ends step `lvl` of a batched BFS: the bits newly set in next[v] become the
+// frontier, join vis[v], and give their queries level lvl + 1 - start[q] at v
+// (levels is rows of n; start[q] is the step at which query q began, or
+// null for 0).  alive (when given) receives the OR of the new frontier.
 __global__ void __launch_bounds__(kBlock) k_bfsb_finish(int32_t n, uint64_t* front, uint64_t* vis, uint64_t* next,
-                                                        int32_t* levels, int32_t lvl) {
+                                                        int32_t* levels, int32_t lvl, const int32_t* start,
+                                                        unsigned long long* alive) {
+  uint64_t any = 0;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t nx = next[v] & ~vis[v];
     next[v] = 0;
     front[v] = nx;
     if (!nx) continue;
+    any |= nx;
     vis[v] |= nx;
     for (uint64_t b = nx; b; b &= b - 1) {
       const int64_t q = __ffsll((unsigned long long)b) - 1;
-      levels[q * n + v] = lvl + 1;
+      levels[q * n + v] = lvl + 1 - (start ? start[q] : 0);
     }
+  }
+  if (alive) {
+    for (int off = 32; off > 0; off >>= 1) any |= (uint64_t)__shfl_xor((unsigned long long)any, off, 64);
+    if ((threadIdx.x & 63) == 0 && any) atomicOr(alive, (unsigned long long)any);
+  }
+}
+
+// Admits the lanes of `lanes` (a bit mask) into a running batched BFS: clears
+// their bits from vis over all vertices and their level rows, then starts
+// lane q at src[q].
+__global__ void __launch_bounds__(kBlock) k_bfsb_admit(int32_t n, uint64_t lanes, const int32_t* src, uint64_t* vis,
+                                                       uint64_t* front, int32_t* levels) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    vis[v] &= ~lanes;
+    for (uint64_t b = lanes; b; b &= b - 1) levels[(int64_t)(__ffsll((unsigned long long)b) - 1) * n + v] = kInf;
+  }
+}
+
+__global__ void k_bfsb_admit_src(int32_t n, uint64_t lanes, const int32_t* src, uint64_t* vis, uint64_t* front,
+                                 int32_t* levels) {
+  if (threadIdx.x != 0) return;
+  for (uint64_t b = lanes; b; b &= b - 1) {
+    const int q = __ffsll((unsigned long long)b) - 1;
+    const int32_t s = src[q];
+    front[s] |= 1ull << q;
+    vis[s] |= 1ull << q;
+    levels[(int64_t)q * n + s] = 0;
   }
 }
 
@@ -1664,12 +1695,158 @@ extern "C" int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int3
     if ((rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), sel, false, false, hc))) return rc;
     if (total_items(hc) == 0) break;
     LAUNCH_QUEUED(k_bfsb, hc, W, g->bq_front, g->bq_vis, g->bq_next, tab);
-    k_bfsb_finish<<<g->grid(n), kBlock, 0, g->stream>>>((int32_t)n, g->bq_front, g->bq_vis, g->bq_next, lv, lvl);
+    k_bfsb_finish<<<g->grid(n), kBlock, 0, g->stream>>>((int32_t)n, g->bq_front, g->bq_vis, g->bq_next, lv, lvl,
+                                                        nullptr, nullptr);
     HIP_TRY(hipGetLastError());
   }
   if (!(flags & TGA_OUT_DEVICE))
     HIP_TRY(hipMemcpyAsync(levels, lv, (size_t)nq * (size_t)n * 4, hipMemcpyDeviceToHost, g->stream));
   return g->check_err();
+}
+
+// ------------------------------------------------------------ BFS server --
+//
+// Continuous batching of BFS queries over the batched kernels: up to 64 lanes;
+// queries are admitted into free lanes between steps, every step advances all
+// running lanes by one level in one decode pass, and a lane whose frontier
+// empties is finished (its level row is final) until its result is taken.
+struct tga_bfs_server {
+  tga_graph* g = nullptr;
+  uint64_t *front = nullptr, *vis = nullptr, *next = nullptr, *mask = nullptr;
+  int64_t* bound = nullptr;
+  int32_t *src = nullptr, *start = nullptr, *levels = nullptr;
+  unsigned long long* alive = nullptr;
+  int32_t nb = 0;
+  int32_t step = 0;
+  uint64_t running = 0, pending = 0, done = 0;
+  int32_t h_src[kMaxQueries] = {};
+  int32_t h_start[kMaxQueries] = {};
+  tga_window h_win[kMaxQueries] = {};
+  std::vector<void*> mem;
+  template <class T>
+  int alloc(T*& p, size_t count) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return fail(TGA_ENOMEM, "hipMalloc failed");
+    mem.push_back(q);
+    p = (T*)q;
+    return TGA_OK;
+  }
+  ~tga_bfs_server() {
+    if (g && g->stream) (void)hipStreamSynchronize(g->stream);
+    for (void* p : mem) (void)hipFree(p);
+  }
+};
+
+extern "C" int tga_bfs_server_open(tga_graph* g, tga_bfs_server** out) {
+  if (!g || !out) return fail(TGA_EINVAL, "tga_bfs_server_open: NULL argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(g->device));
+  auto* s = new tga_bfs_server();
+  s->g = g;
+  const size_t n = (size_t)g->n;
+  int rc;
+  if ((rc = s->alloc(s->front, n)) || (rc = s->alloc(s->vis, n)) || (rc = s->alloc(s->next, n)) ||
+      (rc = s->alloc(s->mask, kMaxBounds + 1)) || (rc = s->alloc(s->bound, kMaxBounds)) ||
+      (rc = s->alloc(s->src, kMaxQueries)) || (rc = s->alloc(s->start, kMaxQueries)) ||
+      (rc = s->alloc(s->levels, (size_t)kMaxQueries * n)) || (rc = s->alloc(s->alive, 1))) {
+    delete s;
+    return rc;
+  }
+  k_fill<uint64_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(s->front, g->n, 0);
+  k_fill<uint64_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(s->vis, g->n, 0);
+  k_fill<uint64_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(s->next, g->n, 0);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) {
+    delete s;
+    return fail(TGA_EDEVICE, "tga_bfs_server_open: initialisation failed");
+  }
+  *out = s;
+  return TGA_OK;
+}
+
+extern "C" void tga_bfs_server_close(tga_bfs_server* s) { delete s; }
+
+extern "C" int tga_bfs_server_admit(tga_bfs_server* s, int32_t src, tga_window w, int32_t* lane) {
+  if (!s || !lane || bad_window(w) || src < 0 || src >= s->g->n)
+    return fail(TGA_EINVAL, "tga_bfs_server_admit: bad arguments");
+  const uint64_t used = s->running | s->pending | s->done;
+  if (~used == 0) return fail(TGA_EBUSY, "tga_bfs_server_admit: all 64 lanes are busy");
+  const int q = __builtin_ctzll(~used);
+  s->h_src[q] = src;
+  s->h_win[q] = w;
+  s->pending |= 1ull << q;
+  *lane = q;
+  return TGA_OK;
+}
+
+extern "C" int tga_bfs_server_step(tga_bfs_server* s, uint64_t* finished) {
+  if (!s || !finished) return fail(TGA_EINVAL, "tga_bfs_server_step: NULL argument");
+  tga_graph* g = s->g;
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  *finished = 0;
+  if (s->pending) {
+    for (uint64_t b = s->pending; b; b &= b - 1) s->h_start[__builtin_ctzll(b)] = s->step;
+    s->running |= s->pending;
+    // window table of the running lanes (see tga_bfs_batch)
+    std::vector<int64_t> bd;
+    for (uint64_t b = s->running; b; b &= b - 1) {
+      const int q = __builtin_ctzll(b);
+      bd.push_back(s->h_win[q].start);
+      bd.push_back(s->h_win[q].end + 1);
+    }
+    std::sort(bd.begin(), bd.end());
+    bd.erase(std::unique(bd.begin(), bd.end()), bd.end());
+    std::vector<uint64_t> m(bd.size() + 1, 0);
+    for (size_t i = 1; i <= bd.size(); ++i)
+      for (uint64_t b = s->running; b; b &= b - 1) {
+        const int q = __builtin_ctzll(b);
+        if (s->h_win[q].start <= bd[i - 1] && bd[i - 1] <= s->h_win[q].end) m[i] |= 1ull << q;
+      }
+    s->nb = (int32_t)bd.size();
+    HIP_TRY(hipMemcpyAsync(s->bound, bd.data(), bd.size() * 8, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(s->mask, m.data(), m.size() * 8, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(s->src, s->h_src, sizeof s->h_src, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(s->start, s->h_start, sizeof s->h_start, hipMemcpyHostToDevice, g->stream));
+    k_bfsb_admit<<<g->grid(n), kBlock, 0, g->stream>>>((int32_t)n, s->pending, s->src, s->vis, s->front, s->levels);
+    k_bfsb_admit_src<<<1, 64, 0, g->stream>>>((int32_t)n, s->pending, s->src, s->vis, s->front, s->levels);
+    HIP_TRY(hipGetLastError());
+    s->pending = 0;
+  }
+  if (!s->running) return TGA_OK;
+  uint32_t hc[kGroups + 1];
+  Sel sel{kSelMask, nullptr, 0};
+  sel.m = s->front;
+  int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), sel, false, false, hc);
+  if (rc) return rc;
+  uint64_t alive = 0;
+  if (total_items(hc) > 0) {
+    const WinTable tab{s->bound, s->mask, s->nb};
+    const Window W = g->window(tga_window{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END});
+    LAUNCH_QUEUED(k_bfsb, hc, W, s->front, s->vis, s->next, tab);
+    HIP_TRY(hipMemsetAsync(s->alive, 0, 8, g->stream));
+    k_bfsb_finish<<<g->grid(n), kBlock, 0, g->stream>>>((int32_t)n, s->front, s->vis, s->next, s->levels, s->step,
+                                                        s->start, s->alive);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(&alive, s->alive, 8, hipMemcpyDeviceToHost, g->stream));
+  }
+  if ((rc = g->check_err())) return rc;
+  ++s->step;
+  *finished = s->running & ~alive;
+  s->done |= *finished;
+  s->running &= alive;
+  return TGA_OK;
+}
+
+extern "C" int tga_bfs_server_result(tga_bfs_server* s, int32_t lane, int32_t* level, uint32_t flags) {
+  if (!s || !level || lane < 0 || lane >= kMaxQueries || !(s->done >> lane & 1))
+    return fail(TGA_EINVAL, "tga_bfs_server_result: lane has no finished query");
+  tga_graph* g = s->g;
+  HIP_TRY(hipSetDevice(g->device));
+  int rc = g->out_copy(level, s->levels + (int64_t)lane * g->n, (size_t)g->n * 4, flags);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  s->done &= ~(1ull << lane);
+  return TGA_OK;
 }
 
 extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arrival, uint32_t flags) {

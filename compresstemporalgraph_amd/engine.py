@@ -70,6 +70,12 @@ def lib():
         L.tga_degree_centrality.argtypes = [vp, Window, vp, vp, C.c_uint32]
         L.tga_bfs.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_bfs_batch.argtypes = [vp, C.POINTER(Window), p32, C.c_int32, vp, C.c_uint32]
+        L.tga_bfs_server_open.argtypes = [vp, C.POINTER(vp)]
+        L.tga_bfs_server_admit.argtypes = [vp, C.c_int32, Window, p32]
+        L.tga_bfs_server_step.argtypes = [vp, pu64]
+        L.tga_bfs_server_result.argtypes = [vp, C.c_int32, vp, C.c_uint32]
+        L.tga_bfs_server_close.argtypes = [vp]
+        L.tga_bfs_server_close.restype = None
         L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
         L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
@@ -129,6 +135,7 @@ class TemporalGraph:
         h = C.c_void_p()
         _check(lib().tga_open_dir(directory.encode(), name.encode(), C.byref(desc), device, C.byref(h)))
         self.h = h
+        self.device = device
         info = GraphInfo()
         _check(lib().tga_info(self.h, C.byref(info)))
         self.info = {k: getattr(info, k) for k, _ in GraphInfo._fields_}
@@ -201,6 +208,10 @@ class TemporalGraph:
         _check(lib().tga_bfs_batch(self.h, wins, srcs.ctypes.data_as(C.POINTER(C.c_int32)), nq, ptr, flags))
         return arr
 
+    def bfs_server(self):
+        """Opens a BFS server (continuous batching of up to 64 queries)."""
+        return BfsServer(self)
+
     def earliest_arrival(self, src: int, window=None, out=None):
         o = _Out(self.n, np.int32, out)
         _check(lib().tga_earliest_arrival(self.h, _win(window), src, o.ptr, o.flags))
@@ -252,3 +263,44 @@ class TemporalGraph:
         if w == "DC":
             return self.degree_centrality(kw.get("window"))
         return self.graph_coloring(kw.get("window"), kw.get("rounds", 512))
+
+
+class BfsServer:
+    """Continuous batching of BFS queries on one TemporalGraph: admit() puts a
+    query into a free lane, step() advances every running query one level in
+    a single decode pass and returns the lanes that finished, result(lane)
+    returns a finished query's levels (equal to TemporalGraph.bfs) and frees
+    the lane."""
+
+    LANES = 64
+
+    def __init__(self, graph: "TemporalGraph"):
+        self.graph = graph
+        self.h = C.c_void_p()
+        _check(lib().tga_bfs_server_open(graph.h, C.byref(self.h)))
+
+    def admit(self, src: int, window=None) -> int:
+        lane = C.c_int32()
+        _check(lib().tga_bfs_server_admit(self.h, int(src), _win(window), C.byref(lane)))
+        return lane.value
+
+    def step(self) -> list:
+        fin = C.c_uint64()
+        _check(lib().tga_bfs_server_step(self.h, C.byref(fin)))
+        return [q for q in range(self.LANES) if fin.value >> q & 1]
+
+    def result(self, lane: int, out=None):
+        o = _Out(self.graph.n, np.int32, out)
+        _check(lib().tga_bfs_server_result(self.h, int(lane), o.ptr, o.flags))
+        return o.arr
+
+    def close(self) -> None:
+        if self.h:
+            lib().tga_bfs_server_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -41,7 +41,8 @@ enum {
   TGA_EFORMAT = -3,  /* malformed stream or index: out-of-range neighbor, chain deeper than 3, overrun */
   TGA_ERANGE = -4,   /* dataset outside what the reader supports (log2(0) index width, bfs.cu:1785) */
   TGA_EDEVICE = -5,  /* HIP runtime failure */
-  TGA_ENOMEM = -6
+  TGA_ENOMEM = -6,
+  TGA_EBUSY = -7     /* no free lane (BFS server) */
 };
 
 #define TGA_OUT_DEVICE 1u
@@ -123,6 +124,23 @@ int tga_bfs(tga_graph* g, tga_window w, int32_t src, int32_t* level, uint32_t fl
  * batched form of bfs (simpleBfs, bfs.cu:1657-1692) for windowed query mixes. */
 int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int32_t* src, int32_t nq, int32_t* levels,
                   uint32_t flags);
+
+/* BFS server: continuous batching of BFS queries on one handle (no
+ * counterpart in the reference).  Up to 64 queries run at once as bit lanes.
+ * tga_bfs_server_admit queues a query into a free lane (TGA_EBUSY when all 64
+ * lanes are taken); each tga_bfs_server_step admits the queued queries and
+ * advances every running query by one BFS level in one decode pass, setting
+ * the bits of the lanes whose query finished in `finished`;
+ * tga_bfs_server_result copies a finished lane's levels (exactly tga_bfs's
+ * output for that query, whatever ran beside it) and frees the lane.  The
+ * server uses its handle's stream and work queues: do not interleave other
+ * calls on the same handle from another thread. */
+typedef struct tga_bfs_server tga_bfs_server;
+int tga_bfs_server_open(tga_graph* g, tga_bfs_server** out);
+int tga_bfs_server_admit(tga_bfs_server* s, int32_t src, tga_window w, int32_t* lane);
+int tga_bfs_server_step(tga_bfs_server* s, uint64_t* finished);
+int tga_bfs_server_result(tga_bfs_server* s, int32_t lane, int32_t* level, uint32_t flags);
+void tga_bfs_server_close(tga_bfs_server* s);
 
 /* Earliest arrival (the reference GPU BFS rule, bfs.cu:1452-1471): arrival[src]
  * = max(ws, 0); the first window contact (x, r, t) of each pair relaxes r when

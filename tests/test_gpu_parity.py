@@ -109,6 +109,34 @@ def test_bfs_batch(pair):
     np.testing.assert_array_equal(got2, got[:5])
 
 
+def test_bfs_server_mixed_admission(pair):
+    """Queries admitted at different steps, more than there are lanes over
+    the run, lanes reused: each result equals the oracle's BFS alone."""
+    ds, g = pair
+    rng = np.random.default_rng(77)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    queries = []
+    for q in range(90):
+        a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+        queries.append((int(ds.src[rng.integers(0, len(ds.src))]), None if q % 3 == 0 else (a, b)))
+    o = ds.oracle()
+    with g.bfs_server() as srv:
+        lane_of, nxt, got = {}, 0, {}
+        while nxt < len(queries) or lane_of:
+            # admit a few queries per step (fewer than the free lanes)
+            for _ in range(int(rng.integers(0, 6))):
+                if nxt < len(queries) and len(lane_of) < 64:
+                    lane_of[srv.admit(*queries[nxt])] = nxt
+                    nxt += 1
+            if not lane_of:
+                continue
+            for lane in srv.step():
+                got[lane_of.pop(lane)] = srv.result(lane)
+    assert len(got) == len(queries)
+    for i, (s, w) in enumerate(queries):
+        np.testing.assert_array_equal(got[i], o.bfs(s, window=w), err_msg=f"query {i}")
+
+
 def test_earliest_arrival(pair):
     ds, g = pair
     for w in windows_for(ds)[:2]:
