@@ -18,6 +18,12 @@ Prints one JSON line with the driver's contract fields plus:
   cpu_baseline  the CPU oracle's neighbor access (single thread, the build's
                 C restatement of the reference reader, kind "port") timed on
                 this host over a bounded vertex sample.
+  workloads_ms  the seven entry points (plus earliest arrival) over windows
+                covering 100%, 50% and 10% of the timestamp span.
+  batched_bfs   64 windowed BFS queries in one batched call against 64 plain
+                calls (results compared, outputs in HBM).
+  served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
+                against sequential plain BFS, queries/s and p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -134,6 +140,49 @@ def batched_bfs(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
             "mix": "BFS, random sources, windows log-uniform in [2^10, 2^24] with uniform starts"}
 
 
+def served_bfs(g, t_lo: int, t_hi: int) -> dict:
+    """Serving: BFS window queries at Poisson rates of 1k and 10k per second
+    (log-uniform window lengths in [2^10, 2^24], random sources), continuous
+    batching against sequential plain BFS on the same arrivals; results stay
+    in HBM; identity of all results is checked at 1k/s."""
+    import torch
+    from compresstemporalgraph_amd import serving
+    out = {}
+    for rate, count in ((1000, 500), (10000, 2000)):
+        req = serving.poisson_requests(rate, count, g.info["n"], t_lo, t_hi, seed=rate)
+        kb, kp = ({}, {}) if rate == 1000 else (None, None)
+        sb = serving.serve_batched(g, req, keep=kb, device_out=True)
+        sp = serving.serve_plain(g, req, keep=kp, device_out=True)
+        row = {"batched": sb, "plain": sp}
+        if kb is not None:
+            row["identical"] = bool(all(torch.equal(kb[i], kp[i]) for i in kb)) and len(kb) == count
+            if not row["identical"]:
+                raise SystemExit("served BFS results differ from the plain results")
+        out[f"{rate}/s"] = row
+    return out
+
+
+def query_legs(args) -> int:
+    """Batched and served BFS legs (child process of the default run)."""
+    import torch
+    torch.zeros(1, device="cuda:0")
+    from compresstemporalgraph_amd import datasets
+    from compresstemporalgraph_amd.engine import TemporalGraph
+    root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
+    desc = datasets.ensure(root, args.workload)
+    g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    t_lo, t_hi = desc.get("t_min", 0), desc.get("t_max", 1 << 24)
+    batched = batched_bfs(g, t_lo, t_hi)
+    log(f"[bench] batched BFS: {batched}")
+    if not batched["identical"]:
+        raise SystemExit("batched BFS results differ from the plain results")
+    served = served_bfs(g, t_lo, t_hi)
+    log(f"[bench] served BFS: {served}")
+    g.close()
+    print(json.dumps({"batched_bfs": batched, "served_bfs": served}), flush=True)
+    return 0
+
+
 def cpu_baseline(ds_dir: str, name: str, n: int, budget_s: float) -> dict:
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -169,7 +218,10 @@ def main() -> int:
     ap.add_argument("--data-root", default=None)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-workloads", action="store_true", help="skip the seven-workload timing table")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the seven-workload table and the batched / served query legs")
+    ap.add_argument("--query-legs", action="store_true",
+                    help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="HBM bytes per pass from rocprof FETCH_SIZE + WRITE_SIZE (default: the committed "
                          "measurement in profiles/ when it matches the engine sources)")
@@ -193,6 +245,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"[bench] copied {nbytes} bytes three times")
         return 0
+
+    if args.query_legs:
+        return query_legs(args)
 
     from compresstemporalgraph_amd import datasets
     from compresstemporalgraph_amd.engine import TemporalGraph
@@ -239,18 +294,28 @@ def main() -> int:
                 "kernel": "k_ra (the three concurrent group launches of one pass)",
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
-    batched = None
-    if not args.no_workloads:
-        batched = batched_bfs(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
-        log(f"[bench] batched BFS: {batched}")
-        if not batched["identical"]:
-            raise SystemExit("batched BFS results differ from the plain results")
     workloads = None
     if not args.no_workloads:
         t0 = time.perf_counter()
         workloads = workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
+
+    batched = served = None
+    if not args.no_workloads:
+        # The query legs keep results in torch device tensors.  torch brings
+        # its own HIP runtime (which must initialise before the engine's in a
+        # process, and whose streams share the hardware queues), so they run
+        # in a child process and leave the pass timing above untouched.
+        import subprocess
+        cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
+        if args.data_root:
+            cmd += ["--data-root", args.data_root]
+        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        if child.returncode != 0:
+            raise SystemExit(f"query legs failed (exit {child.returncode})")
+        legs = json.loads(child.stdout.strip().splitlines()[-1])
+        batched, served = legs["batched_bfs"], legs["served_bfs"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -279,6 +344,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
         "batched_bfs": batched,
+        "served_bfs": served,
     }
     print(json.dumps(line), flush=True)
     return 0
