@@ -192,16 +192,23 @@ enum : int32_t {
   kSelEq = 1,         // a[v] == val (BFS level, round stamp of a changed vertex)
   kSelUncolored = 2,  // a[v] == kInf (graph coloring's worklist)
   kSelMask = 3,       // m[v] != 0 (batched BFS: some query has v in its frontier)
+  kSelGc = 4,         // a[v] == kInf and not held by its cached blocker: b[v] < 0 or a[b[v]] != kInf
 };
 struct Sel {
   int32_t kind;
   const int32_t* a;
   int32_t val;
   const uint64_t* m = nullptr;
+  const int32_t* b = nullptr;
   __device__ __forceinline__ bool operator()(int32_t v) const {
     if (kind == kSelAll) return true;
     if (kind == kSelMask) return m[v] != 0;
     const int32_t x = a[v];
+    if (kind == kSelGc) {
+      if (x != kInf) return false;
+      const int32_t u = b[v];
+      return u < 0 || a[u] != kInf;
+    }
     return kind == kSelEq ? x == val : x == kInf;
   }
 };
@@ -471,15 +478,18 @@ __global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
 // uncolored at the start of the round (its color is kInf, or `round` if it was
 // colored earlier in this same round).  Chunks of a long record report to
-// blocked[x]; k_gc_finish decides those vertices.  Blocked vertices stay
-// uncolored and are selected again next round.
+// blocked[x]; k_gc_finish decides those vertices.  A blocked vertex caches
+// its blocker in blocker[x] and is not decoded again while that blocker
+// stays uncolored (one uncolored higher-priority neighbor is enough to block,
+// and priorities, windows and activity do not change between rounds).
 struct GcAct {
   Window W;
   const int32_t* prio;
   int32_t* color;
   int32_t* blocked;
+  int32_t* blocker;
   int32_t round;
-  int32_t v = 0, pv = 0;
+  int32_t v = 0, pv = 0, bu = -1;
   bool blk = false;
   __device__ bool begin(int32_t x, bool chunked, Split) {
     if (!W.active(x) || color[x] != kInf) return false;
@@ -494,12 +504,14 @@ struct GcAct {
       const int32_t pu = prio[u];
       if ((pu > pv || (pu == pv && u < v)) && color[u] >= round) {
         blk = true;
+        bu = u;
         return false;
       }
     }
     return true;
   }
   __device__ void end(int32_t x, bool chunked) {
+    if (blk) blocker[x] = bu;  // any chunk's blocker will do
     if (chunked) {
       if (blk) atomicOr(&blocked[x], 1);
     } else if (!blk) {
@@ -510,8 +522,8 @@ struct GcAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_gc(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
-                                               int32_t* color, int32_t* blocked, int32_t round) {
-  GcAct a{W, prio, color, blocked, round};
+                                               int32_t* color, int32_t* blocked, int32_t* blocker, int32_t round) {
+  GcAct a{W, prio, color, blocked, blocker, round};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
@@ -1936,14 +1948,20 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
   }
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i2, g->n, 0);
+  int32_t* blocker = g->q0;  // cached blocker per vertex (-1: none)
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(blocker, g->n, -1);
   for (int32_t round = 0; round < rounds; ++round) {
     // Worklist: the active vertices still uncolored (vertices without
-    // contacts included: they are colored in the first round).
+    // contacts included: they are colored in the first round) whose cached
+    // blocker, if any, got colored.  Empty only when every vertex is colored:
+    // the highest-priority uncolored vertex never has an uncolored blocker.
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelUncolored, g->i0, 0}, true, true, hc);
+    Sel sel{kSelGc, g->i0, 0};
+    sel.b = blocker;
+    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), sel, true, true, hc);
     if (rc) return rc;
     if (total_items(hc) == 0) break;
-    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, round);
+    LAUNCH_QUEUED(k_gc, hc, W, prio, g->i0, g->i2, blocker, round);
     if (hc[kGroups] > 0) {
       k_gc_finish<<<g->grid(hc[kGroups]), kBlock, 0, g->stream>>>(g->xq_big, hc[kGroups], W, g->i0, g->i2, round);
       HIP_TRY(hipGetLastError());
