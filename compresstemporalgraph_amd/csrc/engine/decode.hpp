@@ -770,6 +770,14 @@ __device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& t
   }
 }
 
+// Actions that keep per-item state define item(code); it runs first.
+template <class Act>
+__device__ __forceinline__ auto act_item(Act& a, int32_t code, int) -> decltype(a.item(code), void()) {
+  a.item(code);
+}
+template <class Act>
+__device__ __forceinline__ void act_item(Act&, int32_t, long) {}
+
 // Runs one work item.  code >= 0: the whole record of vertex `code`;
 // code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked,
 // split) may veto the item; act.end(x, chunked) runs after the walk.
@@ -779,6 +787,7 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
   TsCursor tc;
   int32_t x, k = 0, kend;
   const bool chunked = code < 0;
+  act_item(act, code, 0);
   if (!chunked) {
     x = code;
     if (!act.begin(x, false, Split{-1, -1, -1, -1})) return;

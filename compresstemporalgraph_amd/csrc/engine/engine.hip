@@ -539,6 +539,258 @@ __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint3
   }
 }
 
+// ------------------------------------------- graph coloring over lists ---
+//
+// The rounds above decode every candidate again.  When memory allows, the
+// records are decoded twice in all instead: a counting pass and a filling
+// pass write, for each active vertex x, the list of window contacts to an
+// active u that outranks x (priority[u] > priority[x], ties to the smaller
+// id).  x is blocked in round c iff some u on its list has color[u] >= c.
+// Colors never change once set, so an entry with color[u] < c never blocks
+// again: each vertex keeps a cursor that only moves forward, and a round is
+// one scan step per candidate (a whole wave per candidate for long lists).
+//
+// List layout: item index i = code for a whole record, n + j for checkpoint
+// chunk j; off[] is the exclusive scan of the per-item counts, so the list
+// of a chunked vertex (its chunks are consecutive checkpoints) is one range.
+__device__ __forceinline__ int32_t item_index(int32_t code, int32_t n) { return code >= 0 ? code : n + (-code - 1); }
+
+struct GcListAct {
+  Window W;
+  const int32_t* prio;
+  uint32_t* cnt;       // counting pass: per item
+  const uint64_t* off;  // filling pass: per item
+  int32_t* hp;
+  int32_t n;
+  uint64_t pos = 0;
+  uint32_t c = 0;
+  int32_t idx = 0, v = 0, pv = 0;
+  __device__ void item(int32_t code) { idx = item_index(code, n); }
+  __device__ bool begin(int32_t x, bool, Split) {
+    if (!W.active(x)) return false;
+    v = x;
+    pv = prio[x];
+    c = 0;
+    if (off) pos = off[idx];
+    return true;
+  }
+  __device__ bool contact(int32_t, int32_t u, int32_t t, int32_t) {
+    if (W.in(t) && W.active(u)) {
+      const int32_t pu = prio[u];
+      if (pu > pv || (pu == pv && u < v)) {
+        if (off)
+          hp[pos++] = u;
+        else
+          ++c;
+      }
+    }
+    return true;
+  }
+  __device__ void end(int32_t, bool) {
+    if (!off) cnt[idx] = c;
+  }
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_gc_list(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
+                                                    uint32_t* cnt, const uint64_t* off, int32_t* hp) {
+  GcListAct a{W, prio, cnt, off, hp, S.n};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Exclusive prefix of v over the block (thread order); total receives the sum.
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t& total) {
+  __shared__ uint64_t ws[kBlock / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t u = __shfl_up((unsigned long long)inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  if (lane == 63) ws[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0;
+  total = 0;
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) before += ws[w];
+    total += ws[w];
+  }
+  __syncthreads();
+  return before + inc - v;
+}
+
+// Exclusive scan of m 32-bit counts into m + 1 64-bit offsets, in tiles of
+// kScanTile: tile sums, a one-block scan of the tile sums, then each tile.
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kBlock * kScanPer;
+
+__device__ __forceinline__ uint64_t scan_thread_sum(const uint32_t* c, int64_t m, int64_t b) {
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j) s += b + j < m ? c[b + j] : 0u;
+  return s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const uint32_t* c, int64_t m, uint64_t* tsum) {
+  const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  uint64_t total;
+  block_excl_scan64(scan_thread_sum(c, m, b), total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_top(uint64_t* tsum, int64_t tiles) {
+  uint64_t carry = 0;
+  for (int64_t b = 0; b < tiles; b += kBlock) {
+    const int64_t i = b + threadIdx.x;
+    const uint64_t v = i < tiles ? tsum[i] : 0;
+    uint64_t total;
+    const uint64_t pre = block_excl_scan64(v, total);
+    if (i < tiles) tsum[i] = carry + pre;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const uint32_t* c, int64_t m, const uint64_t* tpre,
+                                                       uint64_t* off) {
+  const int64_t b = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  uint64_t total;
+  uint64_t run = tpre[blockIdx.x] + block_excl_scan64(scan_thread_sum(c, m, b), total);
+#pragma unroll
+  for (int j = 0; j < kScanPer; ++j) {
+    if (b + j <= m) off[b + j] = run;
+    if (b + j < m) run += c[b + j];
+  }
+}
+
+// Candidates are (vertex, cursor, blocker, -): the blocker is the list entry
+// at the cursor (-1 before the first scan); vertex -1 marks a candidate
+// colored since the last compaction.  A round reads the blocker's color
+// first; only when it got colored does the list scan resume, and only then
+// is the candidate written back (rounds take no atomics: k_gc_compact drops
+// the colored candidates every few rounds).  Scans with more than kGcSerial
+// entries left are done by the whole wave, 64 entries a step, one such
+// candidate after another; shorter ones by their own lane.
+constexpr int32_t kGcSerial = 64;
+
+// List range of every active vertex, and the first candidates: every active
+// vertex, blocked by its first entry if it has one, in vertex order.
+__global__ void __launch_bounds__(kBlock) k_gc_ranges(int32_t ab, int32_t ae, int32_t n, const int32_t* vinfo,
+                                                      const Ckpt* ck, const uint64_t* off, const int32_t* hp,
+                                                      uint64_t* lbeg, int32_t* llen, int4* list, uint32_t* count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = (uint32_t)(ae - ab);
+  for (int64_t v = ab + (int64_t)blockIdx.x * kBlock + threadIdx.x; v < ae; v += (int64_t)gridDim.x * kBlock) {
+    const int32_t info = vinfo[v];
+    uint64_t b, end;
+    if (info >= 0) {
+      b = off[n + info];
+      end = off[n + info + ck[info].nchunk];
+    } else {
+      b = off[v];
+      end = off[v + 1];
+    }
+    lbeg[v] = b;
+    llen[v] = (int32_t)(end - b);
+    list[v - ab] = int4{(int32_t)v, 0, end > b ? hp[b] : -1, 0};
+  }
+}
+
+// One coloring round over list[0, *cnt), in place.
+__global__ void __launch_bounds__(kBlock) k_gc_round(int4* list, const uint32_t* cnt_p, const int32_t* hp,
+                                                     const uint64_t* lbeg, const int32_t* llen, int32_t* color,
+                                                     int32_t round) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t cnt = *cnt_p;
+  for (uint32_t i0 = blockIdx.x * kBlock + (threadIdx.x & ~63u); i0 < cnt; i0 += gridDim.x * kBlock) {
+    const uint32_t i = i0 + lane;
+    int4 e{-1, 0, -1, 0};
+    bool coop = false, dirty = false;
+    int32_t len = 0, start = 0;
+    uint64_t lb = 0;
+    if (i < cnt) e = list[i];
+    if (e.x >= 0 && !(e.z >= 0 && color[e.z] >= round)) {
+      // the blocker got colored (or there was none yet): resume the scan
+      dirty = true;
+      len = llen[e.x];
+      lb = lbeg[e.x];
+      start = e.z >= 0 ? e.y + 1 : e.y;
+      if (len - start > kGcSerial) {
+        coop = true;
+      } else {
+        bool blocked = false;
+        for (int32_t c = start; c < len; ++c) {
+          const int32_t u = hp[lb + c];
+          if (color[u] >= round) {
+            blocked = true;
+            e.y = c;
+            e.z = u;
+            break;
+          }
+        }
+        if (!blocked) {
+          color[e.x] = round;
+          e.x = -1;
+        }
+      }
+    }
+    uint64_t todo = __ballot(coop);
+    while (todo) {
+      const int l = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const int32_t s = __shfl(start, l, 64), ln = __shfl(len, l, 64);
+      const uint64_t b = __shfl((unsigned long long)lb, l, 64);
+      int32_t found = -1, fu = -1;
+      for (int32_t c = s; c < ln; c += 64) {
+        const int32_t j = c + lane;
+        const int32_t u = j < ln ? hp[b + j] : -1;
+        const uint64_t m = __ballot(u >= 0 && color[u] >= round);
+        if (m) {
+          const int f = __ffsll((unsigned long long)m) - 1;
+          found = c + f;
+          fu = __shfl(u, f, 64);
+          break;
+        }
+      }
+      if (lane == l) {
+        if (found >= 0) {
+          e.y = found;
+          e.z = fu;
+        } else {
+          color[e.x] = round;
+          e.x = -1;
+        }
+      }
+    }
+    if (dirty) list[i] = e;
+  }
+}
+
+// Copies the live candidates of in[0, *cin) to out, appending at *cout (one
+// atomic per block; each block keeps its slice's order).
+__global__ void __launch_bounds__(kBlock) k_gc_compact(const int4* in, const uint32_t* cin, int4* out,
+                                                       uint32_t* cout) {
+  __shared__ uint32_t at;
+  const uint32_t count = *cin;
+  const uint32_t per = (count + gridDim.x - 1) / gridDim.x;
+  const uint32_t slice = (per + kBlock - 1) / kBlock * kBlock;
+  const uint32_t lo = min(count, blockIdx.x * slice), hi = min(count, lo + slice);
+  uint64_t own = 0, tot;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) own += in[i].x >= 0;
+  block_excl_scan64(own, tot);
+  if (threadIdx.x == 0) at = tot ? atomicAdd(cout, (uint32_t)tot) : 0u;
+  __syncthreads();
+  uint32_t run = at;
+  for (uint32_t t0 = lo; t0 < hi; t0 += kBlock) {
+    const uint32_t i = t0 + threadIdx.x;
+    int4 e{-1, 0, -1, 0};
+    if (i < hi) e = in[i];
+    uint64_t t;
+    const uint32_t pre = (uint32_t)block_excl_scan64(e.x >= 0 ? 1u : 0u, t);
+    if (e.x >= 0) out[run + pre] = e;
+    run += (uint32_t)t;
+  }
+}
+
 // ---------------------------------------------------------- batched BFS ---
 //
 // Up to 64 BFS queries, each with its own source and window, share one
@@ -1078,6 +1330,15 @@ struct tga_graph {
   double* scal = nullptr;     // 4 scalars
   float *f0 = nullptr, *f1 = nullptr;
   int32_t* prio = nullptr;  // default GC priorities, built on first use
+  // graph coloring lists, allocated on first use (k_gc_round)
+  uint32_t* gc_cnt = nullptr;   // per item: n + nck
+  uint64_t* gc_off = nullptr;   // n + nck + 1
+  uint64_t* gc_tsum = nullptr;  // scan tiles
+  int32_t* gc_hp = nullptr;     // one entry per contact at most
+  uint64_t* gc_lbeg = nullptr;
+  int32_t* gc_llen = nullptr;
+  int4* gc_list[2] = {};        // candidate lists (ping-pong), n each
+  uint32_t* gc_ctr = nullptr;   // candidate counts of the two lists
   // batched BFS state, allocated on first use
   uint64_t *bq_front = nullptr, *bq_vis = nullptr, *bq_next = nullptr, *bq_mask = nullptr;
   int64_t* bq_bound = nullptr;
@@ -1927,6 +2188,68 @@ static void java_random_ints(int64_t seed, int32_t bound, int64_t count, int32_t
   }
 }
 
+// Graph coloring over higher-priority lists (k_gc_list, k_gc_round) into
+// g->i0, which holds kInf everywhere on entry.  Returns 1, having done
+// nothing, when the lists would exceed the budget: TGA_GC_LIST_MAX_BYTES from
+// the environment, else a quarter of the free device memory.
+static int gc_lists(tga_graph* g, const Window& W, const int32_t* prio, int32_t rounds) {
+  const int64_t n = g->n, m = n + g->nck;
+  const int64_t tiles = (m + 1 + kScanTile - 1) / kScanTile;
+  const uint64_t entries = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
+  const uint64_t need = entries * 4 + (uint64_t)m * 12 + 8 + (uint64_t)tiles * 8 + (uint64_t)n * (12 + 2 * 16) + 12;
+  uint64_t budget = UINT64_MAX;
+  const char* env = std::getenv("TGA_GC_LIST_MAX_BYTES");
+  if (env && *env) {
+    budget = std::strtoull(env, nullptr, 10);
+  } else if (!g->gc_hp) {
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    budget = fr / 4;
+  }
+  if (need > budget) return 1;
+  if (!g->gc_hp) {
+    int rc;
+    if ((rc = g->alloc(g->gc_cnt, (size_t)m)) || (rc = g->alloc(g->gc_off, (size_t)m + 1)) ||
+        (rc = g->alloc(g->gc_tsum, (size_t)tiles)) || (rc = g->alloc(g->gc_hp, (size_t)entries)) ||
+        (rc = g->alloc(g->gc_lbeg, (size_t)n)) || (rc = g->alloc(g->gc_llen, (size_t)n)) ||
+        (rc = g->alloc(g->gc_ctr, 2)))
+      return rc;
+    for (auto& l : g->gc_list)
+      if ((rc = g->alloc(l, (size_t)n))) return rc;
+  }
+  hipStream_t st = g->stream;
+  k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->gc_cnt, m, 0u);
+  LAUNCH_STATIC(k_gc_list, W, prio, g->gc_cnt, (const uint64_t*)nullptr, (int32_t*)nullptr);
+  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(g->gc_cnt, m, g->gc_tsum);
+  k_scan_top<<<1, kBlock, 0, st>>>(g->gc_tsum, tiles);
+  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(g->gc_cnt, m, g->gc_tsum, g->gc_off);
+  HIP_TRY(hipGetLastError());
+  LAUNCH_STATIC(k_gc_list, W, prio, (uint32_t*)nullptr, g->gc_off, g->gc_hp);
+  k_gc_ranges<<<g->grid(g->ae - g->ab), kBlock, 0, st>>>(g->ab, g->ae, (int32_t)n, g->vinfo, g->ck, g->gc_off,
+                                                        g->gc_hp, g->gc_lbeg, g->gc_llen, g->gc_list[0], g->gc_ctr);
+  HIP_TRY(hipGetLastError());
+  // Every 16 rounds the colored candidates are dropped and the count read
+  // back, to stop early and to size the grid.
+  int a = 0;
+  uint32_t blocks = (uint32_t)std::min(g->grid(g->ae - g->ab), 2048);
+  for (int32_t r = 0; r < rounds; ++r) {
+    k_gc_round<<<blocks, kBlock, 0, st>>>(g->gc_list[a], g->gc_ctr + a, g->gc_hp, g->gc_lbeg, g->gc_llen, g->i0, r);
+    HIP_TRY(hipGetLastError());
+    if ((r & 15) == 15 && r + 1 < rounds) {
+      k_fill<uint32_t><<<1, kBlock, 0, st>>>(g->gc_ctr + (a ^ 1), 1, 0u);
+      k_gc_compact<<<blocks, kBlock, 0, st>>>(g->gc_list[a], g->gc_ctr + a, g->gc_list[a ^ 1], g->gc_ctr + (a ^ 1));
+      HIP_TRY(hipGetLastError());
+      a ^= 1;
+      uint32_t h = 0;
+      int rc = g->read_counts(g->gc_ctr + a, &h, 1);
+      if (rc) return rc;
+      if (h == 0) break;
+      blocks = std::min<uint32_t>(2048, (h + kBlock - 1) / kBlock);
+    }
+  }
+  return TGA_OK;
+}
+
 extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, const int32_t* priority, int32_t* color,
                                   uint32_t flags) {
   if (!g || !color || bad_window(w) || rounds < 0) return fail(TGA_EINVAL, "tga_graph_coloring: bad arguments");
@@ -1947,6 +2270,15 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
     prio = g->prio;
   }
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kInf);
+  if (rounds > 0) {
+    const int rc = gc_lists(g, W, prio, rounds);
+    if (rc == TGA_OK) {
+      const int rc2 = g->out_copy(color, g->i0, (size_t)g->n * 4, flags);
+      return rc2 ? rc2 : g->check_err();
+    }
+    if (rc != 1) return rc;
+  }
+  // Decoding rounds (the lists do not fit).
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i2, g->n, 0);
   int32_t* blocker = g->q0;  // cached blocker per vertex (-1: none)
   k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(blocker, g->n, -1);

@@ -150,13 +150,22 @@ def test_connected_components(pair):
         np.testing.assert_array_equal(g.connected_components(w), ds.oracle().cc(window=w))
 
 
-def test_graph_coloring(pair):
+@pytest.mark.parametrize("path", ["lists", "decode"])
+def test_graph_coloring(pair, path, monkeypatch):
+    """Both round engines: higher-priority lists (the default) and decoding
+    rounds (used when the lists exceed TGA_GC_LIST_MAX_BYTES)."""
     ds, g = pair
     from compresstemporalgraph_amd import fmt
+    if path == "decode":
+        monkeypatch.setenv("TGA_GC_LIST_MAX_BYTES", "0")
     prio = fmt.java_random_ints(123, 256, ds.n)
+    o = ds.oracle()
     for w in windows_for(ds)[:2]:
-        got = g.graph_coloring(w)
-        np.testing.assert_array_equal(got, ds.oracle().gc(prio, 512, window=w))
+        np.testing.assert_array_equal(g.graph_coloring(w), o.gc(prio, 512, window=w))
+    # few rounds (vertices left uncolored) and custom priorities with ties
+    rng = np.random.default_rng(5)
+    p2 = rng.integers(0, 4, ds.n).astype(np.int32)
+    np.testing.assert_array_equal(g.graph_coloring(None, rounds=3, priority=p2), o.gc(p2, 3))
 
 
 def test_pagerank(pair):
