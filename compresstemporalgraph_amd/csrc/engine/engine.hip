@@ -36,6 +36,19 @@
 #include "../../../include/tga.h"
 #include "decode.hpp"
 
+namespace tga {
+// sort.hip: stable radix sort of (key, value) pairs by key bits [0, end_bit).
+hipError_t sort_pairs_u32_u64(void* tmp, size_t& tmp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                              const uint64_t* vals_in, uint64_t* vals_out, size_t n, int end_bit,
+                              hipStream_t stream);
+// sort.hip: per-row sums over pairs [begin, begin + count) (deterministic
+// reduce-by-key); mode 0 PageRank in-rows, 1 HITS authority in-rows, 2 HITS
+// hub out-rows.
+hipError_t reduce_rows(int mode, void* tmp, size_t& tmp_bytes, const uint32_t* sk, const uint64_t* val,
+                       const uint32_t* pk, const float* vec, int32_t n, size_t begin, size_t count, uint32_t* ukeys,
+                       double* sums, uint32_t* runs, hipStream_t stream);
+}  // namespace tga
+
 using namespace tga;
 
 namespace {
@@ -1153,6 +1166,116 @@ __global__ void __launch_bounds__(kBlock) k_hits_fix(const Ckpt* ck, int32_t nck
   }
 }
 
+// ------------------------------------------- PageRank / HITS over pairs ---
+//
+// Both iterate over the distinct (x, r) neighbor pairs with a window contact
+// between active vertices (PageRank weights a pair by its first window
+// contact t).  Decoded every iteration, as above, the sums are device-wide
+// FP64 atomics, and their throughput bounds the pass.  When memory allows,
+// the pairs are decoded once per call instead: a counting pass, a scan and a
+// filling pass (item order, as for the coloring lists; split runs go to one
+// extra slot per checkpoint after the items' pairs), then a radix sort by r.
+// Each iteration is then a pull with no atomics and a fixed summation order:
+// a row sum over r's in-pairs, or over x's out-pairs in fill order (the HITS
+// hub step), computed by a load-balanced reduce-by-key (rows run from one
+// pair to a hub's hundreds of thousands).  A pair is key r and value (x << 32 | float bits of
+// w = 1 - exp(-0.1 t)).
+__device__ __forceinline__ uint64_t pair_val(int32_t x, int32_t t) {
+  const float w = 1.0f - expf(-0.1f * (float)t);
+  return ((uint64_t)(uint32_t)x << 32) | __float_as_uint(w);
+}
+
+struct PairAct {
+  Window W;
+  uint32_t* cnt;        // counting pass: per item
+  const uint64_t* off;  // filling pass: per item
+  uint32_t* key;
+  uint64_t* val;
+  uint64_t* runkey;
+  int32_t n;
+  Split sp{-1, -1, -1, -1};
+  uint64_t pos = 0;
+  uint32_t c = 0;
+  int32_t idx = 0, x = 0, last = -1;
+  __device__ void item(int32_t code) { idx = item_index(code, n); }
+  __device__ bool begin(int32_t v, bool, Split s) {
+    if (!W.active(v)) return false;
+    x = v;
+    sp = s;
+    last = -1;
+    c = 0;
+    if (off) pos = off[idx];
+    return true;
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    if (off) {
+      if (defer(sp, runkey, r, t, k, W)) return true;
+    } else if (sp.slot_of(r) >= 0) {
+      return true;  // split run: one extra slot, filled by k_pair_fix
+    }
+    if (!W.in(t) || r == last) return true;
+    last = r;
+    if (!W.active(r)) return true;
+    if (off) {
+      key[pos] = (uint32_t)r;
+      val[pos] = pair_val(x, t);
+      ++pos;
+    } else {
+      ++c;
+    }
+    return true;
+  }
+  __device__ void end(int32_t, bool) {
+    if (!off) cnt[idx] = c;
+  }
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_pairs(Streams S, const Ckpt* ck, Items I, Window W, uint32_t* cnt,
+                                                  const uint64_t* off, uint32_t* key, uint64_t* val,
+                                                  uint64_t* runkey) {
+  PairAct a{W, cnt, off, key, val, runkey, S.n};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Split-run pairs into key/val[0, nck) (one slot per checkpoint).  An empty
+// slot gets r = n (sorted after every vertex, never summed) and w = 0, with
+// its chunk's vertex as x: the slots of one vertex stay one run of equal x
+// (k_scatter_rows adds each run's sum without atomics).
+__global__ void __launch_bounds__(kBlock) k_pair_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
+                                                     int32_t n, uint32_t* key, uint64_t* val) {
+  for (int32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nck; s += gridDim.x * blockDim.x) {
+    SplitRun q;
+    if (split_run(ck, runkey, s, q) && W.active(q.r) && W.active(q.x)) {
+      key[s] = (uint32_t)q.r;
+      val[s] = pair_val(q.x, q.t);
+    } else {
+      key[s] = (uint32_t)n;
+      val[s] = (uint64_t)(uint32_t)ck[s].x << 32;
+    }
+  }
+}
+
+// PageRank coefficient of every vertex: d * rank[x] / du[x] (0 without
+// window contacts or when inactive).
+__global__ void __launch_bounds__(kBlock) k_pr_coef(Window W, int32_t n, const float* rank, const int32_t* du,
+                                                    float damping, float* coef) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    coef[v] = (W.active((int32_t)v) && du[v] > 0) ? pr_coef(damping, rank, du, (int32_t)v) : 0.0f;
+}
+
+// acc[k] = base + sums[i] (or acc[k] += sums[i] when add) for every row run
+// i with key k < n.  Keys must be distinct among the runs of one call.
+__global__ void __launch_bounds__(kBlock) k_scatter_rows(const uint32_t* ukeys, const double* sums,
+                                                         const uint32_t* runs, int32_t n, double base, int add,
+                                                         double* acc) {
+  const uint32_t cnt = *runs;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t k = ukeys[i];
+    if (k < (uint32_t)n) acc[k] = add ? acc[k] + sums[i] : base + sums[i];
+  }
+}
+
 // ------------------------------------------------------ elementwise, sums ---
 
 template <class T>
@@ -1330,15 +1453,24 @@ struct tga_graph {
   double* scal = nullptr;     // 4 scalars
   float *f0 = nullptr, *f1 = nullptr;
   int32_t* prio = nullptr;  // default GC priorities, built on first use
+  // per-item list offsets (graph coloring, PageRank, HITS), on first use
+  uint32_t* lcnt = nullptr;   // per item: n + nck
+  uint64_t* loff = nullptr;   // n + nck + 1
+  uint64_t* ltsum = nullptr;  // scan tiles
   // graph coloring lists, allocated on first use (k_gc_round)
-  uint32_t* gc_cnt = nullptr;   // per item: n + nck
-  uint64_t* gc_off = nullptr;   // n + nck + 1
-  uint64_t* gc_tsum = nullptr;  // scan tiles
   int32_t* gc_hp = nullptr;     // one entry per contact at most
   uint64_t* gc_lbeg = nullptr;
   int32_t* gc_llen = nullptr;
   int4* gc_list[2] = {};        // candidate lists (ping-pong), n each
   uint32_t* gc_ctr = nullptr;   // candidate counts of the two lists
+  // neighbor pairs of PageRank / HITS, allocated on first use (k_pairs)
+  uint32_t *pk = nullptr, *sk = nullptr;  // r: fill order, sorted by r
+  uint64_t *pv = nullptr, *sv = nullptr;  // (x, w): fill order, sorted by r
+  uint32_t* ukeys = nullptr;  // n + 2: row keys of a reduce_rows call
+  double* usums = nullptr;    // n + 2: their sums
+  uint32_t* nruns = nullptr;
+  void* sort_tmp = nullptr;  // radix sort and reduce_rows temporary storage
+  size_t sort_tmp_bytes = 0;
   // batched BFS state, allocated on first use
   uint64_t *bq_front = nullptr, *bq_vis = nullptr, *bq_next = nullptr, *bq_mask = nullptr;
   int64_t* bq_bound = nullptr;
@@ -2188,44 +2320,76 @@ static void java_random_ints(int64_t seed, int32_t bound, int64_t count, int32_t
   }
 }
 
-// Graph coloring over higher-priority lists (k_gc_list, k_gc_round) into
-// g->i0, which holds kInf everywhere on entry.  Returns 1, having done
-// nothing, when the lists would exceed the budget: TGA_GC_LIST_MAX_BYTES from
-// the environment, else a quarter of the free device memory.
-static int gc_lists(tga_graph* g, const Window& W, const int32_t* prio, int32_t rounds) {
-  const int64_t n = g->n, m = n + g->nck;
-  const int64_t tiles = (m + 1 + kScanTile - 1) / kScanTile;
-  const uint64_t entries = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
-  const uint64_t need = entries * 4 + (uint64_t)m * 12 + 8 + (uint64_t)tiles * 8 + (uint64_t)n * (12 + 2 * 16) + 12;
+// Whether list state of `need` device bytes may be built: at most
+// TGA_LIST_MAX_BYTES from the environment, else a quarter of the free device
+// memory (always, when `held` says it is allocated already).
+static int list_budget(uint64_t need, bool held, bool* ok) {
+  const char* env = std::getenv("TGA_LIST_MAX_BYTES");
   uint64_t budget = UINT64_MAX;
-  const char* env = std::getenv("TGA_GC_LIST_MAX_BYTES");
   if (env && *env) {
     budget = std::strtoull(env, nullptr, 10);
-  } else if (!g->gc_hp) {
+  } else if (!held) {
     size_t fr = 0, tot = 0;
     HIP_TRY(hipMemGetInfo(&fr, &tot));
     budget = fr / 4;
   }
-  if (need > budget) return 1;
+  *ok = need <= budget;
+  return TGA_OK;
+}
+
+static int64_t scan_tiles(int64_t m) { return (m + 1 + kScanTile - 1) / kScanTile; }
+
+// Exclusive scan of g->lcnt[0, n + nck) into g->loff (allocated on first use).
+static int item_scan(tga_graph* g) {
+  const int64_t m = g->n + g->nck, tiles = scan_tiles(m);
+  hipStream_t st = g->stream;
+  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(g->lcnt, m, g->ltsum);
+  k_scan_top<<<1, kBlock, 0, st>>>(g->ltsum, tiles);
+  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(g->lcnt, m, g->ltsum, g->loff);
+  HIP_TRY(hipGetLastError());
+  return TGA_OK;
+}
+
+static int item_scan_alloc(tga_graph* g) {
+  if (g->lcnt) return TGA_OK;
+  const int64_t m = g->n + g->nck;
+  int rc;
+  if ((rc = g->alloc(g->lcnt, (size_t)m)) || (rc = g->alloc(g->loff, (size_t)m + 1)) ||
+      (rc = g->alloc(g->ltsum, (size_t)scan_tiles(m))))
+    return rc;
+  return TGA_OK;
+}
+
+static uint64_t item_scan_bytes(const tga_graph* g) {
+  const int64_t m = g->n + g->nck;
+  return g->lcnt ? 0 : (uint64_t)m * 12 + 8 + (uint64_t)scan_tiles(m) * 8;
+}
+
+// Graph coloring over higher-priority lists (k_gc_list, k_gc_round) into
+// g->i0, which holds kInf everywhere on entry.  Returns 1, having done
+// nothing, when the lists do not fit list_budget.
+static int gc_lists(tga_graph* g, const Window& W, const int32_t* prio, int32_t rounds) {
+  const int64_t n = g->n;
+  const uint64_t entries = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
+  const uint64_t need = entries * 4 + (uint64_t)n * (12 + 2 * 16) + 8 + item_scan_bytes(g);
+  bool ok = false;
+  int rc = list_budget(need, g->gc_hp != nullptr, &ok);
+  if (rc) return rc;
+  if (!ok) return 1;
+  if ((rc = item_scan_alloc(g))) return rc;
   if (!g->gc_hp) {
-    int rc;
-    if ((rc = g->alloc(g->gc_cnt, (size_t)m)) || (rc = g->alloc(g->gc_off, (size_t)m + 1)) ||
-        (rc = g->alloc(g->gc_tsum, (size_t)tiles)) || (rc = g->alloc(g->gc_hp, (size_t)entries)) ||
-        (rc = g->alloc(g->gc_lbeg, (size_t)n)) || (rc = g->alloc(g->gc_llen, (size_t)n)) ||
-        (rc = g->alloc(g->gc_ctr, 2)))
+    if ((rc = g->alloc(g->gc_hp, (size_t)entries)) || (rc = g->alloc(g->gc_lbeg, (size_t)n)) ||
+        (rc = g->alloc(g->gc_llen, (size_t)n)) || (rc = g->alloc(g->gc_ctr, 2)))
       return rc;
     for (auto& l : g->gc_list)
       if ((rc = g->alloc(l, (size_t)n))) return rc;
   }
   hipStream_t st = g->stream;
-  k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->gc_cnt, m, 0u);
-  LAUNCH_STATIC(k_gc_list, W, prio, g->gc_cnt, (const uint64_t*)nullptr, (int32_t*)nullptr);
-  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(g->gc_cnt, m, g->gc_tsum);
-  k_scan_top<<<1, kBlock, 0, st>>>(g->gc_tsum, tiles);
-  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(g->gc_cnt, m, g->gc_tsum, g->gc_off);
-  HIP_TRY(hipGetLastError());
-  LAUNCH_STATIC(k_gc_list, W, prio, (uint32_t*)nullptr, g->gc_off, g->gc_hp);
-  k_gc_ranges<<<g->grid(g->ae - g->ab), kBlock, 0, st>>>(g->ab, g->ae, (int32_t)n, g->vinfo, g->ck, g->gc_off,
+  k_fill<uint32_t><<<g->grid(n + g->nck), kBlock, 0, st>>>(g->lcnt, n + g->nck, 0u);
+  LAUNCH_STATIC(k_gc_list, W, prio, g->lcnt, (const uint64_t*)nullptr, (int32_t*)nullptr);
+  if ((rc = item_scan(g))) return rc;
+  LAUNCH_STATIC(k_gc_list, W, prio, (uint32_t*)nullptr, g->loff, g->gc_hp);
+  k_gc_ranges<<<g->grid(g->ae - g->ab), kBlock, 0, st>>>(g->ab, g->ae, (int32_t)n, g->vinfo, g->ck, g->loff,
                                                         g->gc_hp, g->gc_lbeg, g->gc_llen, g->gc_list[0], g->gc_ctr);
   HIP_TRY(hipGetLastError());
   // Every 16 rounds the colored candidates are dropped and the count read
@@ -2241,8 +2405,7 @@ static int gc_lists(tga_graph* g, const Window& W, const int32_t* prio, int32_t 
       HIP_TRY(hipGetLastError());
       a ^= 1;
       uint32_t h = 0;
-      int rc = g->read_counts(g->gc_ctr + a, &h, 1);
-      if (rc) return rc;
+      if ((rc = g->read_counts(g->gc_ctr + a, &h, 1))) return rc;
       if (h == 0) break;
       blocks = std::min<uint32_t>(2048, (h + kBlock - 1) / kBlock);
     }
@@ -2315,6 +2478,73 @@ static int reduce(tga_graph* g, int mode, const double* x, const float* old, con
   return TGA_OK;
 }
 
+// Decodes the neighbor pairs of window W (k_pairs) and sorts them by r.
+// Returns 1, having done nothing, when they do not fit list_budget.  *extra
+// receives the position of the split-run slots, after the items' pairs, and
+// *count the number of pairs and slots.
+static int build_pairs(tga_graph* g, const Window& W, uint64_t* extra, uint64_t* count_out) {
+  const int64_t n = g->n, nck = g->nck, m = n + nck;
+  const uint64_t cap = (uint64_t)std::max<int64_t>(g->info.contacts, 0) + (uint64_t)nck;
+  const int end_bit = 32 - __builtin_clz((uint32_t)n | 1u);  // keys <= n
+  size_t tmp_bytes = 0;
+  HIP_TRY(sort_pairs_u32_u64(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, (size_t)cap, end_bit,
+                             g->stream));
+  for (int mode = 0; mode < 3; ++mode) {
+    size_t b = 0;
+    HIP_TRY(reduce_rows(mode, nullptr, b, nullptr, nullptr, nullptr, nullptr, (int32_t)n, 0, (size_t)cap, nullptr,
+                        nullptr, nullptr, g->stream));
+    tmp_bytes = std::max(tmp_bytes, b);
+  }
+  const uint64_t need = cap * 24 + tmp_bytes + (uint64_t)(n + 2) * 12 + 4 + item_scan_bytes(g);
+  bool ok = false;
+  int rc = list_budget(need, g->pk != nullptr, &ok);
+  if (rc) return rc;
+  if (!ok) return 1;
+  if ((rc = item_scan_alloc(g))) return rc;
+  if (!g->pk) {
+    uint8_t* tmp = nullptr;
+    if ((rc = g->alloc(g->pk, (size_t)cap)) || (rc = g->alloc(g->pv, (size_t)cap)) ||
+        (rc = g->alloc(g->sk, (size_t)cap)) || (rc = g->alloc(g->sv, (size_t)cap)) ||
+        (rc = g->alloc(g->ukeys, (size_t)n + 2)) || (rc = g->alloc(g->usums, (size_t)n + 2)) ||
+        (rc = g->alloc(g->nruns, 1)) || (rc = g->alloc(tmp, tmp_bytes)))
+      return rc;
+    g->sort_tmp = tmp;
+    g->sort_tmp_bytes = tmp_bytes;
+  }
+  hipStream_t st = g->stream;
+  k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->lcnt, m, 0u);
+  LAUNCH_STATIC(k_pairs, W, g->lcnt, (const uint64_t*)nullptr, (uint32_t*)nullptr, (uint64_t*)nullptr, g->runkey);
+  if ((rc = item_scan(g))) return rc;
+  uint64_t T = 0;
+  HIP_TRY(hipMemcpyAsync(&T, g->loff + m, sizeof T, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (T + (uint64_t)nck > cap) return fail(TGA_EFORMAT, "neighbor pairs exceed the contact count");
+  LAUNCH_STATIC(k_pairs, W, (uint32_t*)nullptr, g->loff, g->pk, g->pv, g->runkey);
+  if (nck > 0) {
+    k_pair_fix<<<g->grid(nck), kBlock, 0, st>>>(g->ck, (int32_t)nck, g->runkey, W, (int32_t)n, g->pk + T, g->pv + T);
+    HIP_TRY(hipGetLastError());
+  }
+  const uint64_t count = T + (uint64_t)nck;
+  size_t tb = g->sort_tmp_bytes;
+  HIP_TRY(sort_pairs_u32_u64(g->sort_tmp, tb, g->pk, g->sk, g->pv, g->sv, (size_t)count, end_bit, st));
+  *extra = T;
+  *count_out = count;
+  return TGA_OK;
+}
+
+// acc[k] = base + (row sum of k) for the rows of pairs [begin, begin + count)
+// (add: acc[k] += row sum); see reduce_rows for the modes.
+static int pull_rows(tga_graph* g, int mode, const float* vec, uint64_t begin, uint64_t count, double base, bool add) {
+  if (count == 0) return TGA_OK;
+  size_t tb = g->sort_tmp_bytes;
+  HIP_TRY(reduce_rows(mode, g->sort_tmp, tb, g->sk, mode == 2 ? g->pv : g->sv, g->pk, vec, (int32_t)g->n,
+                      (size_t)begin, (size_t)count, g->ukeys, g->usums, g->nruns, g->stream));
+  k_scatter_rows<<<g->grid(g->n + 2), kBlock, 0, g->stream>>>(g->ukeys, g->usums, g->nruns, (int32_t)g->n, base,
+                                                              add ? 1 : 0, g->acc);
+  HIP_TRY(hipGetLastError());
+  return TGA_OK;
+}
+
 extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float damping, float tol, float* rank,
                             int32_t* iters_run, uint32_t flags) {
   if (!g || !rank || bad_window(w) || iters < 0) return fail(TGA_EINVAL, "tga_pagerank: bad arguments");
@@ -2333,13 +2563,24 @@ extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float dam
   const float inv = 1.0f / (float)N;
   const float base = (1.0f - damping) / (float)N;
   k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, inv, 0.0f);
+  uint64_t extra = 0, npairs = 0;
+  const int lp = iters > 0 ? build_pairs(g, W, &extra, &npairs) : 1;
+  if (lp < 0) return lp;
   int32_t it = 0;
   for (; it < iters; ++it) {
-    k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
-    LAUNCH_STATIC(k_pr_push, W, g->f0, du, damping, g->acc, g->runkey);
-    if (g->nck > 0)
-      k_pr_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, du, damping,
-                                                          g->acc);
+    if (lp == 0) {
+      // row sums over the sorted pairs (f1 holds the coefficients)
+      k_pr_coef<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, du, damping, g->f1);
+      k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
+      int rc = pull_rows(g, 0, g->f1, 0, npairs, (double)base, false);
+      if (rc) return rc;
+    } else {
+      k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
+      LAUNCH_STATIC(k_pr_push, W, g->f0, du, damping, g->acc, g->runkey);
+      if (g->nck > 0)
+        k_pr_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, du,
+                                                            damping, g->acc);
+    }
     int rc = reduce(g, 0, g->acc, nullptr, nullptr, g->scal, false);
     if (!rc && tol > 0) rc = reduce(g, 2, g->acc, g->f0, g->scal, g->scal + 1, false);
     if (rc) return rc;
@@ -2367,18 +2608,34 @@ extern "C" int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, f
   const Window W = g->window(w);
   k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, 1.0f, 0.0f);  // hub
   k_fill_active<float><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f1, 1.0f, 0.0f);  // auth
+  uint64_t extra = 0, npairs = 0;
+  const int lp = iters > 0 ? build_pairs(g, W, &extra, &npairs) : 1;
+  if (lp < 0) return lp;
+  const int32_t n = (int32_t)g->n;
   for (int32_t it = 0; it < iters; ++it) {
-    k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
-    LAUNCH_STATIC(k_hits_pull, W, g->f1, g->acc, g->runkey);
-    if (g->nck > 0)
-      k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f1, g->acc, 0);
+    k_fill<double><<<g->grid(n), kBlock, 0, g->stream>>>(g->acc, n, 0.0);
+    if (lp == 0) {
+      // hub: out-rows in fill order, then the split-run slots
+      int rc = pull_rows(g, 2, g->f1, 0, extra, 0.0, false);
+      if (!rc) rc = pull_rows(g, 2, g->f1, extra, npairs - extra, 0.0, true);
+      if (rc) return rc;
+    } else {
+      LAUNCH_STATIC(k_hits_pull, W, g->f1, g->acc, g->runkey);
+      if (g->nck > 0)
+        k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f1, g->acc, 0);
+    }
     int rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
-    k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f0);
-    k_fill<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->n, 0.0);
-    LAUNCH_STATIC(k_hits_push, W, g->f0, g->acc, g->runkey);
-    if (g->nck > 0)
-      k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, g->acc, 1);
+    k_scale_store<<<g->grid(n), kBlock, 0, g->stream>>>(g->acc, g->scal, n, g->f0);
+    k_fill<double><<<g->grid(n), kBlock, 0, g->stream>>>(g->acc, n, 0.0);
+    if (lp == 0) {
+      rc = pull_rows(g, 1, g->f0, 0, npairs, 0.0, false);  // authority: sorted in-rows
+      if (rc) return rc;
+    } else {
+      LAUNCH_STATIC(k_hits_push, W, g->f0, g->acc, g->runkey);
+      if (g->nck > 0)
+        k_hits_fix<<<g->grid(g->nck), kBlock, 0, g->stream>>>(g->ck, (int32_t)g->nck, g->runkey, W, g->f0, g->acc, 1);
+    }
     rc = reduce(g, 1, g->acc, nullptr, nullptr, g->scal, true);
     if (rc) return rc;
     k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f1);
