@@ -374,22 +374,34 @@ __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items 
 }
 
 // Degree count (and, for whole records, the centrality formula).
+// Contact time of a contact that does not count (k_dc_rec's buffer).
+constexpr int64_t kNoTime = INT64_MIN;
+
+// Degree count and Java running mean of one record.  With tbuf, the chunks
+// of long records also store every contact's time at tbuf[coff[x] + k]
+// (kNoTime when it does not count) for k_dc_rec.
 struct DcAct {
   Window W;
   int32_t* count;
   int32_t* centr;
+  int64_t* tbuf = nullptr;
+  const uint64_t* coff = nullptr;
+  int64_t* tb = nullptr;
   int32_t deg = 0;
   int64_t avg = 0;
-  __device__ bool begin(int32_t x, bool, Split) {
+  __device__ bool begin(int32_t x, bool chunked, Split) {
     deg = 0;
     avg = 0;
+    tb = (tbuf && chunked) ? tbuf + coff[x] : nullptr;
     return W.active(x);
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
-    if (W.in(t) && W.active(r)) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    const bool counts = W.in(t) && W.active(r);
+    if (counts) {
       ++deg;
       avg += ((int64_t)t - avg) / deg;  // truncating, as Java's long division
     }
+    if (tb) tb[k] = counts ? (int64_t)t : kNoTime;
     return true;
   }
   __device__ void end(int32_t x, bool chunked);
@@ -419,9 +431,130 @@ __device__ void DcAct::end(int32_t x, bool chunked) {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_dc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* count,
-                                               int32_t* centr) {
-  DcAct a{W, count, centr};
+                                               int32_t* centr, int64_t* tbuf, const uint64_t* coff) {
+  DcAct a{W, count, centr, tbuf, coff};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Java's long division a / k (truncating toward zero) for k >= 1 and
+// |a| < 2^40, given rk = 1.0 / k in fp64: a * rk is within 2^-12 of a / k,
+// so its truncation is within one of the quotient; one correction step.
+__device__ __forceinline__ int64_t div_trunc(int64_t a, int64_t k, double rk) {
+  int64_t q = (int64_t)((double)a * rk);
+  const int64_t r = a - q * k;
+  if (a >= 0) {
+    if (r >= k) ++q;
+    else if (r < 0) --q;
+  } else {
+    if (r <= -k) --q;
+    else if (r > 0) ++q;
+  }
+  return q;
+}
+
+// Lane moves of a 64-bit value: DPP (row shifts and the GFX9 row broadcasts,
+// no LDS round trip) and readlane with a wave-uniform lane.
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp64(int64_t v) {
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)((uint64_t)v >> 32), CTRL, 0xf, 0xf, false);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Inclusive prefix sum over the wave: row_shr 1, 2, 4, 8 within rows of 16
+// lanes, then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2
+// and 3.  Every lane must call it.
+__device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
+  const int lane = threadIdx.x & 63, rl = lane & 15;
+  int64_t t = dpp64<0x111>(v);
+  if (rl >= 1) v += t;
+  t = dpp64<0x112>(v);
+  if (rl >= 2) v += t;
+  t = dpp64<0x114>(v);
+  if (rl >= 4) v += t;
+  t = dpp64<0x118>(v);
+  if (rl >= 8) v += t;
+  t = dpp64<0x142>(v);
+  if ((lane & 31) >= 16) v += t;
+  t = dpp64<0x143>(v);
+  if (lane >= 32) v += t;
+  return v;
+}
+
+// Running mean of each long record over its stored contact times, one wave
+// per record, 64 contacts a step.  The mean is sequential (avg_k = avg_k-1 +
+// (t_k - avg_k-1) / k, truncating), so each step is speculative: every lane
+// divides with the step's starting mean, a prefix sum of those quotients
+// gives each lane its starting mean, and the lanes recompute; from the first
+// lane whose quotient changed (the lanes before it are exact, and so is its
+// new quotient) the step restarts.  Each round fixes at least one lane.
+__global__ void __launch_bounds__(kBlock) k_dc_rec(const int32_t* bigv, int32_t nbig, const int32_t* vdeg,
+                                                   const uint64_t* coff, const int64_t* tbuf, Window W,
+                                                   int32_t* centr) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (wid >= nbig) return;
+  const int32_t x = bigv[wid];
+  if (!W.active(x)) {
+    if (lane == 0) centr[x] = dc_formula(0, 0, W.ws, W.we);
+    return;
+  }
+  const int64_t* tb = tbuf + coff[x];
+  const int32_t len = vdeg[x];
+  int64_t avg = 0;
+  int32_t cnt = 0;
+  // Times are loaded kAhead steps of 64 at a time, the next group while the
+  // current one is processed.
+  constexpr int kAhead = 4;
+  int64_t cur[kAhead], nxt[kAhead];
+#pragma unroll
+  for (int j = 0; j < kAhead; ++j) {
+    const int32_t i = j * 64 + lane;
+    cur[j] = i < len ? tb[i] : kNoTime;
+  }
+  for (int32_t g0 = 0; g0 < len; g0 += kAhead * 64) {
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) {
+      const int32_t i = g0 + (kAhead + j) * 64 + lane;
+      nxt[j] = i < len ? tb[i] : kNoTime;
+    }
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) {
+      const int64_t t = cur[j];
+      const bool valid = t != kNoTime;
+      const uint64_t vm = __ballot(valid);
+      if (!vm) continue;
+      const int64_t k = cnt + __popcll(vm & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
+      const double rk = 1.0 / (double)k;  // off the sequential chain
+      int first = 0;
+      int64_t base = avg;
+      int64_t q = valid ? div_trunc(t - base, k, rk) : 0;
+      while (true) {
+        const int64_t inc = wave_incl_scan64(q);
+        const int64_t before = base + inc - q;
+        const int64_t q2 = (valid && lane >= first) ? div_trunc(t - before, k, rk) : 0;
+        const uint64_t mm = __ballot(q2 != q);
+        if (!mm) {
+          avg = base + readlane64(inc, 63);
+          break;
+        }
+        const int f = __ffsll((unsigned long long)mm) - 1;
+        base = readlane64(before + q2, f);
+        first = f + 1;
+        q = (valid && lane >= first) ? div_trunc(t - base, k, rk) : 0;
+      }
+      cnt += __popcll(vm);
+    }
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) cur[j] = nxt[j];
+  }
+  if (lane == 0) centr[x] = dc_formula(cnt, avg, W.ws, W.we);
 }
 
 // The running mean of long records is sequential: one thread per record.
@@ -1466,6 +1599,8 @@ struct tga_graph {
   // neighbor pairs of PageRank / HITS, allocated on first use (k_pairs)
   uint32_t *pk = nullptr, *sk = nullptr;  // r: fill order, sorted by r
   uint64_t *pv = nullptr, *sv = nullptr;  // (x, w): fill order, sorted by r
+  int64_t* dc_tbuf = nullptr;   // contact times of long records (k_dc_rec), on first use
+  uint64_t* dc_coff = nullptr;  // n + 1: first contact of every record
   uint32_t* ukeys = nullptr;  // n + 2: row keys of a reduce_rows call
   double* usums = nullptr;    // n + 2: their sums
   uint32_t* nruns = nullptr;
@@ -1991,6 +2126,12 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   return TGA_OK;
 }
 
+// List state (defined with the graph coloring lists below).
+static int list_budget(uint64_t need, bool held, bool* ok);
+static uint64_t item_scan_bytes(const tga_graph* g);
+static int item_scan_alloc(tga_graph* g);
+static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off);
+
 extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* centrality, uint32_t flags) {
   if (!g || !count || bad_window(w)) return fail(TGA_EINVAL, "tga_degree_centrality: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
@@ -2002,10 +2143,29 @@ extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count,
     const int32_t c0 = (w.start == w.end) ? 0 : 1;
     k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(centr, g->n, c0);
   }
-  LAUNCH_STATIC(k_dc, W, g->i0, centr);
+  // Long records: stored contact times and one wave per record (k_dc_rec)
+  // when the buffer fits list_budget, else one thread decoding each record.
+  bool rec = false;
+  if (centr) {
+    const uint64_t contacts = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
+    int rc = list_budget(contacts * 8 + (uint64_t)(g->n + 1) * 8 + item_scan_bytes(g), g->dc_tbuf != nullptr, &rec);
+    if (rc) return rc;
+    if (rec && !g->dc_tbuf) {
+      if ((rc = item_scan_alloc(g)) || (rc = g->alloc(g->dc_tbuf, (size_t)contacts)) ||
+          (rc = g->alloc(g->dc_coff, (size_t)g->n + 1)))
+        return rc;
+      if ((rc = scan_counts(g, (const uint32_t*)g->vdeg, g->n, g->dc_coff))) return rc;
+    }
+  }
+  LAUNCH_STATIC(k_dc, W, g->i0, centr, rec ? g->dc_tbuf : nullptr, (const uint64_t*)g->dc_coff);
   if (centr) {
     for (int c = 0; c < kClasses; ++c) {
       if (!g->n_big[c]) continue;
+      if (rec) {
+        k_dc_rec<<<(g->n_big[c] + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, g->stream>>>(
+            g->big[c], g->n_big[c], g->vdeg, g->dc_coff, g->dc_tbuf, W, centr);
+        continue;
+      }
       const int blocks = (g->n_big[c] + 63) / 64;
       if (c == 0) k_dc_long<0><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
       if (c == 1) k_dc_long<1><<<blocks, 64, 0, g->stream>>>(g->S, g->big[c], g->n_big[c], W, centr);
@@ -2339,16 +2499,20 @@ static int list_budget(uint64_t need, bool held, bool* ok) {
 
 static int64_t scan_tiles(int64_t m) { return (m + 1 + kScanTile - 1) / kScanTile; }
 
-// Exclusive scan of g->lcnt[0, n + nck) into g->loff (allocated on first use).
-static int item_scan(tga_graph* g) {
-  const int64_t m = g->n + g->nck, tiles = scan_tiles(m);
+// Exclusive scan of c[0, m) into off[0, m] (m <= n + nck; the tile sums
+// use g->ltsum, see item_scan_alloc).
+static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off) {
+  const int64_t tiles = scan_tiles(m);
   hipStream_t st = g->stream;
-  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(g->lcnt, m, g->ltsum);
+  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(c, m, g->ltsum);
   k_scan_top<<<1, kBlock, 0, st>>>(g->ltsum, tiles);
-  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(g->lcnt, m, g->ltsum, g->loff);
+  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(c, m, g->ltsum, off);
   HIP_TRY(hipGetLastError());
   return TGA_OK;
 }
+
+// Exclusive scan of the per-item counts g->lcnt into g->loff.
+static int item_scan(tga_graph* g) { return scan_counts(g, g->lcnt, g->n + g->nck, g->loff); }
 
 static int item_scan_alloc(tga_graph* g) {
   if (g->lcnt) return TGA_OK;

@@ -72,8 +72,12 @@ def test_neighbor_access_list(pair):
     assert g.neighbor_access(verts) == ds.oracle().neighbor_access(verts)
 
 
-def test_degree_centrality(pair):
+@PATHS
+def test_degree_centrality(pair, path, monkeypatch):
+    """Long records' running means: stored contact times and one wave per
+    record ('lists'), or one thread decoding each record ('decode')."""
     ds, g = pair
+    use_path(path, monkeypatch)
     for w in windows_for(ds):
         cnt, cen = g.degree_centrality(w, centrality=True)
         ocnt, ocen = ds.oracle().degree(window=w, centrality=True)
