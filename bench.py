@@ -221,6 +221,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the seven-workload table and the batched / served query legs")
+    ap.add_argument("--no-query-legs", action="store_true",
+                    help="time the seven workloads but skip the batched / served BFS legs")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -303,7 +305,7 @@ def main() -> int:
     g.close()
 
     batched = served = None
-    if not args.no_workloads:
+    if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
         # process, and whose streams share the hardware queues), so they run
