@@ -25,6 +25,12 @@
  *    vertices are traversed and only active neighbors are followed.
  *  - A handle owns one HIP stream; calls on it are synchronous and a handle
  *    must not be used from two threads at once.
+ *  - Degree centrality, graph coloring, PageRank and HITS build per-call list
+ *    state (up to 24 bytes per contact for PageRank/HITS) in device buffers
+ *    the handle keeps, when it fits TGA_LIST_MAX_BYTES (environment, bytes;
+ *    default: a quarter of the free device memory at first use); otherwise
+ *    they decode the records every round or iteration.  Results are the same
+ *    either way (PageRank/HITS: FP64 sums in another order).
  */
 #ifndef TGA_H
 #define TGA_H
