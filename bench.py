@@ -22,6 +22,8 @@ Prints one JSON line with the driver's contract fields plus:
                 covering 100%, 50% and 10% of the timestamp span.
   batched_bfs   64 windowed BFS queries in one batched call against 64 plain
                 calls (results compared, outputs in HBM).
+  batched_dc    64 windowed degree-centrality queries in one batched call
+                against 64 plain calls (results compared, outputs in HBM).
   served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
                 against sequential plain BFS, queries/s and p50/p99 latency.
 """
@@ -141,6 +143,40 @@ def batched_bfs(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
             "mix": "BFS, random sources, windows log-uniform in [2^10, 2^24] with uniform starts"}
 
 
+def batched_dc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 degree-centrality queries (counts and the Java centrality) over
+    windows log-uniform in [2^10, 2^24] with uniform starts (the batching
+    mix's DC share): one tga_degree_centrality_batch call against nq plain
+    calls, outputs on the device in both cases, every row compared."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(7)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    wins = [(int(a), int(a + b)) for a, b in zip(starts, lens)]
+    cnt, cen = (torch.empty((nq, n), dtype=torch.int32, device="cuda:0") for _ in range(2))
+    pcnt, pcen = (torch.empty((nq, n), dtype=torch.int32, device="cuda:0") for _ in range(2))
+    g.degree_centrality_batch(wins, out=cnt, out_centrality=cen)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.degree_centrality_batch(wins, out=cnt, out_centrality=cen)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    g.degree_centrality(wins[0], out=pcnt[0], out_centrality=pcen[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(nq):
+        g.degree_centrality(wins[q], out=pcnt[q], out_centrality=pcen[q])
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    same = bool(torch.equal(cnt, pcnt) and torch.equal(cen, pcen))
+    return {"queries": nq, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+            "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1), "speedup": round(tp / tb, 2),
+            "identical": same,
+            "mix": "DC with centrality, windows log-uniform in [2^10, 2^24] with uniform starts"}
+
+
 def served_bfs(g, t_lo: int, t_hi: int) -> dict:
     """Serving: BFS window queries at Poisson rates of 1k and 10k per second
     (log-uniform window lengths in [2^10, 2^24], random sources), continuous
@@ -177,10 +213,14 @@ def query_legs(args) -> int:
     log(f"[bench] batched BFS: {batched}")
     if not batched["identical"]:
         raise SystemExit("batched BFS results differ from the plain results")
+    bdc = batched_dc(g, t_lo, t_hi)
+    log(f"[bench] batched DC: {bdc}")
+    if not bdc["identical"]:
+        raise SystemExit("batched DC results differ from the plain results")
     served = served_bfs(g, t_lo, t_hi)
     log(f"[bench] served BFS: {served}")
     g.close()
-    print(json.dumps({"batched_bfs": batched, "served_bfs": served}), flush=True)
+    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "served_bfs": served}), flush=True)
     return 0
 
 
@@ -304,7 +344,7 @@ def main() -> int:
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
-    batched = served = None
+    batched = bdc = served = None
     if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -318,7 +358,7 @@ def main() -> int:
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
-        batched, served = legs["batched_bfs"], legs["served_bfs"]
+        batched, bdc, served = legs["batched_bfs"], legs["batched_dc"], legs["served_bfs"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -347,6 +387,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
         "batched_bfs": batched,
+        "batched_dc": bdc,
         "served_bfs": served,
     }
     print(json.dumps(line), flush=True)

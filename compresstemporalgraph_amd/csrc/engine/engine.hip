@@ -373,35 +373,53 @@ __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items 
   }
 }
 
-// Degree count (and, for whole records, the centrality formula).
-// Contact time of a contact that does not count (k_dc_rec's buffer).
+// Contact-time cache (window-independent, built once per handle by
+// k_ctimes): the time of contact k of active x at ct[coff[x] + k], or kNoTime
+// when its neighbor is inactive.  The degree centrality of long records and
+// the batched degree centrality read it instead of decoding.
 constexpr int64_t kNoTime = INT64_MIN;
 
-// Degree count and Java running mean of one record.  With tbuf, the chunks
-// of long records also store every contact's time at tbuf[coff[x] + k]
-// (kNoTime when it does not count) for k_dc_rec.
+struct CtAct {
+  Window W;  // active range
+  int64_t* ct;
+  const uint64_t* coff;
+  int64_t* tb = nullptr;
+  __device__ bool begin(int32_t x, bool, Split) {
+    tb = ct + coff[x];
+    return W.active(x);
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
+    tb[k] = W.active(r) ? (int64_t)t : kNoTime;
+    return true;
+  }
+  __device__ void end(int32_t, bool) {}
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_ctimes(Streams S, const Ckpt* ck, Items I, Window W, int64_t* ct,
+                                                   const uint64_t* coff) {
+  CtAct a{W, ct, coff};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Degree count and Java running mean of one record (whole records also get
+// the centrality formula).
 struct DcAct {
   Window W;
   int32_t* count;
   int32_t* centr;
-  int64_t* tbuf = nullptr;
-  const uint64_t* coff = nullptr;
-  int64_t* tb = nullptr;
   int32_t deg = 0;
   int64_t avg = 0;
-  __device__ bool begin(int32_t x, bool chunked, Split) {
+  __device__ bool begin(int32_t x, bool, Split) {
     deg = 0;
     avg = 0;
-    tb = (tbuf && chunked) ? tbuf + coff[x] : nullptr;
     return W.active(x);
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
-    const bool counts = W.in(t) && W.active(r);
-    if (counts) {
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
+    if (W.in(t) && W.active(r)) {
       ++deg;
       avg += ((int64_t)t - avg) / deg;  // truncating, as Java's long division
     }
-    if (tb) tb[k] = counts ? (int64_t)t : kNoTime;
     return true;
   }
   __device__ void end(int32_t x, bool chunked);
@@ -431,8 +449,8 @@ __device__ void DcAct::end(int32_t x, bool chunked) {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_dc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* count,
-                                               int32_t* centr, int64_t* tbuf, const uint64_t* coff) {
-  DcAct a{W, count, centr, tbuf, coff};
+                                               int32_t* centr) {
+  DcAct a{W, count, centr};
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
 }
 
@@ -487,26 +505,17 @@ __device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
   return v;
 }
 
-// Running mean of each long record over its stored contact times, one wave
-// per record, 64 contacts a step.  The mean is sequential (avg_k = avg_k-1 +
-// (t_k - avg_k-1) / k, truncating), so each step is speculative: every lane
-// divides with the step's starting mean, a prefix sum of those quotients
-// gives each lane its starting mean, and the lanes recompute; from the first
-// lane whose quotient changed (the lanes before it are exact, and so is its
-// new quotient) the step restarts.  Each round fixes at least one lane.
-__global__ void __launch_bounds__(kBlock) k_dc_rec(const int32_t* bigv, int32_t nbig, const int32_t* vdeg,
-                                                   const uint64_t* coff, const int64_t* tbuf, Window W,
-                                                   int32_t* centr) {
+// Count and Java running mean of the cached contact times tb[0, len) that lie
+// in [ws, we], by one wave, 64 contacts a step.  The mean is sequential
+// (avg_k = avg_k-1 + (t_k - avg_k-1) / k, truncating), so each step is
+// speculative: every lane divides with the step's starting mean, a prefix sum
+// of those quotients gives each lane its starting mean, and the lanes
+// recompute; from the first lane whose quotient changed (the lanes before it
+// are exact, and so is its new quotient) the step restarts.  Each round fixes
+// at least one lane.  Every lane of the wave must call it.
+__device__ void wave_mean(const int64_t* tb, int32_t len, int64_t ws, int64_t we, int32_t& cnt_out,
+                          int64_t& avg_out) {
   const int lane = threadIdx.x & 63;
-  const int32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (wid >= nbig) return;
-  const int32_t x = bigv[wid];
-  if (!W.active(x)) {
-    if (lane == 0) centr[x] = dc_formula(0, 0, W.ws, W.we);
-    return;
-  }
-  const int64_t* tb = tbuf + coff[x];
-  const int32_t len = vdeg[x];
   int64_t avg = 0;
   int32_t cnt = 0;
   // Times are loaded kAhead steps of 64 at a time, the next group while the
@@ -527,7 +536,7 @@ __global__ void __launch_bounds__(kBlock) k_dc_rec(const int32_t* bigv, int32_t 
 #pragma unroll
     for (int j = 0; j < kAhead; ++j) {
       const int64_t t = cur[j];
-      const bool valid = t != kNoTime;
+      const bool valid = t != kNoTime && t >= ws && t <= we;
       const uint64_t vm = __ballot(valid);
       if (!vm) continue;
       const int64_t k = cnt + __popcll(vm & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
@@ -554,7 +563,77 @@ __global__ void __launch_bounds__(kBlock) k_dc_rec(const int32_t* bigv, int32_t 
 #pragma unroll
     for (int j = 0; j < kAhead; ++j) cur[j] = nxt[j];
   }
+  cnt_out = cnt;
+  avg_out = avg;
+}
+
+// Centrality of each long record from the contact-time cache, one wave per
+// record.
+__global__ void __launch_bounds__(kBlock) k_dc_rec(const int32_t* bigv, int32_t nbig, const int32_t* vdeg,
+                                                   const uint64_t* coff, const int64_t* ct, Window W,
+                                                   int32_t* centr) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (wid >= nbig) return;
+  const int32_t x = bigv[wid];
+  int32_t cnt = 0;
+  int64_t avg = 0;
+  if (W.active(x)) wave_mean(ct + coff[x], vdeg[x], W.ws, W.we, cnt, avg);
   if (lane == 0) centr[x] = dc_formula(cnt, avg, W.ws, W.we);
+}
+
+// Batched degree centrality, up to 64 windows (wins: ws, we per query):
+// count / centr row q receives what tga_degree_centrality gives for window q.
+// Records that are not chunked: one wave per vertex, lane q running query q's
+// count and mean over the cached times (the loads are the same for all
+// lanes).  Long records are left to k_dcb_long.
+__global__ void __launch_bounds__(kBlock) k_dcb(Window A, int32_t n, const int32_t* vinfo, const int32_t* vdeg,
+                                                const uint64_t* coff, const int64_t* ct, const int64_t* wins,
+                                                int32_t nq, int32_t* count, int32_t* centr) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? wins[2 * lane] : 0, we = on ? wins[2 * lane + 1] : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); v < n; v += nw) {
+    const bool act = A.active((int32_t)v);
+    if (act && vinfo[v] >= 0) continue;  // chunked: k_dcb_long
+    int32_t cnt = 0;
+    int64_t avg = 0;
+    if (act) {
+      const int64_t* tb = ct + coff[v];
+      const int32_t len = vdeg[v];
+      for (int32_t i = 0; i < len; ++i) {
+        const int64_t t = tb[i];
+        if (t != kNoTime && t >= ws && t <= we) {
+          ++cnt;
+          avg += div_trunc(t - avg, cnt, 1.0 / (double)cnt);
+        }
+      }
+    }
+    if (on) {
+      count[(int64_t)lane * n + v] = cnt;
+      if (centr) centr[(int64_t)lane * n + v] = dc_formula(cnt, avg, ws, we);
+    }
+  }
+}
+
+// Batched degree centrality of long records: one wave per (record, query).
+__global__ void __launch_bounds__(kBlock) k_dcb_long(const int32_t* bigv, int32_t nbig, Window A, int32_t n,
+                                                     const int32_t* vdeg, const uint64_t* coff, const int64_t* ct,
+                                                     const int64_t* wins, int32_t nq, int32_t* count,
+                                                     int32_t* centr) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
+  if (wid >= (int64_t)nbig * nq) return;
+  const int32_t x = bigv[wid / nq], q = (int32_t)(wid % nq);
+  const int64_t ws = wins[2 * q], we = wins[2 * q + 1];
+  int32_t cnt = 0;
+  int64_t avg = 0;
+  if (A.active(x)) wave_mean(ct + coff[x], vdeg[x], ws, we, cnt, avg);
+  if (lane == 0) {
+    count[(int64_t)q * n + x] = cnt;
+    if (centr) centr[(int64_t)q * n + x] = dc_formula(cnt, avg, ws, we);
+  }
 }
 
 // The running mean of long records is sequential: one thread per record.
@@ -1599,8 +1678,11 @@ struct tga_graph {
   // neighbor pairs of PageRank / HITS, allocated on first use (k_pairs)
   uint32_t *pk = nullptr, *sk = nullptr;  // r: fill order, sorted by r
   uint64_t *pv = nullptr, *sv = nullptr;  // (x, w): fill order, sorted by r
-  int64_t* dc_tbuf = nullptr;   // contact times of long records (k_dc_rec), on first use
-  uint64_t* dc_coff = nullptr;  // n + 1: first contact of every record
+  int64_t* ct = nullptr;     // contact-time cache (k_ctimes), on first use
+  uint64_t* coff = nullptr;  // n + 1: first contact of every record
+  int64_t* dcb_win = nullptr;  // batched DC windows (2 x 64)
+  int32_t* dcb_out = nullptr;  // batched DC rows when the caller's output is on the host
+  int32_t dcb_rows = 0;
   uint32_t* ukeys = nullptr;  // n + 2: row keys of a reduce_rows call
   double* usums = nullptr;    // n + 2: their sums
   uint32_t* nruns = nullptr;
@@ -2132,6 +2214,20 @@ static uint64_t item_scan_bytes(const tga_graph* g);
 static int item_scan_alloc(tga_graph* g);
 static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off);
 
+// Builds the contact-time cache (g->ct, g->coff) on first use; *ok is false,
+// with nothing built, when it does not fit list_budget.
+static int ensure_ctimes(tga_graph* g, bool* ok) {
+  const uint64_t contacts = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
+  int rc = list_budget(contacts * 8 + (uint64_t)(g->n + 1) * 8 + item_scan_bytes(g), g->ct != nullptr, ok);
+  if (rc || !*ok || g->ct) return rc;
+  if ((rc = item_scan_alloc(g)) || (rc = g->alloc(g->coff, (size_t)g->n + 1)) ||
+      (rc = scan_counts(g, (const uint32_t*)g->vdeg, g->n, g->coff)) || (rc = g->alloc(g->ct, (size_t)contacts)))
+    return rc;
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  LAUNCH_STATIC(k_ctimes, A, g->ct, (const uint64_t*)g->coff);
+  return TGA_OK;
+}
+
 extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* centrality, uint32_t flags) {
   if (!g || !count || bad_window(w)) return fail(TGA_EINVAL, "tga_degree_centrality: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
@@ -2143,27 +2239,20 @@ extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count,
     const int32_t c0 = (w.start == w.end) ? 0 : 1;
     k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(centr, g->n, c0);
   }
-  // Long records: stored contact times and one wave per record (k_dc_rec)
-  // when the buffer fits list_budget, else one thread decoding each record.
+  // Long records: one wave per record over the contact-time cache (k_dc_rec)
+  // when the cache fits list_budget, else one thread decoding each record.
   bool rec = false;
   if (centr) {
-    const uint64_t contacts = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
-    int rc = list_budget(contacts * 8 + (uint64_t)(g->n + 1) * 8 + item_scan_bytes(g), g->dc_tbuf != nullptr, &rec);
+    int rc = ensure_ctimes(g, &rec);
     if (rc) return rc;
-    if (rec && !g->dc_tbuf) {
-      if ((rc = item_scan_alloc(g)) || (rc = g->alloc(g->dc_tbuf, (size_t)contacts)) ||
-          (rc = g->alloc(g->dc_coff, (size_t)g->n + 1)))
-        return rc;
-      if ((rc = scan_counts(g, (const uint32_t*)g->vdeg, g->n, g->dc_coff))) return rc;
-    }
   }
-  LAUNCH_STATIC(k_dc, W, g->i0, centr, rec ? g->dc_tbuf : nullptr, (const uint64_t*)g->dc_coff);
+  LAUNCH_STATIC(k_dc, W, g->i0, centr);
   if (centr) {
     for (int c = 0; c < kClasses; ++c) {
       if (!g->n_big[c]) continue;
       if (rec) {
         k_dc_rec<<<(g->n_big[c] + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, g->stream>>>(
-            g->big[c], g->n_big[c], g->vdeg, g->dc_coff, g->dc_tbuf, W, centr);
+            g->big[c], g->n_big[c], g->vdeg, g->coff, g->ct, W, centr);
         continue;
       }
       const int blocks = (g->n_big[c] + 63) / 64;
@@ -2177,6 +2266,60 @@ extern "C" int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count,
   int rc = g->out_copy(count, g->i0, (size_t)g->n * 4, flags);
   if (!rc && centrality) rc = g->out_copy(centrality, centr, (size_t)g->n * 4, flags);
   if (rc) return rc;
+  return g->check_err();
+}
+
+extern "C" int tga_degree_centrality_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t* count,
+                                           int32_t* centrality, uint32_t flags) {
+  if (!g || !windows || !count || nq < 1 || nq > 64)
+    return fail(TGA_EINVAL, "tga_degree_centrality_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q])) return fail(TGA_EINVAL, "tga_degree_centrality_batch: window start > end");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  bool ok = false;
+  int rc = ensure_ctimes(g, &ok);
+  if (rc) return rc;
+  if (!ok) {
+    // No room for the cache: one call per window.
+    for (int32_t q = 0; q < nq; ++q)
+      if ((rc = tga_degree_centrality(g, windows[q], count + q * n, centrality ? centrality + q * n : nullptr, flags)))
+        return rc;
+    return TGA_OK;
+  }
+  if (!g->dcb_win && (rc = g->alloc(g->dcb_win, 2 * 64))) return rc;
+  int64_t hw[2 * 64];
+  for (int32_t q = 0; q < nq; ++q) {
+    hw[2 * q] = windows[q].start;
+    hw[2 * q + 1] = windows[q].end;
+  }
+  HIP_TRY(hipMemcpyAsync(g->dcb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, g->stream));
+  int32_t *cnt = count, *cen = centrality;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    const int32_t rows = centrality ? 2 * nq : nq;
+    if (g->dcb_rows < rows) {
+      if ((rc = g->alloc(g->dcb_out, (size_t)rows * (size_t)n))) return rc;
+      g->dcb_rows = rows;
+    }
+    cnt = g->dcb_out;
+    cen = centrality ? g->dcb_out + (int64_t)nq * n : nullptr;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  g->kbegin();
+  k_dcb<<<g->grid(n * 64), kBlock, 0, g->stream>>>(A, (int32_t)n, g->vinfo, g->vdeg, g->coff, g->ct, g->dcb_win, nq,
+                                                 cnt, cen);
+  for (int c = 0; c < kClasses; ++c)
+    if (g->n_big[c])
+      k_dcb_long<<<g->grid((int64_t)g->n_big[c] * nq * 64), kBlock, 0, g->stream>>>(
+          g->big[c], g->n_big[c], A, (int32_t)n, g->vdeg, g->coff, g->ct, g->dcb_win, nq, cnt, cen);
+  g->kend();
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE)) {
+    HIP_TRY(hipMemcpyAsync(count, cnt, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, g->stream));
+    if (centrality)
+      HIP_TRY(hipMemcpyAsync(centrality, cen, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost,
+                             g->stream));
+  }
   return g->check_err();
 }
 

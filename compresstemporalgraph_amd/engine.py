@@ -68,6 +68,7 @@ def lib():
         L.tga_info.argtypes = [vp, C.POINTER(GraphInfo)]
         L.tga_neighbor_access.argtypes = [vp, p32, C.c_int64, pu64, p64]
         L.tga_degree_centrality.argtypes = [vp, Window, vp, vp, C.c_uint32]
+        L.tga_degree_centrality_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, vp, vp, C.c_uint32]
         L.tga_bfs.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_bfs_batch.argtypes = [vp, C.POINTER(Window), p32, C.c_int32, vp, C.c_uint32]
         L.tga_bfs_server_open.argtypes = [vp, C.POINTER(vp)]
@@ -187,6 +188,23 @@ class TemporalGraph:
         c = _Out(self.n, np.int32, out_centrality) if (centrality or out_centrality is not None) else None
         _check(lib().tga_degree_centrality(self.h, _win(window), o.ptr, c.ptr if c else None, o.flags))
         return (o.arr, c.arr) if c else o.arr
+
+    def degree_centrality_batch(self, windows, centrality=True, out=None, out_centrality=None):
+        """Batched degree centrality: up to 64 windows in one call.  Returns
+        (nq, n) int32 counts (and centralities) whose row q equals
+        degree_centrality(windows[q], centrality=...); `out` / `out_centrality`
+        may be a caller's (nq, n) device tensors instead."""
+        nq = len(windows)
+        wins = (Window * nq)(*[_win(w) for w in windows])
+        if out is None:
+            cnt = np.empty((nq, self.n), np.int32)
+            cen = np.empty((nq, self.n), np.int32) if centrality else None
+            pc, pe, flags = cnt.ctypes.data_as(C.c_void_p), cen.ctypes.data_as(C.c_void_p) if centrality else None, 0
+        else:
+            cnt, cen = out, (out_centrality if centrality else None)
+            pc, pe, flags = C.c_void_p(out.data_ptr()), C.c_void_p(cen.data_ptr()) if centrality else None, TGA_OUT_DEVICE
+        _check(lib().tga_degree_centrality_batch(self.h, wins, nq, pc, pe, flags))
+        return (cnt, cen) if centrality else cnt
 
     def bfs(self, src: int, window=None, out=None):
         o = _Out(self.n, np.int32, out)

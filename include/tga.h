@@ -117,6 +117,16 @@ int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint
  * degree_centrality.cu:1606-1633). */
 int tga_degree_centrality(tga_graph* g, tga_window w, int32_t* count, int32_t* centrality, uint32_t flags);
 
+/* Batched degree centrality: nq (1..64) windows in one call.  count (and
+ * centrality, nullable) receive nq rows of n: row q is exactly what
+ * tga_degree_centrality(g, windows[q], ...) returns.  The contacts are
+ * decoded once per handle into a window-independent time cache (8 bytes per
+ * contact, under TGA_LIST_MAX_BYTES) that every query of the batch scans; the
+ * running means of long records run one wave per (record, query).  No
+ * counterpart in the reference (one window per executable run). */
+int tga_degree_centrality_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t* count,
+                                int32_t* centrality, uint32_t flags);
+
 /* BFS levels: hop distance from src over window contacts between active
  * vertices; INT32_MAX when unreached (BaselineRunner.java:339-383; replaces
  * simpleBfs + host level loop, bfs.cu:1657-1692, :1999-2038). */

@@ -85,6 +85,36 @@ def test_degree_centrality(pair, path, monkeypatch):
         np.testing.assert_array_equal(cen, ocen)
 
 
+def test_degree_centrality_batch(pair):
+    """Up to 64 windows in one call (full, ranges, single instants, empty):
+    every row equals the oracle's single-window counts and centrality, and a
+    smaller batch after it on the same handle."""
+    ds, g = pair
+    rng = np.random.default_rng(31)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    wins = []
+    for q in range(64):
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))
+    cnt, cen = g.degree_centrality_batch(wins)
+    o = ds.oracle()
+    for q, w in enumerate(wins):
+        ocnt, ocen = o.degree(window=w, centrality=True)
+        np.testing.assert_array_equal(cnt[q], ocnt, err_msg=f"count, window {q}")
+        np.testing.assert_array_equal(cen[q], ocen, err_msg=f"centrality, window {q}")
+    cnt2 = g.degree_centrality_batch(wins[1:4], centrality=False)
+    np.testing.assert_array_equal(cnt2, cnt[1:4])
+
+
 def test_bfs(pair):
     ds, g = pair
     srcs = [0, int(ds.src[len(ds.src) // 2])]
