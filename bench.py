@@ -24,6 +24,8 @@ Prints one JSON line with the driver's contract fields plus:
                 calls (results compared, outputs in HBM).
   batched_dc    64 windowed degree-centrality queries in one batched call
                 against 64 plain calls (results compared, outputs in HBM).
+  served_mix    Poisson BFS + DC arrivals (DC a fifth) at 1k/s and 5k/s:
+                continuous batching against sequential plain execution.
   served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
                 against sequential plain BFS, queries/s and p50/p99 latency.
 """
@@ -199,6 +201,33 @@ def served_bfs(g, t_lo: int, t_hi: int) -> dict:
     return out
 
 
+def served_mix(g, g_dc, t_lo: int, t_hi: int) -> dict:
+    """Serving a mixed stream: BFS and DC window queries (DC a fifth, as in
+    the batching mix's BFS:DC ratio) at Poisson rates of 1k and 5k per
+    second; BFS by continuous batching in lanes, DC by batched calls on a
+    second handle in a worker thread, against sequential plain execution;
+    results in HBM; identity of all results checked at 1k/s."""
+    import torch
+    from compresstemporalgraph_amd import serving
+    out = {}
+    for rate, count in ((1000, 500), (5000, 1000)):
+        req = serving.mixed_requests(rate, count, g.info["n"], t_lo, t_hi, dc_share=0.2, seed=rate + 7)
+        kb, kp = ({}, {}) if rate == 1000 else (None, None)
+        sb = serving.serve_batched(g, req, keep=kb, device_out=True, dc_graph=g_dc)
+        sp = serving.serve_plain(g, req, keep=kp, device_out=True)
+        row = {"batched": sb, "plain": sp}
+        if kb is not None:
+            def same(a, b):
+                if isinstance(a, tuple):
+                    return all(torch.equal(x, y) for x, y in zip(a, b))
+                return torch.equal(a, b)
+            row["identical"] = bool(len(kb) == count and all(same(kb[i], kp[i]) for i in kb))
+            if not row["identical"]:
+                raise SystemExit("served mix results differ from the plain results")
+        out[f"{rate}/s"] = row
+    return out
+
+
 def query_legs(args) -> int:
     """Batched and served BFS legs (child process of the default run)."""
     import torch
@@ -219,8 +248,13 @@ def query_legs(args) -> int:
         raise SystemExit("batched DC results differ from the plain results")
     served = served_bfs(g, t_lo, t_hi)
     log(f"[bench] served BFS: {served}")
+    g_dc = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    smix = served_mix(g, g_dc, t_lo, t_hi)
+    g_dc.close()
+    log(f"[bench] served BFS+DC mix: {smix}")
     g.close()
-    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "served_bfs": served}), flush=True)
+    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "served_bfs": served, "served_mix": smix}),
+          flush=True)
     return 0
 
 
@@ -344,7 +378,7 @@ def main() -> int:
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
-    batched = bdc = served = None
+    batched = bdc = served = smix = None
     if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -354,11 +388,17 @@ def main() -> int:
         cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
         if args.data_root:
             cmd += ["--data-root", args.data_root]
-        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        # Eight hardware queues: the served mix runs DC batches on a second
+        # handle beside the BFS loop, and with HIP's default of four the two
+        # handles' streams share queues, so a long DC kernel stalls BFS launches
+        # queued behind it.
+        env = dict(os.environ)
+        env.setdefault("GPU_MAX_HW_QUEUES", "8")
+        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
-        batched, bdc, served = legs["batched_bfs"], legs["batched_dc"], legs["served_bfs"]
+        batched, bdc, served, smix = legs["batched_bfs"], legs["batched_dc"], legs["served_bfs"], legs["served_mix"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -388,6 +428,7 @@ def main() -> int:
         "workloads_ms": workloads,
         "batched_bfs": batched,
         "batched_dc": bdc,
+        "served_mix": smix,
         "served_bfs": served,
     }
     print(json.dumps(line), flush=True)

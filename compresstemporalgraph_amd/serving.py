@@ -1,14 +1,16 @@
-"""Serving form: BFS window queries arriving over time, served by continuous
-batching on one GPU.
+"""Serving form: BFS and degree-centrality (DC) window queries arriving over
+time, served by continuous batching on one GPU.
 
-Requests are (arrival time, source, window).  The loop admits every request
-that has arrived into a free lane of the BFS server (up to 64 at once, fewer
-when the per-query state would exceed the memory budget), advances all
-running queries by one BFS level per decode pass, and completes a query as
-soon as its frontier empties -- so a query that arrives while others are on
-the GPU joins them at the next level instead of waiting for the batch to
-drain.  The plain baseline serves the same arrivals one query at a time with
-TemporalGraph.bfs, in arrival order.
+Requests are (arrival time, kind, source, window).  The loop admits every BFS
+request that has arrived into a free lane of the BFS server (up to 64 at
+once, fewer when the per-query state would exceed the memory budget),
+advances all running queries by one BFS level per decode pass, and completes
+a query as soon as its frontier empties -- so a query that arrives while
+others are on the GPU joins them at the next level instead of waiting for the
+batch to drain.  DC requests that have arrived are answered together by one
+batched call (up to 64 windows) at the next loop step.  The plain baseline
+serves the same arrivals one query at a time (TemporalGraph.bfs,
+TemporalGraph.degree_centrality), in arrival order.
 
 Latency is completion time minus arrival time; throughput is completed
 queries over the span from the first arrival to the last completion.
@@ -28,6 +30,7 @@ class Request:
     arrival: float  # seconds after the start of the run
     src: int
     window: tuple
+    kind: str = "BFS"  # "BFS" or "DC"
 
 
 def poisson_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, seed: int = 0) -> list:
@@ -41,6 +44,17 @@ def poisson_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, seed
     return [Request(float(a), int(s), (int(b), int(b + l))) for a, s, b, l in zip(arrivals, srcs, starts, lens)]
 
 
+def mixed_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, dc_share: float = 0.2,
+                   seed: int = 0) -> list:
+    """poisson_requests with a share of DC requests (kind "DC"; src unused)."""
+    req = poisson_requests(rate, count, n, t_lo, t_hi, seed)
+    rng = np.random.default_rng(seed + 1)
+    for r, u in zip(req, rng.random(count)):
+        if u < dc_share:
+            r.kind = "DC"
+    return req
+
+
 def lanes_for_budget(n: int, budget_bytes: int) -> int:
     """Queries that fit the memory budget: each holds an int32 level row of n
     entries (the three shared u64 masks are charged once)."""
@@ -51,47 +65,149 @@ def lanes_for_budget(n: int, budget_bytes: int) -> int:
 def _stats(req, done_at) -> dict:
     lat = np.array([done_at[i] - r.arrival for i, r in enumerate(req)])
     span = max(done_at) - min(r.arrival for r in req)
-    return {"queries": len(req), "qps": round(len(req) / span, 1),
-            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    out = {"queries": len(req), "qps": round(len(req) / span, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    kinds = sorted({r.kind for r in req})
+    if len(kinds) > 1:
+        for k in kinds:
+            lk = np.array([lat[i] for i, r in enumerate(req) if r.kind == k])
+            out[k] = {"queries": int(len(lk)), "p50_ms": round(float(np.percentile(lk, 50)) * 1e3, 3),
+                      "p99_ms": round(float(np.percentile(lk, 99)) * 1e3, 3)}
+    return out
 
 
-def _sink(g: TemporalGraph, device_out: bool):
+def _sink(g: TemporalGraph, device_out: bool, rows: int = 0):
     """Output buffer for one result: a host array, or a device row (results
-    then stay in HBM, as a server handing them to GPU consumers would)."""
+    then stay in HBM, as a server handing them to GPU consumers would); with
+    rows, a (rows, n) block."""
     if not device_out:
         return lambda: None
     import torch
-    return lambda: torch.empty(g.n, dtype=torch.int32, device=f"cuda:{g.device}")
+    shape = (rows, g.n) if rows else (g.n,)
+    return lambda: torch.empty(shape, dtype=torch.int32, device=f"cuda:{g.device}")
+
+
+def _dc_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
+    """One batched DC call for requests idx; returns [(count, centrality)] rows."""
+    wins = [req[i].window for i in idx]
+    if device_out:
+        blk = _sink(g, True, len(idx))
+        cnt, cen = g.degree_centrality_batch(wins, out=blk(), out_centrality=blk())
+    else:
+        cnt, cen = g.degree_centrality_batch(wins)
+    return [(cnt[j], cen[j]) for j in range(len(idx))]
+
+
+class _DcWorker:
+    """Answers DC requests on a second handle of the same graph, in a thread
+    of its own: each wake-up takes every pending request (up to 64) into one
+    batched call.  The handle has its own stream, so these batches overlap
+    the BFS levels of the main loop on the device (the engine's calls release
+    the interpreter lock while they wait for the GPU)."""
+
+    def __init__(self, g: TemporalGraph, req: list, device_out: bool, t0: float, done_at: list, keep):
+        import threading
+        self.g, self.req, self.device_out, self.t0 = g, req, device_out, t0
+        self.done_at, self.keep = done_at, keep
+        self.cv = threading.Condition()
+        self.pending, self.open, self.outstanding, self.error = [], True, 0, None
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def put(self, i: int) -> None:
+        with self.cv:
+            self.pending.append(i)
+            self.outstanding += 1
+            self.cv.notify()
+
+    def busy(self) -> bool:
+        with self.cv:
+            if self.error is not None:
+                raise self.error
+            return self.outstanding > 0
+
+    def close(self) -> None:
+        with self.cv:
+            self.open = False
+            self.cv.notify()
+        self.thread.join()
+
+    def _run(self) -> None:
+        while True:
+            with self.cv:
+                while self.open and not self.pending:
+                    self.cv.wait()
+                if not self.pending:
+                    return
+                idx, self.pending = self.pending[:64], self.pending[64:]
+            try:
+                rows = _dc_batch(self.g, self.req, idx, self.device_out)
+            except Exception as e:  # surfaced by busy()
+                with self.cv:
+                    self.error, self.outstanding = e, 0
+                return
+            t = time.perf_counter() - self.t0
+            with self.cv:
+                for i, row in zip(idx, rows):
+                    self.done_at[i] = t
+                    if self.keep is not None:
+                        self.keep[i] = row
+                self.outstanding -= len(idx)
 
 
 def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep=None,
-                  device_out: bool = False) -> dict:
+                  device_out: bool = False, dc_graph: TemporalGraph = None) -> dict:
     """Continuous batching.  `keep` (a dict) receives the results per request
-    index when given (for identity checks)."""
+    index when given (for identity checks).  With `dc_graph` (a second handle
+    of the same graph), DC batches run beside the BFS loop in a worker thread
+    instead of between its steps."""
     lanes = lanes_for_budget(g.n, budget_bytes)
     sink = _sink(g, device_out)
     done_at = [0.0] * len(req)
     order = sorted(range(len(req)), key=lambda i: req[i].arrival)
     with g.bfs_server() as srv:
-        lane_of = {}
+        lane_of, dc_wait = {}, []
         nxt = 0
         t0 = time.perf_counter()
-        while len(done_at) and (nxt < len(order) or lane_of):
-            now = time.perf_counter() - t0
-            while nxt < len(order) and req[order[nxt]].arrival <= now and len(lane_of) < lanes:
-                i = order[nxt]
-                lane_of[srv.admit(req[i].src, req[i].window)] = i
-                nxt += 1
-            if not lane_of:  # idle until the next arrival
-                time.sleep(max(0.0, req[order[nxt]].arrival - now))
-                continue
-            for lane in srv.step():
-                i = lane_of.pop(lane)
-                levels = srv.result(lane, out=sink())
-                done_at[i] = time.perf_counter() - t0
-                if keep is not None:
-                    keep[i] = levels
+        worker = _DcWorker(dc_graph, req, device_out, t0, done_at, keep) if dc_graph is not None else None
+        try:
+            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or (worker and worker.busy())):
+                now = time.perf_counter() - t0
+                while nxt < len(order) and req[order[nxt]].arrival <= now:
+                    i = order[nxt]
+                    if req[i].kind == "DC":
+                        if worker:
+                            worker.put(i)
+                        else:
+                            dc_wait.append(i)
+                    elif len(lane_of) < lanes:
+                        lane_of[srv.admit(req[i].src, req[i].window)] = i
+                    else:
+                        break  # BFS lanes full: this request waits (arrival order kept)
+                    nxt += 1
+                if not lane_of and not dc_wait:  # idle until the next arrival (or the worker)
+                    wake = req[order[nxt]].arrival - now if nxt < len(order) else 1e-4
+                    time.sleep(max(0.0, min(wake, 1e-3) if worker else wake))
+                    continue
+                if dc_wait:
+                    idx, dc_wait = dc_wait[:64], dc_wait[64:]
+                    rows = _dc_batch(g, req, idx, device_out)
+                    t = time.perf_counter() - t0
+                    for i, row in zip(idx, rows):
+                        done_at[i] = t
+                        if keep is not None:
+                            keep[i] = row
+                if lane_of:
+                    for lane in srv.step():
+                        i = lane_of.pop(lane)
+                        levels = srv.result(lane, out=sink())
+                        done_at[i] = time.perf_counter() - t0
+                        if keep is not None:
+                            keep[i] = levels
+        finally:
+            if worker:
+                worker.close()
     out = _stats(req, done_at)
     out["lanes"] = lanes
     return out
@@ -107,8 +223,11 @@ def serve_plain(g: TemporalGraph, req: list, keep=None, device_out: bool = False
         now = time.perf_counter() - t0
         if req[i].arrival > now:
             time.sleep(req[i].arrival - now)
-        levels = g.bfs(req[i].src, req[i].window, out=sink())
+        if req[i].kind == "DC":
+            res = g.degree_centrality(req[i].window, centrality=True, out=sink(), out_centrality=sink())
+        else:
+            res = g.bfs(req[i].src, req[i].window, out=sink())
         done_at[i] = time.perf_counter() - t0
         if keep is not None:
-            keep[i] = levels
+            keep[i] = res
     return _stats(req, done_at)

@@ -242,3 +242,34 @@ def test_hits(pair, path, monkeypatch):
         h32, a32 = ds.oracle().hits(20, window=w, dtype=np.float32)
         assert rel_linf(hub, h32) <= REL_TOL
         assert rel_linf(auth, a32) <= REL_TOL
+
+
+@pytest.mark.parametrize("worker", [False, True])
+def test_served_mix(pair, worker):
+    """A BFS + DC request stream through the serving loop (BFS in server
+    lanes, DC in batched calls, between the BFS steps or on a second handle
+    in a worker thread): every result equals the oracle's answer to that
+    query alone."""
+    from compresstemporalgraph_amd import serving
+    ds, g = pair
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    req = serving.mixed_requests(rate=2e4, count=80, n=ds.n, t_lo=lo, t_hi=hi + 1, dc_share=0.3, seed=13)
+    for r in req:  # sources with contacts, windows clipped to the span
+        r.src = int(ds.src[r.src % len(ds.src)])
+        r.window = (r.window[0], min(r.window[1], hi))
+    keep = {}
+    g2 = TemporalGraph(ds.dir, ds.name) if worker else None
+    try:
+        serving.serve_batched(g, req, keep=keep, dc_graph=g2)
+    finally:
+        if g2 is not None:
+            g2.close()
+    o = ds.oracle()
+    assert sorted(keep) == list(range(len(req)))
+    for i, r in enumerate(req):
+        if r.kind == "DC":
+            ocnt, ocen = o.degree(window=r.window, centrality=True)
+            np.testing.assert_array_equal(keep[i][0], ocnt, err_msg=f"request {i} (DC)")
+            np.testing.assert_array_equal(keep[i][1], ocen, err_msg=f"request {i} (DC)")
+        else:
+            np.testing.assert_array_equal(keep[i], o.bfs(r.src, window=r.window), err_msg=f"request {i} (BFS)")

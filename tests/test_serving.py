@@ -1,0 +1,150 @@
+"""Host logic of the serving loop (compresstemporalgraph_amd/serving.py) on
+the CPU: request generation, the memory budget, latency statistics, and the
+admission / batching / result bookkeeping of serve_batched and serve_plain
+against a stand-in graph whose queries finish after a known number of steps.
+The GPU parity of the served results is in test_gpu_parity.py."""
+import numpy as np
+
+from compresstemporalgraph_amd import serving
+
+
+def test_poisson_requests_shape():
+    req = serving.poisson_requests(rate=1000, count=400, n=5000, t_lo=100, t_hi=1 << 20, seed=9)
+    assert len(req) == 400
+    arr = np.array([r.arrival for r in req])
+    assert np.all(np.diff(arr) > 0)
+    # mean inter-arrival close to 1/rate
+    assert abs(arr[-1] / len(req) - 1e-3) < 3e-4
+    lens = np.array([r.window[1] - r.window[0] for r in req])
+    assert lens.min() >= 2 ** 10 - 1 and lens.max() <= 2 ** 24
+    assert all(100 <= r.window[0] < (1 << 20) for r in req)
+    assert all(0 <= r.src < 5000 and r.kind == "BFS" for r in req)
+    # seeded
+    again = serving.poisson_requests(rate=1000, count=400, n=5000, t_lo=100, t_hi=1 << 20, seed=9)
+    assert [(r.arrival, r.src, r.window) for r in again] == [(r.arrival, r.src, r.window) for r in req]
+
+
+def test_mixed_requests_share():
+    base = serving.poisson_requests(2000, 3000, 100, 0, 1 << 24, seed=4)
+    mix = serving.mixed_requests(2000, 3000, 100, 0, 1 << 24, dc_share=0.25, seed=4)
+    assert [(r.arrival, r.src, r.window) for r in mix] == [(r.arrival, r.src, r.window) for r in base]
+    kinds = [r.kind for r in mix]
+    assert set(kinds) == {"BFS", "DC"}
+    assert abs(kinds.count("DC") / len(kinds) - 0.25) < 0.03
+
+
+def test_lanes_for_budget():
+    n = 1 << 20
+    shared = 3 * 8 * n
+    assert serving.lanes_for_budget(n, 8 << 30) == 64
+    assert serving.lanes_for_budget(n, shared + 10 * 4 * n + 5) == 10
+    assert serving.lanes_for_budget(n, 0) == 1
+
+
+def test_stats_percentiles_and_kinds():
+    req = [serving.Request(arrival=float(i), src=0, window=(0, 1), kind="DC" if i % 4 == 0 else "BFS")
+           for i in range(100)]
+    lat = np.linspace(0.001, 0.100, 100)
+    done = [r.arrival + l for r, l in zip(req, lat)]
+    st = serving._stats(req, done)
+    assert st["queries"] == 100
+    assert abs(st["p50_ms"] - np.percentile(lat, 50) * 1e3) < 1e-6
+    assert abs(st["p99_ms"] - np.percentile(lat, 99) * 1e3) < 1e-6
+    assert abs(st["qps"] - round(100 / (done[-1] - 0.0), 1)) < 1e-9
+    assert st["DC"]["queries"] == 25 and st["BFS"]["queries"] == 75
+    assert abs(st["DC"]["p50_ms"] - np.percentile(lat[::4], 50) * 1e3) < 1e-6
+
+
+class _FakeServer:
+    """Finishes the query admitted to a lane after `src % 3 + 1` steps."""
+
+    LANES = 64
+
+    def __init__(self, log):
+        self.left, self.src, self.log = {}, {}, log
+
+    def admit(self, src, window):
+        lane = min(set(range(self.LANES)) - set(self.left))
+        self.left[lane], self.src[lane] = src % 3 + 1, (src, window)
+        self.log.append(("admit", src))
+        return lane
+
+    def step(self):
+        assert self.left, "step with no running query"
+        fin = []
+        for lane in list(self.left):
+            self.left[lane] -= 1
+            if self.left[lane] == 0:
+                fin.append(lane)
+                del self.left[lane]
+        return fin
+
+    def result(self, lane, out=None):
+        return ("bfs",) + self.src.pop(lane)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        pass
+
+
+class _FakeGraph:
+    n = 10
+    device = 0
+
+    def __init__(self):
+        self.log, self.dc_calls = [], []
+
+    def bfs_server(self):
+        return _FakeServer(self.log)
+
+    def degree_centrality_batch(self, wins, out=None, out_centrality=None):
+        assert 1 <= len(wins) <= 64
+        self.dc_calls.append(list(wins))
+        return [("cnt", w) for w in wins], [("cen", w) for w in wins]
+
+    def bfs(self, src, window=None, out=None):
+        return ("bfs", src, window)
+
+    def degree_centrality(self, window=None, centrality=False, out=None, out_centrality=None):
+        return ("cnt", window), ("cen", window)
+
+
+def test_serve_loop_bookkeeping():
+    """Every request is answered exactly once with its own result, in both
+    loops; DC requests go through batched calls of at most 64; BFS requests
+    are admitted in arrival order."""
+    req = serving.mixed_requests(rate=1e6, count=300, n=1000, t_lo=0, t_hi=1 << 20, dc_share=0.3, seed=2)
+    g = _FakeGraph()
+    kb, kp = {}, {}
+    sb = serving.serve_batched(g, req, keep=kb)
+    sp = serving.serve_plain(g, req, keep=kp)
+    assert sorted(kb) == list(range(len(req))) == sorted(kp)
+    for i, r in enumerate(req):
+        if r.kind == "DC":
+            assert kb[i] == (("cnt", r.window), ("cen", r.window)) == kp[i]
+        else:
+            assert kb[i] == ("bfs", r.src, r.window) == kp[i]
+    ndc = sum(r.kind == "DC" for r in req)
+    assert sum(len(c) for c in g.dc_calls) == ndc and max(len(c) for c in g.dc_calls) <= 64
+    order = sorted((r.arrival, r.src) for r in req if r.kind == "BFS")
+    assert [s for k, s in g.log if k == "admit"] == [s for _, s in order]
+    assert sb["queries"] == sp["queries"] == len(req) and sb["lanes"] == 64
+
+
+def test_serve_loop_dc_worker():
+    """The same stream with DC batches on a second handle in a worker thread:
+    every request answered once with its own result, DC calls only on the
+    second handle, at most 64 windows each."""
+    req = serving.mixed_requests(rate=1e5, count=300, n=1000, t_lo=0, t_hi=1 << 20, dc_share=0.3, seed=5)
+    g, g2 = _FakeGraph(), _FakeGraph()
+    kb = {}
+    st = serving.serve_batched(g, req, keep=kb, dc_graph=g2)
+    assert sorted(kb) == list(range(len(req))) and st["queries"] == len(req)
+    for i, r in enumerate(req):
+        want = (("cnt", r.window), ("cen", r.window)) if r.kind == "DC" else ("bfs", r.src, r.window)
+        assert kb[i] == want
+    assert not g.dc_calls
+    assert sum(len(c) for c in g2.dc_calls) == sum(r.kind == "DC" for r in req)
+    assert max(len(c) for c in g2.dc_calls) <= 64
