@@ -24,6 +24,8 @@ Prints one JSON line with the driver's contract fields plus:
                 calls (results compared, outputs in HBM).
   batched_dc    64 windowed degree-centrality queries in one batched call
                 against 64 plain calls (results compared, outputs in HBM).
+  batched_pr    64 PageRank queries (20 iterations) in one batched call
+                against 64 plain calls, windowed and full windows.
   served_mix    Poisson BFS + DC arrivals (DC a fifth) at 1k/s and 5k/s:
                 continuous batching against sequential plain execution.
   served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
@@ -179,6 +181,46 @@ def batched_dc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
             "mix": "DC with centrality, windows log-uniform in [2^10, 2^24] with uniform starts"}
 
 
+def batched_pr(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 PageRank queries (20 iterations, FP32) in one tga_pagerank_batch
+    call against 64 plain calls, outputs in HBM: windows log-uniform in
+    [2^10, 2^24] (the batching mix's PR share) and full windows.  The first
+    batched call builds the per-handle run structure and is reported apart;
+    rows are compared with the plain results (max relative L-inf)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(9)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    out = {}
+    for name, wins in (("windowed", [(int(a), int(a + b)) for a, b in zip(starts, lens)]), ("full", [None] * nq)):
+        bat = torch.empty((nq, n), dtype=torch.float32, device="cuda:0")
+        pla = torch.empty((nq, n), dtype=torch.float32, device="cuda:0")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.pagerank_batch(wins, iters=20, out=bat)
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g.pagerank_batch(wins, iters=20, out=bat)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        g.pagerank(wins[0], iters=20, out=pla[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in range(nq):
+            g.pagerank(wins[q], iters=20, out=pla[q])
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        rel = float(((bat.double() - pla.double()).abs().amax(dim=1) /
+                     pla.double().abs().amax(dim=1).clamp_min(1e-300)).max())
+        out[name] = {"queries": nq, "batched_ms": round(tb * 1e3, 3), "first_call_ms": round(first * 1e3, 3),
+                     "plain_ms": round(tp * 1e3, 3), "batched_qps": round(nq / tb, 1),
+                     "plain_qps": round(nq / tp, 1), "speedup": round(tp / tb, 2), "max_rel_diff": rel}
+    return out
+
+
 def served_bfs(g, t_lo: int, t_hi: int) -> dict:
     """Serving: BFS window queries at Poisson rates of 1k and 10k per second
     (log-uniform window lengths in [2^10, 2^24], random sources), continuous
@@ -246,6 +288,10 @@ def query_legs(args) -> int:
     log(f"[bench] batched DC: {bdc}")
     if not bdc["identical"]:
         raise SystemExit("batched DC results differ from the plain results")
+    bpr = batched_pr(g, t_lo, t_hi)
+    log(f"[bench] batched PR: {bpr}")
+    if max(v["max_rel_diff"] for v in bpr.values()) > 1e-5:
+        raise SystemExit("batched PageRank differs from the plain results beyond 1e-5")
     served = served_bfs(g, t_lo, t_hi)
     log(f"[bench] served BFS: {served}")
     g_dc = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
@@ -253,8 +299,8 @@ def query_legs(args) -> int:
     g_dc.close()
     log(f"[bench] served BFS+DC mix: {smix}")
     g.close()
-    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "served_bfs": served, "served_mix": smix}),
-          flush=True)
+    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_pr": bpr, "served_bfs": served,
+                      "served_mix": smix}), flush=True)
     return 0
 
 
@@ -378,7 +424,7 @@ def main() -> int:
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
-    batched = bdc = served = smix = None
+    batched = bdc = bpr = served = smix = None
     if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -398,7 +444,8 @@ def main() -> int:
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
-        batched, bdc, served, smix = legs["batched_bfs"], legs["batched_dc"], legs["served_bfs"], legs["served_mix"]
+        batched, bdc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_pr"]
+        served, smix = legs["served_bfs"], legs["served_mix"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -428,6 +475,7 @@ def main() -> int:
         "workloads_ms": workloads,
         "batched_bfs": batched,
         "batched_dc": bdc,
+        "batched_pr": bpr,
         "served_mix": smix,
         "served_bfs": served,
     }

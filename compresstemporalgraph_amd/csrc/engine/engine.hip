@@ -127,6 +127,10 @@ struct Items {
 #define FOR_ITEMS(I, i) \
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (I).count; i += gridDim.x * blockDim.x)
 
+// Per-item arrays (list counts and offsets) index whole records by vertex
+// and checkpoint chunk j as n + j.
+__device__ __forceinline__ int32_t item_index(int32_t code, int32_t n) { return code >= 0 ? code : n + (-code - 1); }
+
 // ------------------------------------------------------- schedule kernels ---
 
 // Record header of every vertex: outdegree and reference distance.
@@ -374,10 +378,17 @@ __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items 
 }
 
 // Contact-time cache (window-independent, built once per handle by
-// k_ctimes): the time of contact k of active x at ct[coff[x] + k], or kNoTime
-// when its neighbor is inactive.  The degree centrality of long records and
-// the batched degree centrality read it instead of decoding.
-constexpr int64_t kNoTime = INT64_MIN;
+// k_ctimes): the time t of contact k of active x at ct[coff[x] + k], or
+// t + kInactiveBias when its neighbor is inactive.  Biased values lie above
+// the int32 range, so a window clamped to int32 (clamp_lo / clamp_hi; t is
+// int32, so nothing changes) never holds them.  The degree centrality of long
+// records and the batched degree centrality and PageRank read it instead of
+// decoding.
+constexpr int64_t kNoTime = INT64_MIN;  // below every window (padding)
+constexpr int64_t kInactiveBias = (int64_t)1 << 40;
+__device__ __forceinline__ int64_t ct_time(int64_t v) { return v >= (kInactiveBias >> 1) ? v - kInactiveBias : v; }
+__host__ __device__ __forceinline__ int64_t clamp_lo(int64_t ws) { return ws < INT32_MIN ? (int64_t)INT32_MIN : ws; }
+__host__ __device__ __forceinline__ int64_t clamp_hi(int64_t we) { return we > INT32_MAX ? (int64_t)INT32_MAX : we; }
 
 struct CtAct {
   Window W;  // active range
@@ -389,7 +400,7 @@ struct CtAct {
     return W.active(x);
   }
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t k) {
-    tb[k] = W.active(r) ? (int64_t)t : kNoTime;
+    tb[k] = W.active(r) ? (int64_t)t : (int64_t)t + kInactiveBias;
     return true;
   }
   __device__ void end(int32_t, bool) {}
@@ -516,6 +527,8 @@ __device__ __forceinline__ int64_t wave_incl_scan64(int64_t v) {
 __device__ void wave_mean(const int64_t* tb, int32_t len, int64_t ws, int64_t we, int32_t& cnt_out,
                           int64_t& avg_out) {
   const int lane = threadIdx.x & 63;
+  ws = clamp_lo(ws);
+  we = clamp_hi(we);
   int64_t avg = 0;
   int32_t cnt = 0;
   // Times are loaded kAhead steps of 64 at a time, the next group while the
@@ -593,6 +606,7 @@ __global__ void __launch_bounds__(kBlock) k_dcb(Window A, int32_t n, const int32
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
   const int64_t ws = on ? wins[2 * lane] : 0, we = on ? wins[2 * lane + 1] : 0;
+  const int64_t cws = clamp_lo(ws), cwe = clamp_hi(we);
   const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
   for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); v < n; v += nw) {
     const bool act = A.active((int32_t)v);
@@ -604,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_dcb(Window A, int32_t n, const int32
       const int32_t len = vdeg[v];
       for (int32_t i = 0; i < len; ++i) {
         const int64_t t = tb[i];
-        if (t != kNoTime && t >= ws && t <= we) {
+        if (t >= cws && t <= cwe) {
           ++cnt;
           avg += div_trunc(t - avg, cnt, 1.0 / (double)cnt);
         }
@@ -633,6 +647,398 @@ __global__ void __launch_bounds__(kBlock) k_dcb_long(const int32_t* bigv, int32_
   if (lane == 0) {
     count[(int64_t)q * n + x] = cnt;
     if (centr) centr[(int64_t)q * n + x] = dc_formula(cnt, avg, ws, we);
+  }
+}
+
+// ------------------------------------------------- batched PageRank ---
+//
+// Up to 64 PageRank queries (windows) advance together as FP32 columns: rank,
+// coefficient and window degree are [vertex][64] arrays, so lane q of a wave
+// holds query q and a vertex's 64 values are one coalesced row.  The pair
+// structure is window-independent and built once per handle: every run of
+// equal neighbors in every record, (x, r, contacts [k0, k1)), from a counting
+// and a filling decode pass (a chunk skips the run that continues into it from
+// the previous chunk, so each run is listed once, where it starts, and ends
+// where the next run of its record starts), radix-sorted by r (runs to
+// inactive r last) and cut into segments of at most kPrbSeg runs.  Query q's
+// weight of a run is 1 - exp(-0.1 t) at the run's first contact inside its
+// window, read from the contact-time cache.  An iteration is one wave per
+// segment: lane q sums (double)(coef[x][q] * w) over the segment's runs in
+// order; rows of several segments are summed in segment order by k_prb_fix.
+constexpr int kPrbSeg = 512;
+constexpr int kPrbLongDeg = 1024;  // window degrees of longer records: one wave per (record, query)
+constexpr int kPrbRedBlocks = 512;
+
+struct RunAct {
+  Window A;             // active range
+  uint32_t* cnt;        // counting pass: per item
+  const uint64_t* off;  // filling pass: per item
+  int32_t *rx, *rr, *rk;
+  int32_t n;
+  uint64_t pos = 0;
+  uint32_t c = 0;
+  int32_t idx = 0, x = 0, last = -1, cont_r = -1;
+  bool cont = false;
+  __device__ void item(int32_t code) { idx = item_index(code, n); }
+  __device__ bool begin(int32_t v, bool, Split s) {
+    if (!A.active(v)) return false;
+    x = v;
+    c = 0;
+    last = -1;
+    cont = s.cont_slot >= 0;
+    cont_r = s.cont_r;
+    if (off) pos = off[idx];
+    return true;
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t, int32_t k) {
+    if (cont) {
+      if (r == cont_r) return true;  // the run that started in the previous chunk
+      cont = false;
+    }
+    if (r == last) return true;
+    last = r;
+    if (off) {
+      rx[pos] = x;
+      rr[pos] = r;
+      rk[pos] = k;
+      ++pos;
+    } else {
+      ++c;
+    }
+    return true;
+  }
+  __device__ void end(int32_t, bool) {
+    if (!off) cnt[idx] = c;
+  }
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_runs(Streams S, const Ckpt* ck, Items I, Window A, uint32_t* cnt,
+                                                 const uint64_t* off, int32_t* rx, int32_t* rr, int32_t* rk) {
+  RunAct a{A, cnt, off, rx, rr, rk, S.n};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
+// Run ends (the next run of the record, or its outdegree) and sort keys
+// (r, or n for an inactive r, which sorts last); vals = run index.
+__global__ void __launch_bounds__(kBlock) k_run_keys(int64_t nr, const int32_t* rx, const int32_t* rr,
+                                                     const int32_t* rk, const int32_t* vdeg, Window A, int32_t n,
+                                                     int32_t* k1, uint32_t* keys, uint64_t* vals) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nr; i += (int64_t)gridDim.x * kBlock) {
+    k1[i] = (i + 1 < nr && rx[i + 1] == rx[i]) ? rk[i + 1] : vdeg[rx[i]];
+    keys[i] = A.active(rr[i]) ? (uint32_t)rr[i] : (uint32_t)n;
+    vals[i] = (uint64_t)i;
+  }
+}
+
+// in_off[v] = first position of key v in the sorted keys, v in [0, n].
+__global__ void __launch_bounds__(kBlock) k_key_bounds(const uint32_t* sk, uint64_t count, int32_t n,
+                                                       uint64_t* in_off) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v <= n; v += (int64_t)gridDim.x * kBlock) {
+    uint64_t lo = 0, hi = count;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (sk[mid] < (uint32_t)v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    in_off[v] = lo;
+  }
+}
+
+// Sorted runs as (x, k0, k1, t0) with t0 the time of the run's first contact,
+// and f0 = 1 - exp(-0.1 t0), its weight for every query whose window holds t0.
+__global__ void __launch_bounds__(kBlock) k_run_gather(uint64_t nv, const uint64_t* sv, const int32_t* rx,
+                                                       const int32_t* rk, const int32_t* k1, const uint64_t* coff,
+                                                       const int64_t* ct, int4* runs, float* f0) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < (int64_t)nv; j += (int64_t)gridDim.x * kBlock) {
+    const uint64_t i = sv[j];
+    const int32_t t0 = (int32_t)ct[coff[rx[i]] + rk[i]];  // r is active: the time is stored as is
+    runs[j] = int4{rx[i], rk[i], k1[i], t0};
+    f0[j] = 1.0f - expf(-0.1f * (float)t0);
+  }
+}
+
+// Runs with more than one contact: flag (for a scan), then their weight slot
+// midx (-1 for a single-contact run) and the list of them.
+__global__ void __launch_bounds__(kBlock) k_multi_flag(uint64_t nv, const int4* runs, uint32_t* flag) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < (int64_t)nv; j += (int64_t)gridDim.x * kBlock)
+    flag[j] = runs[j].z - runs[j].y > 1 ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_multi_list(uint64_t nv, const int4* runs, const uint64_t* off,
+                                                       int32_t* midx, int32_t* mlist) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < (int64_t)nv; j += (int64_t)gridDim.x * kBlock) {
+    const bool multi = runs[j].z - runs[j].y > 1;
+    midx[j] = multi ? (int32_t)off[j] : -1;
+    if (multi) mlist[off[j]] = (int32_t)j;
+  }
+}
+
+// Per batch: query q's weight of every multi-contact run, 1 - exp(-0.1 t) at
+// its first contact inside window q (0 when none), one wave per run with the
+// lanes over 64 contacts at a time and one ballot per query still open.
+__global__ void __launch_bounds__(kBlock) k_prb_wmulti(const int32_t* mlist, int32_t nm, const int4* runs,
+                                                       const uint64_t* coff, const int64_t* ct, const int64_t* wins,
+                                                       int32_t nq, float* wm) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const uint64_t all = nq >= 64 ? ~0ull : ((1ull << nq) - 1);
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nm; i += nw) {
+    const int4 run = runs[mlist[i]];
+    const int64_t* tb = ct + coff[run.x];
+    uint64_t found = 0;
+    float w = 0.0f;
+    for (int32_t b0 = run.y; b0 < run.z && found != all; b0 += 64) {
+      const int32_t k = b0 + lane;
+      const int64_t t = k < run.z ? tb[k] : kNoTime;  // r is active: times as stored
+      for (int q = 0; q < nq; ++q) {
+        if (found >> q & 1) continue;
+        const int64_t qs = readlane64(ws, q), qe = readlane64(we, q);
+        const uint64_t m = __ballot(t >= qs && t <= qe);
+        if (!m) continue;
+        const int64_t tf = readlane64(t, __ffsll((unsigned long long)m) - 1);
+        if (lane == q) w = 1.0f - expf(-0.1f * (float)tf);
+        found |= 1ull << q;
+      }
+    }
+    wm[(int64_t)i * 64 + lane] = w;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_seg_count(int32_t n, const uint64_t* in_off, uint32_t* nseg) {
+  for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < n; r += (int64_t)gridDim.x * kBlock) {
+    const uint64_t c = in_off[r + 1] - in_off[r];
+    nseg[r] = (uint32_t)((c + kPrbSeg - 1) / kPrbSeg);
+  }
+}
+
+// Segment s of row r: runs [beg, beg + len); rows of several segments are
+// listed in mrows (their count in *nm).
+__global__ void __launch_bounds__(kBlock) k_seg_fill(int32_t n, const uint64_t* in_off, const uint64_t* seg_off,
+                                                     int32_t* seg_row, uint64_t* seg_beg, int32_t* seg_len,
+                                                     int32_t* mrows, uint32_t* nm) {
+  for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < n; r += (int64_t)gridDim.x * kBlock) {
+    const uint64_t b = in_off[r], e = in_off[r + 1], s0 = seg_off[r], s1 = seg_off[r + 1];
+    for (uint64_t s = s0; s < s1; ++s) {
+      const uint64_t at = b + (s - s0) * kPrbSeg;
+      seg_row[s] = (int32_t)r;
+      seg_beg[s] = at;
+      seg_len[s] = (int32_t)min((uint64_t)kPrbSeg, e - at);
+    }
+    if (s1 - s0 > 1) mrows[atomicAdd(nm, 1u)] = (int32_t)r;
+  }
+}
+
+// Window degree of every vertex for every query (all window contacts, any
+// neighbor): one wave per vertex, lane q counting query q; records longer
+// than kPrbLongDeg are left to k_prb_du_long.
+__global__ void __launch_bounds__(kBlock) k_prb_du(Window A, int32_t n, const int32_t* vdeg, const uint64_t* coff,
+                                                   const int64_t* ct, const int64_t* wins, int32_t nq, int32_t* du) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); v < n; v += nw) {
+    const bool act = A.active((int32_t)v);
+    const int32_t len = act ? vdeg[v] : 0;
+    if (len > kPrbLongDeg) {
+      du[v * 64 + lane] = 0;  // counted by k_prb_du_long
+      continue;
+    }
+    int32_t c = 0;
+    const int64_t* tb = ct + (act ? coff[v] : 0);
+    for (int32_t i = 0; i < len; ++i) {
+      const int64_t t = ct_time(tb[i]);
+      c += (t >= ws && t <= we) ? 1 : 0;
+    }
+    du[v * 64 + lane] = c;
+  }
+}
+
+// Window degrees of long records: one wave per (record, kPrbDuChunk contacts)
+// item, lane q counting query q over the item's contacts, added with one
+// integer atomic per query.
+constexpr int kPrbDuChunk = 4096;
+
+__global__ void __launch_bounds__(kBlock) k_prb_du_long(const int2* items, int32_t ni, const uint64_t* coff,
+                                                        const int32_t* vdeg, const int64_t* ct, const int64_t* wins,
+                                                        int32_t nq, int32_t* du) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < ni; i += nw) {
+    const int2 it = items[i];
+    const int32_t k1 = min(vdeg[it.x], it.y + kPrbDuChunk);
+    const int64_t* tb = ct + coff[it.x];
+    int32_t c = 0;
+    for (int32_t k = it.y; k < k1; ++k) {
+      const int64_t t = ct_time(tb[k]);
+      c += (t >= ws && t <= we) ? 1 : 0;
+    }
+    if (on && c) atomicAdd(&du[(int64_t)it.x * 64 + lane], c);
+  }
+}
+
+// The (record, first contact) items of the long records.
+__global__ void __launch_bounds__(kBlock) k_prb_du_items(const int32_t* lv, int32_t nl, const int32_t* vdeg,
+                                                         int2* items, uint32_t* cnt) {
+  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < nl; i += gridDim.x * kBlock) {
+    const int32_t x = lv[i], nch = (vdeg[x] + kPrbDuChunk - 1) / kPrbDuChunk;
+    const uint32_t at = atomicAdd(cnt, (uint32_t)nch);
+    for (int32_t j = 0; j < nch; ++j) items[at + j] = int2{x, j * kPrbDuChunk};
+  }
+}
+
+// rank = 1/N for active vertices (all 64 columns), coef and base per iteration.
+__global__ void __launch_bounds__(kBlock) k_prb_init(Window A, int32_t n, float inv, float* rank) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock)
+    rank[i] = A.active((int32_t)(i >> 6)) ? inv : 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_prb_coef(Window A, int32_t n, const float* rank, const int32_t* du,
+                                                     float damping, float* coef, double base, double* acc) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const bool act = A.active((int32_t)(i >> 6));
+    coef[i] = (act && du[i] > 0) ? damping * rank[i] / (float)du[i] : 0.0f;
+    acc[i] = act ? base : 0.0;
+  }
+}
+
+// One wave per segment: lane q sums query q's terms over the segment's runs,
+// in run order.  The runs' descriptors are read 64 at a time (one per lane)
+// and handed out by readlane; the 64-wide coefficient rows are requested
+// kPrbAhead runs at a time so their loads overlap.  A single-contact run's
+// weight is f0 where its time lies in the query's window; a multi-contact
+// run's comes from this batch's k_prb_wmulti.  A row of one segment gets
+// base + sum in acc, else the sum goes to partial.
+constexpr int kPrbAhead = 8;
+
+__device__ __forceinline__ int32_t rl32(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__global__ void __launch_bounds__(kBlock) k_prb_pull(int64_t nseg, const int32_t* seg_row, const uint64_t* seg_beg,
+                                                     const int32_t* seg_len, const uint64_t* seg_off, const int4* runs,
+                                                     const float* f0, const int32_t* midx, const float* wm,
+                                                     const float* coef, const int64_t* wins, int32_t nq, double base,
+                                                     double* acc, double* partial) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t s = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); s < nseg; s += nw) {
+    const int32_t r = seg_row[s];
+    const uint64_t sb = seg_beg[s];
+    const int32_t len = seg_len[s];
+    double sum = 0;
+    for (int32_t b0 = 0; b0 < len; b0 += 64) {
+      const int32_t j = b0 + lane;
+      const int4 my = j < len ? runs[sb + j] : int4{0, 0, 0, 0};
+      const float myf = j < len ? f0[sb + j] : 0.0f;
+      const int32_t mym = j < len ? midx[sb + j] : -1;
+      const int32_t cnt = min(64, len - b0);
+      for (int32_t u0 = 0; u0 < cnt; u0 += kPrbAhead) {
+        float c[kPrbAhead];
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u)
+          c[u] = (u0 + u < cnt) ? coef[(int64_t)rl32(my.x, u0 + u) * 64 + lane] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          if (u0 + u >= cnt) break;
+          const int l = u0 + u;
+          const int32_t mi = rl32(mym, l);
+          float w;
+          bool hit;
+          if (mi >= 0) {  // several contacts: this batch's weight, per query
+            w = wm[(int64_t)mi * 64 + lane];
+            hit = w != 0.0f;
+          } else {
+            const int32_t t0 = rl32(my.w, l);
+            hit = t0 >= ws && t0 <= we;
+            w = hit ? __int_as_float(rl32(__float_as_int(myf), l)) : 0.0f;
+          }
+          if (hit) sum += (double)(c[u] * w);
+        }
+      }
+    }
+    if (seg_off[r + 1] - seg_off[r] > 1)
+      partial[s * 64 + lane] = sum;
+    else
+      acc[(int64_t)r * 64 + lane] = base + sum;
+  }
+}
+
+// Iterations after the first: rank = float(acc / sum) of the previous
+// iteration (the normalisation), then coef and base as k_prb_coef.
+__global__ void __launch_bounds__(kBlock) k_prb_step(Window A, int32_t n, const double* sums, const int32_t* du,
+                                                     float damping, float* coef, double base, double* acc) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const bool act = A.active((int32_t)(i >> 6));
+    const double s = sums[i & 63];
+    const float rank = (float)(s > 0 ? acc[i] / s : acc[i]);
+    coef[i] = (act && du[i] > 0) ? damping * rank / (float)du[i] : 0.0f;
+    acc[i] = act ? base : 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_prb_fix(const int32_t* mrows, const uint32_t* nm, const uint64_t* seg_off,
+                                                    const double* partial, double base, double* acc) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t cnt = *nm;
+  const uint32_t nw = gridDim.x * (kBlock / 64);
+  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < cnt; i += nw) {
+    const int32_t r = mrows[i];
+    double sum = 0;
+    for (uint64_t s = seg_off[r]; s < seg_off[r + 1]; ++s) sum += partial[s * 64 + lane];
+    acc[(int64_t)r * 64 + lane] = base + sum;
+  }
+}
+
+// Column sums of acc (fixed grid and order): per-block partials, then one wave.
+__global__ void __launch_bounds__(kBlock) k_prb_colsum(int32_t n, const double* acc, double* part) {
+  __shared__ double sh[kBlock / 64][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double s = 0;
+  for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + wave; v < n; v += (int64_t)gridDim.x * (kBlock / 64))
+    s += acc[v * 64 + lane];
+  sh[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += sh[w][lane];
+    part[(int64_t)blockIdx.x * 64 + lane] = t;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_prb_colsum2(const double* part, int32_t nb, double* sums) {
+  double t = 0;
+  for (int b = 0; b < nb; ++b) t += part[(int64_t)b * 64 + threadIdx.x];
+  sums[threadIdx.x] = t;
+}
+
+// Active vertices with more than kPrbLongDeg contacts (unordered).
+__global__ void __launch_bounds__(kBlock) k_long_list(Window A, int32_t n, const int32_t* vdeg, int32_t* list,
+                                                      uint32_t* cnt) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    if (A.active((int32_t)v) && vdeg[v] > kPrbLongDeg) list[atomicAdd(cnt, 1u)] = (int32_t)v;
+}
+
+// out[q][v] = rank of v for query q < nq: float(acc / sum) after the last
+// iteration (sums non-null), else the initial rank.
+__global__ void __launch_bounds__(kBlock) k_prb_out(int32_t n, int32_t nq, const double* acc, const double* sums,
+                                                    const float* rank, float* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t q = (int32_t)(i & 63);
+    if (q >= nq) continue;
+    float r = rank[i];
+    if (sums) {
+      const double s = sums[q];
+      r = (float)(s > 0 ? acc[i] / s : acc[i]);
+    }
+    out[(int64_t)q * n + (i >> 6)] = r;
   }
 }
 
@@ -778,7 +1184,6 @@ __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint3
 // List layout: item index i = code for a whole record, n + j for checkpoint
 // chunk j; off[] is the exclusive scan of the per-item counts, so the list
 // of a chunked vertex (its chunks are consecutive checkpoints) is one range.
-__device__ __forceinline__ int32_t item_index(int32_t code, int32_t n) { return code >= 0 ? code : n + (-code - 1); }
 
 struct GcListAct {
   Window W;
@@ -1680,6 +2085,24 @@ struct tga_graph {
   uint64_t *pv = nullptr, *sv = nullptr;  // (x, w): fill order, sorted by r
   int64_t* ct = nullptr;     // contact-time cache (k_ctimes), on first use
   uint64_t* coff = nullptr;  // n + 1: first contact of every record
+  // batched PageRank: run structure (once per handle) and batch state
+  int4* prb_runs = nullptr;  // sorted runs (x, k0, k1, t0)
+  float* prb_f0 = nullptr;   // weight of each run's first contact
+  int32_t *prb_midx = nullptr, *prb_mlist = nullptr;  // multi-contact runs: weight slot, list
+  int32_t prb_nmulti = 0;
+  float* prb_wm = nullptr;   // per batch: 64 weights per multi-contact run
+  uint64_t *prb_in_off = nullptr, *prb_seg_off = nullptr, *prb_seg_beg = nullptr;
+  int32_t *prb_seg_row = nullptr, *prb_seg_len = nullptr, *prb_mrows = nullptr, *prb_long = nullptr;
+  uint32_t* prb_nm = nullptr;
+  int64_t prb_nseg = 0;
+  int32_t prb_nmrows = 0, prb_nlong = 0;
+  int2* prb_du_items = nullptr;  // (long record, first contact) items of k_prb_du_long
+  int32_t prb_ndu_items = 0;
+  int32_t* prb_du = nullptr;
+  float *prb_rank = nullptr, *prb_coef = nullptr, *prb_out = nullptr;
+  double *prb_acc = nullptr, *prb_partial = nullptr, *prb_part = nullptr, *prb_sums = nullptr;
+  int64_t* prb_win = nullptr;
+  int32_t prb_out_rows = 0;
   int64_t* dcb_win = nullptr;  // batched DC windows (2 x 64)
   int32_t* dcb_out = nullptr;  // batched DC rows when the caller's output is on the host
   int32_t dcb_rows = 0;
@@ -2208,11 +2631,15 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   return TGA_OK;
 }
 
+// Scan tiles for m counts (k_scan_tiles).
+static int64_t scan_tiles(int64_t m) { return (m + 1 + kScanTile - 1) / kScanTile; }
+
 // List state (defined with the graph coloring lists below).
 static int list_budget(uint64_t need, bool held, bool* ok);
 static uint64_t item_scan_bytes(const tga_graph* g);
 static int item_scan_alloc(tga_graph* g);
-static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off);
+static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off, uint64_t* tsum = nullptr);
+static int item_scan(tga_graph* g);
 
 // Builds the contact-time cache (g->ct, g->coff) on first use; *ok is false,
 // with nothing built, when it does not fit list_budget.
@@ -2320,6 +2747,219 @@ extern "C" int tga_degree_centrality_batch(tga_graph* g, const tga_window* windo
       HIP_TRY(hipMemcpyAsync(centrality, cen, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost,
                              g->stream));
   }
+  return g->check_err();
+}
+
+// Device memory freed at scope exit (build-time temporaries).
+struct DevTmp {
+  void* p = nullptr;
+  ~DevTmp() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t get(T*& out, size_t count) {
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    out = (T*)p;
+    return e;
+  }
+};
+
+static int read_u64(tga_graph* g, const uint64_t* dev, uint64_t* host) {
+  HIP_TRY(hipMemcpyAsync(host, dev, sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return TGA_OK;
+}
+
+// Builds the batched PageRank run structure (k_runs) once per handle; *ok is
+// false, with nothing built, when it and the batch state do not fit
+// list_budget (the contact-time cache is needed too).
+static int ensure_prb(tga_graph* g, bool* ok) {
+  if (g->prb_runs) {
+    *ok = true;
+    return TGA_OK;
+  }
+  int rc = ensure_ctimes(g, ok);
+  if (rc || !*ok) return rc;
+  const int64_t n = g->n, m = n + g->nck;
+  const uint64_t cap = (uint64_t)std::max<int64_t>(g->info.contacts, 1);
+  const int end_bit = 32 - __builtin_clz((uint32_t)n | 1u);
+  size_t tmp_bytes = 0;
+  HIP_TRY(sort_pairs_u32_u64(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, (size_t)cap, end_bit,
+                             g->stream));
+  // temporaries 40 B per run + sort storage; kept: 16 B per run, 8 + 8 + 4 B
+  // per vertex, batch state 1.8 KB per vertex (64 columns), segments
+  const uint64_t need = cap * (56 + 8) + cap / 2 * 256 + tmp_bytes + (uint64_t)n * (20 + 64 * (4 + 4 + 4 + 8 + 8) + 24) +
+                        64 * 1024;
+  if ((rc = list_budget(need, false, ok)) || !*ok) return rc;
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  hipStream_t st = g->stream;
+  if ((rc = item_scan_alloc(g))) return rc;
+  k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->lcnt, m, 0u);
+  LAUNCH_STATIC(k_runs, A, g->lcnt, (const uint64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                (int32_t*)nullptr);
+  if ((rc = item_scan(g))) return rc;
+  uint64_t nr = 0;
+  if ((rc = read_u64(g, g->loff + m, &nr))) return rc;
+  if (nr > cap) return fail(TGA_EFORMAT, "neighbor runs exceed the contact count");
+  DevTmp t_rx, t_rr, t_rk, t_k1, t_keys, t_vals, t_sk, t_sv, t_sort, t_nseg;
+  int32_t *rx, *rr, *rk, *k1;
+  uint32_t *keys, *sk, *nseg;
+  uint64_t *vals, *sv;
+  uint8_t* sort_tmp;
+  HIP_TRY(t_rx.get(rx, nr));
+  HIP_TRY(t_rr.get(rr, nr));
+  HIP_TRY(t_rk.get(rk, nr));
+  HIP_TRY(t_k1.get(k1, nr));
+  HIP_TRY(t_keys.get(keys, nr));
+  HIP_TRY(t_vals.get(vals, nr));
+  HIP_TRY(t_sk.get(sk, nr));
+  HIP_TRY(t_sv.get(sv, nr));
+  HIP_TRY(t_sort.get(sort_tmp, tmp_bytes));
+  HIP_TRY(t_nseg.get(nseg, (size_t)n));
+  LAUNCH_STATIC(k_runs, A, (uint32_t*)nullptr, g->loff, rx, rr, rk);
+  k_run_keys<<<g->grid((int64_t)nr), kBlock, 0, st>>>((int64_t)nr, rx, rr, rk, g->vdeg, A, (int32_t)n, k1, keys,
+                                                      vals);
+  HIP_TRY(hipGetLastError());
+  size_t tb = tmp_bytes;
+  HIP_TRY(sort_pairs_u32_u64(sort_tmp, tb, keys, sk, vals, sv, (size_t)nr, end_bit, st));
+  if ((rc = g->alloc(g->prb_in_off, (size_t)n + 1)) || (rc = g->alloc(g->prb_seg_off, (size_t)n + 1))) return rc;
+  k_key_bounds<<<g->grid(n + 1), kBlock, 0, st>>>(sk, nr, (int32_t)n, g->prb_in_off);
+  HIP_TRY(hipGetLastError());
+  uint64_t nv = 0;
+  if ((rc = read_u64(g, g->prb_in_off + n, &nv))) return rc;
+  int4* runs = nullptr;
+  if ((rc = g->alloc(runs, (size_t)nv)) || (rc = g->alloc(g->prb_f0, (size_t)nv))) return rc;
+  k_run_gather<<<g->grid((int64_t)nv), kBlock, 0, st>>>(nv, sv, rx, rk, k1, g->coff, g->ct, runs, g->prb_f0);
+  {
+    DevTmp t_flag, t_moff, t_mts;
+    uint32_t* flag;
+    uint64_t *moff, *mts;
+    HIP_TRY(t_flag.get(flag, nv));
+    HIP_TRY(t_moff.get(moff, nv + 1));
+    HIP_TRY(t_mts.get(mts, (size_t)scan_tiles((int64_t)nv)));
+    k_multi_flag<<<g->grid((int64_t)nv), kBlock, 0, st>>>(nv, runs, flag);
+    HIP_TRY(hipGetLastError());
+    if ((rc = scan_counts(g, flag, (int64_t)nv, moff, mts))) return rc;
+    uint64_t nmulti = 0;
+    if ((rc = read_u64(g, moff + nv, &nmulti))) return rc;
+    if ((rc = g->alloc(g->prb_midx, (size_t)nv)) || (rc = g->alloc(g->prb_mlist, (size_t)nmulti))) return rc;
+    k_multi_list<<<g->grid((int64_t)nv), kBlock, 0, st>>>(nv, runs, moff, g->prb_midx, g->prb_mlist);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));  // before the temporaries go
+    g->prb_nmulti = (int32_t)nmulti;
+  }
+  k_seg_count<<<g->grid(n), kBlock, 0, st>>>((int32_t)n, g->prb_in_off, nseg);
+  HIP_TRY(hipGetLastError());
+  if ((rc = scan_counts(g, nseg, n, g->prb_seg_off))) return rc;
+  uint64_t ns = 0;
+  if ((rc = read_u64(g, g->prb_seg_off + n, &ns))) return rc;
+  if ((rc = g->alloc(g->prb_seg_row, (size_t)ns)) || (rc = g->alloc(g->prb_seg_beg, (size_t)ns)) ||
+      (rc = g->alloc(g->prb_seg_len, (size_t)ns)) || (rc = g->alloc(g->prb_mrows, (size_t)n)) ||
+      (rc = g->alloc(g->prb_nm, 2)) || (rc = g->alloc(g->prb_long, (size_t)n)))
+    return rc;
+  k_fill<uint32_t><<<1, kBlock, 0, st>>>(g->prb_nm, 2, 0u);
+  k_seg_fill<<<g->grid(n), kBlock, 0, st>>>((int32_t)n, g->prb_in_off, g->prb_seg_off, g->prb_seg_row,
+                                            g->prb_seg_beg, g->prb_seg_len, g->prb_mrows, g->prb_nm);
+  k_long_list<<<g->grid(n), kBlock, 0, st>>>(A, (int32_t)n, g->vdeg, g->prb_long, g->prb_nm + 1);
+  HIP_TRY(hipGetLastError());
+  uint32_t hn[2] = {0, 0};
+  if ((rc = g->read_counts(g->prb_nm, hn, 2))) return rc;
+  g->prb_nseg = (int64_t)ns;
+  g->prb_nmrows = (int32_t)hn[0];
+  g->prb_nlong = (int32_t)hn[1];
+  if (g->prb_nlong) {
+    // items: at most contacts / kPrbDuChunk + one per long record
+    const size_t cap_items = (size_t)(cap / kPrbDuChunk) + (size_t)g->prb_nlong;
+    if ((rc = g->alloc(g->prb_du_items, cap_items))) return rc;
+    k_fill<uint32_t><<<1, kBlock, 0, st>>>(g->prb_nm + 1, 1, 0u);  // slot 1 is free once read
+    k_prb_du_items<<<g->grid(g->prb_nlong), kBlock, 0, st>>>(g->prb_long, g->prb_nlong, g->vdeg, g->prb_du_items,
+                                                             g->prb_nm + 1);
+    HIP_TRY(hipGetLastError());
+    uint32_t ni = 0;
+    if ((rc = g->read_counts(g->prb_nm + 1, &ni, 1))) return rc;
+    g->prb_ndu_items = (int32_t)ni;
+  }
+  g->prb_runs = runs;  // built
+  return TGA_OK;
+}
+
+extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t iters, float damping,
+                                  float* rank, uint32_t flags) {
+  if (!g || !windows || !rank || nq < 1 || nq > 64 || iters < 0)
+    return fail(TGA_EINVAL, "tga_pagerank_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q])) return fail(TGA_EINVAL, "tga_pagerank_batch: window start > end");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  const int64_t N = g->ae - g->ab;
+  if (N <= 0) return fail(TGA_EINVAL, "tga_pagerank_batch: no active vertices");
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) {
+    // No room for the run structure: one call per window.
+    for (int32_t q = 0; q < nq; ++q)
+      if ((rc = tga_pagerank(g, windows[q], iters, damping, 0.0f, rank + q * n, nullptr, flags))) return rc;
+    return TGA_OK;
+  }
+  if (!g->prb_du) {
+    if ((rc = g->alloc(g->prb_du, (size_t)n * 64)) || (rc = g->alloc(g->prb_rank, (size_t)n * 64)) ||
+        (rc = g->alloc(g->prb_coef, (size_t)n * 64)) || (rc = g->alloc(g->prb_acc, (size_t)n * 64)) ||
+        (rc = g->alloc(g->prb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
+        (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64)) || (rc = g->alloc(g->prb_sums, 64)) ||
+        (rc = g->alloc(g->prb_win, 2 * 64)) || (rc = g->alloc(g->prb_wm, (size_t)g->prb_nmulti * 64)))
+      return rc;
+  }
+  int64_t hw[2 * 64];
+  for (int32_t q = 0; q < nq; ++q) {
+    hw[2 * q] = windows[q].start;
+    hw[2 * q + 1] = windows[q].end;
+  }
+  hipStream_t st = g->stream;
+  HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
+  float* out = rank;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if (g->prb_out_rows < nq) {
+      if ((rc = g->alloc(g->prb_out, (size_t)nq * (size_t)n))) return rc;
+      g->prb_out_rows = nq;
+    }
+    out = g->prb_out;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  const float inv = 1.0f / (float)N;
+  const float base = (1.0f - damping) / (float)N;
+  g->kbegin();
+  k_prb_du<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->vdeg, g->coff, g->ct, g->prb_win, nq, g->prb_du);
+  if (g->prb_ndu_items)
+    k_prb_du_long<<<g->grid((int64_t)g->prb_ndu_items * 64), kBlock, 0, st>>>(
+        g->prb_du_items, g->prb_ndu_items, g->coff, g->vdeg, g->ct, g->prb_win, nq, g->prb_du);
+  if (g->prb_nmulti)
+    k_prb_wmulti<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
+                                                                         g->coff, g->ct, g->prb_win, nq, g->prb_wm);
+  k_prb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, inv, g->prb_rank);
+  for (int32_t it = 0; it < iters; ++it) {
+    if (it == 0)
+      k_prb_coef<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->prb_rank, g->prb_du, damping, g->prb_coef,
+                                                     (double)base, g->prb_acc);
+    else
+      k_prb_step<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->prb_sums, g->prb_du, damping, g->prb_coef,
+                                                     (double)base, g->prb_acc);
+    if (g->prb_nseg)
+      k_prb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+          g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_f0,
+          g->prb_midx, g->prb_wm, g->prb_coef, g->prb_win, nq, (double)base, g->prb_acc, g->prb_partial);
+    if (g->prb_nmrows)
+      k_prb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(g->prb_mrows, g->prb_nm, g->prb_seg_off,
+                                                                       g->prb_partial, (double)base, g->prb_acc);
+    k_prb_colsum<<<kPrbRedBlocks, kBlock, 0, st>>>((int32_t)n, g->prb_acc, g->prb_part);
+    k_prb_colsum2<<<1, 64, 0, st>>>(g->prb_part, kPrbRedBlocks, g->prb_sums);
+  }
+  k_prb_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->prb_acc, iters > 0 ? g->prb_sums : nullptr,
+                                                g->prb_rank, out);
+  g->kend();
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE))
+    HIP_TRY(hipMemcpyAsync(rank, out, sizeof(float) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
   return g->check_err();
 }
 
@@ -2640,16 +3280,16 @@ static int list_budget(uint64_t need, bool held, bool* ok) {
   return TGA_OK;
 }
 
-static int64_t scan_tiles(int64_t m) { return (m + 1 + kScanTile - 1) / kScanTile; }
 
-// Exclusive scan of c[0, m) into off[0, m] (m <= n + nck; the tile sums
-// use g->ltsum, see item_scan_alloc).
-static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off) {
+// Exclusive scan of c[0, m) into off[0, m]; the tile sums go to tsum
+// (scan_tiles(m) entries), or to g->ltsum when m <= n + nck (item_scan_alloc).
+static int scan_counts(tga_graph* g, const uint32_t* c, int64_t m, uint64_t* off, uint64_t* tsum) {
   const int64_t tiles = scan_tiles(m);
+  if (!tsum) tsum = g->ltsum;
   hipStream_t st = g->stream;
-  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(c, m, g->ltsum);
-  k_scan_top<<<1, kBlock, 0, st>>>(g->ltsum, tiles);
-  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(c, m, g->ltsum, off);
+  k_scan_tiles<<<(int)tiles, kBlock, 0, st>>>(c, m, tsum);
+  k_scan_top<<<1, kBlock, 0, st>>>(tsum, tiles);
+  k_scan_apply<<<(int)tiles, kBlock, 0, st>>>(c, m, tsum, off);
   HIP_TRY(hipGetLastError());
   return TGA_OK;
 }

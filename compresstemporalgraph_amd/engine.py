@@ -81,6 +81,7 @@ def lib():
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
         L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
         L.tga_pagerank.argtypes = [vp, Window, C.c_int32, C.c_float, C.c_float, vp, p32, C.c_uint32]
+        L.tga_pagerank_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, C.c_float, vp, C.c_uint32]
         L.tga_hits.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
         L.tga_profile.argtypes = [vp, C.c_int]
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
@@ -258,6 +259,21 @@ class TemporalGraph:
         _check(lib().tga_pagerank(self.h, _win(window), iters, damping, tol, o.ptr, C.byref(it), o.flags))
         self.last_iterations = it.value
         return o.arr
+
+    def pagerank_batch(self, windows, iters: int = 20, damping: float = 0.85, out=None):
+        """Batched PageRank: up to 64 windows advanced together (FP32 columns,
+        fixed iterations).  Returns an (nq, n) float32 array whose row q equals
+        pagerank(windows[q], iters, damping) up to FP64 summation order; `out`
+        may be a caller's (nq, n) device tensor instead."""
+        nq = len(windows)
+        wins = (Window * nq)(*[_win(w) for w in windows])
+        if out is None:
+            arr = np.empty((nq, self.n), np.float32)
+            ptr, flags = arr.ctypes.data_as(C.c_void_p), 0
+        else:
+            arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
+        _check(lib().tga_pagerank_batch(self.h, wins, nq, iters, damping, ptr, flags))
+        return arr
 
     def hits(self, window=None, iters: int = 20, out_hub=None, out_auth=None):
         h = _Out(self.n, np.float32, out_hub)

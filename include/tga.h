@@ -185,6 +185,17 @@ int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, const int32_t
 int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float damping, float tol, float* rank,
                  int32_t* iters_run, uint32_t flags);
 
+/* Batched PageRank: nq (1..64) windows advanced together, `iters` iterations
+ * each (no tolerance stop).  rank receives nq rows of n: row q equals
+ * tga_pagerank(g, windows[q], iters, damping, 0, row, ...) up to FP64
+ * summation order.  The queries are FP32 columns over a window-independent
+ * structure of neighbor runs built once per handle (radix-sorted by
+ * destination; with the contact-time cache, under TGA_LIST_MAX_BYTES), so an
+ * iteration reads every run once for all queries, with no atomics.  No
+ * counterpart in the reference (one window per executable run). */
+int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t iters, float damping, float* rank,
+                       uint32_t flags);
+
 /* HITS with per-iteration reset and L2 normalisation over distinct active
  * neighbors with a window contact; FP32 storage, FP64 sums
  * (BaselineRunner.java:472-550; replaces hits.cu:1898-1987, :2284-2349). */

@@ -223,6 +223,42 @@ def test_pagerank(pair, path, monkeypatch):
         assert rel_linf(got, ref64) <= 1e-4
 
 
+def test_pagerank_batch(pair):
+    """Up to 64 windows advanced together: every row within the PageRank bar
+    of the oracle's single-window FP32 result (and of the FP64 oracle), and
+    equal to the single-window engine result up to FP64 summation order."""
+    ds, g = pair
+    rng = np.random.default_rng(17)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    wins = []
+    for q in range(64):
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))
+    got = g.pagerank_batch(wins, iters=20)
+    o = ds.oracle()
+    for q, w in enumerate(wins):
+        ref32, _ = o.pagerank(20, window=w, dtype=np.float32)
+        assert rel_linf(got[q], ref32) <= REL_TOL, f"window {q}"
+        if q < 8:
+            ref64, _ = o.pagerank(20, window=w, dtype=np.float64)
+            assert rel_linf(got[q], ref64) <= 1e-4, f"window {q}"
+            assert rel_linf(got[q], g.pagerank(w, iters=20)) <= 1e-6, f"window {q}"
+    # a smaller batch on the same handle
+    got2 = g.pagerank_batch(wins[1:3], iters=5)
+    for j, w in enumerate(wins[1:3]):
+        ref32, _ = o.pagerank(5, window=w, dtype=np.float32)
+        assert rel_linf(got2[j], ref32) <= REL_TOL
+
+
 @PATHS
 def test_pagerank_tol(pair, path, monkeypatch):
     ds, g = pair
