@@ -1,5 +1,5 @@
-"""Serving form: BFS and degree-centrality (DC) window queries arriving over
-time, served by continuous batching on one GPU.
+"""Serving form: BFS, CC, PageRank (PR) and degree-centrality (DC) window
+queries arriving over time, served by continuous batching on one GPU.
 
 Requests are (arrival time, kind, source, window).  The loop admits every BFS
 request that has arrived into a free lane of the BFS server (up to 64 at
@@ -7,10 +7,12 @@ once, fewer when the per-query state would exceed the memory budget),
 advances all running queries by one BFS level per decode pass, and completes
 a query as soon as its frontier empties -- so a query that arrives while
 others are on the GPU joins them at the next level instead of waiting for the
-batch to drain.  DC requests that have arrived are answered together by one
-batched call (up to 64 windows) at the next loop step.  The plain baseline
-serves the same arrivals one query at a time (TemporalGraph.bfs,
-TemporalGraph.degree_centrality), in arrival order.
+batch to drain.  DC and PR requests that have arrived are answered together
+by one batched call per kind (up to 64 windows; PR 20 iterations); CC
+requests one at a time.  With a second handle these run in a worker thread
+beside the BFS loop, else between its steps.  The plain baseline serves the
+same arrivals one query at a time (TemporalGraph.bfs, connected_components,
+pagerank, degree_centrality), in arrival order.
 
 Latency is completion time minus arrival time; throughput is completed
 queries over the span from the first arrival to the last completion.
@@ -30,7 +32,7 @@ class Request:
     arrival: float  # seconds after the start of the run
     src: int
     window: tuple
-    kind: str = "BFS"  # "BFS" or "DC"
+    kind: str = "BFS"  # "BFS", "CC", "PR" or "DC"
 
 
 def poisson_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, seed: int = 0) -> list:
@@ -45,13 +47,20 @@ def poisson_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, seed
 
 
 def mixed_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, dc_share: float = 0.2,
-                   seed: int = 0) -> list:
-    """poisson_requests with a share of DC requests (kind "DC"; src unused)."""
+                   seed: int = 0, shares: dict = None) -> list:
+    """poisson_requests with shares of other kinds: `shares` maps "DC", "PR"
+    and "CC" to fractions (the rest stays BFS); without it, dc_share of DC.
+    src is used by BFS only."""
     req = poisson_requests(rate, count, n, t_lo, t_hi, seed)
+    shares = shares if shares is not None else {"DC": dc_share}
     rng = np.random.default_rng(seed + 1)
     for r, u in zip(req, rng.random(count)):
-        if u < dc_share:
-            r.kind = "DC"
+        acc = 0.0
+        for kind in ("DC", "PR", "CC"):
+            acc += shares.get(kind, 0.0)
+            if u < acc:
+                r.kind = kind
+                break
     return req
 
 
@@ -88,6 +97,9 @@ def _sink(g: TemporalGraph, device_out: bool, rows: int = 0):
     return lambda: torch.empty(shape, dtype=torch.int32, device=f"cuda:{g.device}")
 
 
+PR_ITERS = 20
+
+
 def _dc_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
     """One batched DC call for requests idx; returns [(count, centrality)] rows."""
     wins = [req[i].window for i in idx]
@@ -99,12 +111,69 @@ def _dc_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
     return [(cnt[j], cen[j]) for j in range(len(idx))]
 
 
-class _DcWorker:
-    """Answers DC requests on a second handle of the same graph, in a thread
-    of its own: each wake-up takes every pending request (up to 64) into one
-    batched call.  The handle has its own stream, so these batches overlap
-    the BFS levels of the main loop on the device (the engine's calls release
-    the interpreter lock while they wait for the GPU)."""
+def _pr_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
+    """One batched PageRank call for requests idx; returns its rows."""
+    wins = [req[i].window for i in idx]
+    if device_out:
+        import torch
+        out = torch.empty((len(idx), g.n), dtype=torch.float32, device=f"cuda:{g.device}")
+        rank = g.pagerank_batch(wins, iters=PR_ITERS, out=out)
+    else:
+        rank = g.pagerank_batch(wins, iters=PR_ITERS)
+    return [rank[j] for j in range(len(idx))]
+
+
+def _cc_one(g: TemporalGraph, r, device_out: bool):
+    return g.connected_components(r.window, out=_sink(g, device_out)())
+
+
+def _plain_one(g: TemporalGraph, r, device_out: bool):
+    """One request answered by the plain entry point of its kind."""
+    sink = _sink(g, device_out)
+    if r.kind == "DC":
+        return g.degree_centrality(r.window, centrality=True, out=sink(), out_centrality=sink())
+    if r.kind == "PR":
+        out = None
+        if device_out:
+            import torch
+            out = torch.empty(g.n, dtype=torch.float32, device=f"cuda:{g.device}")
+        return g.pagerank(r.window, iters=PR_ITERS, out=out)
+    if r.kind == "CC":
+        return g.connected_components(r.window, out=sink())
+    return g.bfs(r.src, r.window, out=sink())
+
+
+# Fewest pending queries of a kind worth one batched call: a batched call
+# costs about the same whatever its width (one pass over the graph for 64
+# columns), so a few PR queries (a 64-wide batch costs about as much as 16-24
+# windowed plain ones) or one DC query run as plain calls.
+MIN_BATCH = {"DC": 2, "PR": 16}
+
+
+def _answer(g: TemporalGraph, req: list, idx: list, device_out: bool) -> list:
+    """Results for the non-BFS requests idx: DC and PR batched per kind (up
+    to 64 each; plain calls below MIN_BATCH), CC one at a time; returns
+    [(i, result)]."""
+    out = []
+    for kind, fn in (("DC", _dc_batch), ("PR", _pr_batch)):
+        sel = [i for i in idx if req[i].kind == kind]
+        for b in range(0, len(sel), 64):
+            part = sel[b:b + 64]
+            if len(part) >= MIN_BATCH[kind]:
+                out += list(zip(part, fn(g, req, part, device_out)))
+            else:
+                out += [(i, _plain_one(g, req[i], device_out)) for i in part]
+    out += [(i, _cc_one(g, req[i], device_out)) for i in idx if req[i].kind == "CC"]
+    return out
+
+
+class _BatchWorker:
+    """Answers the non-BFS requests on a second handle of the same graph, in
+    a thread of its own: each wake-up takes every pending request (DC and PR
+    batched per kind, CC one at a time).  The handle has its own stream, so
+    this work overlaps the BFS levels of the main loop on the device (the
+    engine's calls release the interpreter lock while they wait for the
+    GPU)."""
 
     def __init__(self, g: TemporalGraph, req: list, device_out: bool, t0: float, done_at: list, keep):
         import threading
@@ -140,16 +209,16 @@ class _DcWorker:
                     self.cv.wait()
                 if not self.pending:
                     return
-                idx, self.pending = self.pending[:64], self.pending[64:]
+                idx, self.pending = self.pending, []
             try:
-                rows = _dc_batch(self.g, self.req, idx, self.device_out)
+                res = _answer(self.g, self.req, idx, self.device_out)
             except Exception as e:  # surfaced by busy()
                 with self.cv:
                     self.error, self.outstanding = e, 0
                 return
             t = time.perf_counter() - self.t0
             with self.cv:
-                for i, row in zip(idx, rows):
+                for i, row in res:
                     self.done_at[i] = t
                     if self.keep is not None:
                         self.keep[i] = row
@@ -157,11 +226,13 @@ class _DcWorker:
 
 
 def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep=None,
-                  device_out: bool = False, dc_graph: TemporalGraph = None) -> dict:
+                  device_out: bool = False, dc_graph: TemporalGraph = None,
+                  cc_graph: TemporalGraph = None) -> dict:
     """Continuous batching.  `keep` (a dict) receives the results per request
     index when given (for identity checks).  With `dc_graph` (a second handle
-    of the same graph), DC batches run beside the BFS loop in a worker thread
-    instead of between its steps."""
+    of the same graph), the DC / PR batches and CC queries run beside the BFS
+    loop in a worker thread instead of between its steps; with `cc_graph` too
+    (a third handle), CC queries get a worker of their own."""
     lanes = lanes_for_budget(g.n, budget_bytes)
     sink = _sink(g, device_out)
     done_at = [0.0] * len(req)
@@ -170,14 +241,18 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
         lane_of, dc_wait = {}, []
         nxt = 0
         t0 = time.perf_counter()
-        worker = _DcWorker(dc_graph, req, device_out, t0, done_at, keep) if dc_graph is not None else None
+        worker = _BatchWorker(dc_graph, req, device_out, t0, done_at, keep) if dc_graph is not None else None
+        ccw = _BatchWorker(cc_graph, req, device_out, t0, done_at, keep) if worker and cc_graph is not None else None
         try:
-            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or (worker and worker.busy())):
+            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or (worker and worker.busy()) or
+                                    (ccw and ccw.busy())):
                 now = time.perf_counter() - t0
                 while nxt < len(order) and req[order[nxt]].arrival <= now:
                     i = order[nxt]
-                    if req[i].kind == "DC":
-                        if worker:
+                    if req[i].kind != "BFS":
+                        if ccw and req[i].kind == "CC":
+                            ccw.put(i)
+                        elif worker:
                             worker.put(i)
                         else:
                             dc_wait.append(i)
@@ -191,10 +266,10 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
                     time.sleep(max(0.0, min(wake, 1e-3) if worker else wake))
                     continue
                 if dc_wait:
-                    idx, dc_wait = dc_wait[:64], dc_wait[64:]
-                    rows = _dc_batch(g, req, idx, device_out)
+                    idx, dc_wait = dc_wait, []
+                    res = _answer(g, req, idx, device_out)
                     t = time.perf_counter() - t0
-                    for i, row in zip(idx, rows):
+                    for i, row in res:
                         done_at[i] = t
                         if keep is not None:
                             keep[i] = row
@@ -208,6 +283,8 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
         finally:
             if worker:
                 worker.close()
+            if ccw:
+                ccw.close()
     out = _stats(req, done_at)
     out["lanes"] = lanes
     return out
@@ -215,7 +292,6 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
 
 def serve_plain(g: TemporalGraph, req: list, keep=None, device_out: bool = False) -> dict:
     """One query at a time, in arrival order."""
-    sink = _sink(g, device_out)
     done_at = [0.0] * len(req)
     order = sorted(range(len(req)), key=lambda i: req[i].arrival)
     t0 = time.perf_counter()
@@ -223,10 +299,7 @@ def serve_plain(g: TemporalGraph, req: list, keep=None, device_out: bool = False
         now = time.perf_counter() - t0
         if req[i].arrival > now:
             time.sleep(req[i].arrival - now)
-        if req[i].kind == "DC":
-            res = g.degree_centrality(req[i].window, centrality=True, out=sink(), out_centrality=sink())
-        else:
-            res = g.bfs(req[i].src, req[i].window, out=sink())
+        res = _plain_one(g, req[i], device_out)
         done_at[i] = time.perf_counter() - t0
         if keep is not None:
             keep[i] = res

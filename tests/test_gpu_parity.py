@@ -282,14 +282,15 @@ def test_hits(pair, path, monkeypatch):
 
 @pytest.mark.parametrize("worker", [False, True])
 def test_served_mix(pair, worker):
-    """A BFS + DC request stream through the serving loop (BFS in server
-    lanes, DC in batched calls, between the BFS steps or on a second handle
-    in a worker thread): every result equals the oracle's answer to that
-    query alone."""
+    """A BFS / CC / PR / DC request stream through the serving loop (BFS in
+    server lanes, DC and PR in batched calls, CC one by one; between the BFS
+    steps or on a second handle in a worker thread): every result equals the
+    oracle's answer to that query alone (PageRank within its bar)."""
     from compresstemporalgraph_amd import serving
     ds, g = pair
     lo, hi = int(ds.t.min()), int(ds.t.max())
-    req = serving.mixed_requests(rate=2e4, count=80, n=ds.n, t_lo=lo, t_hi=hi + 1, dc_share=0.3, seed=13)
+    req = serving.mixed_requests(rate=2e4, count=80, n=ds.n, t_lo=lo, t_hi=hi + 1, seed=13,
+                                 shares={"DC": 0.15, "PR": 0.25, "CC": 0.2})
     for r in req:  # sources with contacts, windows clipped to the span
         r.src = int(ds.src[r.src % len(ds.src)])
         r.window = (r.window[0], min(r.window[1], hi))
@@ -303,9 +304,15 @@ def test_served_mix(pair, worker):
     o = ds.oracle()
     assert sorted(keep) == list(range(len(req)))
     for i, r in enumerate(req):
+        msg = f"request {i} ({r.kind})"
         if r.kind == "DC":
             ocnt, ocen = o.degree(window=r.window, centrality=True)
-            np.testing.assert_array_equal(keep[i][0], ocnt, err_msg=f"request {i} (DC)")
-            np.testing.assert_array_equal(keep[i][1], ocen, err_msg=f"request {i} (DC)")
+            np.testing.assert_array_equal(keep[i][0], ocnt, err_msg=msg)
+            np.testing.assert_array_equal(keep[i][1], ocen, err_msg=msg)
+        elif r.kind == "PR":
+            ref32, _ = o.pagerank(serving.PR_ITERS, window=r.window, dtype=np.float32)
+            assert rel_linf(keep[i], ref32) <= REL_TOL, msg
+        elif r.kind == "CC":
+            np.testing.assert_array_equal(keep[i], o.cc(window=r.window), err_msg=msg)
         else:
-            np.testing.assert_array_equal(keep[i], o.bfs(r.src, window=r.window), err_msg=f"request {i} (BFS)")
+            np.testing.assert_array_equal(keep[i], o.bfs(r.src, window=r.window), err_msg=msg)

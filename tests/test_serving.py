@@ -110,6 +110,17 @@ class _FakeGraph:
     def degree_centrality(self, window=None, centrality=False, out=None, out_centrality=None):
         return ("cnt", window), ("cen", window)
 
+    def pagerank_batch(self, wins, iters=20, out=None):
+        assert 1 <= len(wins) <= 64 and iters == serving.PR_ITERS
+        self.pr_calls = getattr(self, "pr_calls", []) + [list(wins)]
+        return [("pr", w) for w in wins]
+
+    def pagerank(self, window=None, iters=20, out=None):
+        return ("pr", window)
+
+    def connected_components(self, window=None, out=None):
+        return ("cc", window)
+
 
 def test_serve_loop_bookkeeping():
     """Every request is answered exactly once with its own result, in both
@@ -148,3 +159,31 @@ def test_serve_loop_dc_worker():
     assert not g.dc_calls
     assert sum(len(c) for c in g2.dc_calls) == sum(r.kind == "DC" for r in req)
     assert max(len(c) for c in g2.dc_calls) <= 64
+
+
+def test_mixed_requests_four_kinds():
+    mix = serving.mixed_requests(1e4, 4000, 100, 0, 1 << 24, seed=8,
+                                 shares={"DC": 0.1, "PR": 0.3, "CC": 0.2})
+    kinds = [r.kind for r in mix]
+    for k, share in (("BFS", 0.4), ("CC", 0.2), ("PR", 0.3), ("DC", 0.1)):
+        assert abs(kinds.count(k) / len(kinds) - share) < 0.03, k
+
+
+def test_serve_loop_all_kinds():
+    """BFS, CC, PR and DC requests, inline and with the worker: each answered
+    once with its own result; PR and DC in batched calls of at most 64."""
+    req = serving.mixed_requests(rate=1e5, count=400, n=1000, t_lo=0, t_hi=1 << 20, seed=6,
+                                 shares={"DC": 0.1, "PR": 0.3, "CC": 0.2})
+    want = {"BFS": lambda r: ("bfs", r.src, r.window), "CC": lambda r: ("cc", r.window),
+            "PR": lambda r: ("pr", r.window), "DC": lambda r: (("cnt", r.window), ("cen", r.window))}
+    for workers in (0, 1, 2):
+        g, g2, g3 = _FakeGraph(), _FakeGraph(), _FakeGraph()
+        kb, kp = {}, {}
+        serving.serve_batched(g, req, keep=kb, dc_graph=g2 if workers else None,
+                              cc_graph=g3 if workers == 2 else None)
+        serving.serve_plain(g, req, keep=kp)
+        for i, r in enumerate(req):
+            assert kb[i] == want[r.kind](r) == kp[i], (workers, i, r.kind)
+        gb = g2 if workers else g
+        calls = getattr(gb, "pr_calls", [])
+        assert all(serving.MIN_BATCH["PR"] <= len(c) <= 64 for c in calls)
