@@ -1019,6 +1019,134 @@ __global__ void __launch_bounds__(64) k_prb_colsum2(const double* part, int32_t 
   sums[threadIdx.x] = t;
 }
 
+// --------------------------------------------- batched connected components ---
+//
+// Up to 64 min-label queries (windows) over the same run structure, with
+// int32 label columns label[v][q] (active v starts at v, inactive at -1).  A
+// run x -> r links x to r in query q when one of its contacts lies in window
+// q: for a single-contact run its time t0, for a multi-contact run this
+// batch's bit mask (k_ccb_mask).  A round is one wave per segment: lane q
+// takes the least label[x][q] over the linked runs and lowers label[r][q] to
+// it.  Labels are updated in place -- they only fall, and every value read is
+// the label of a vertex that reaches r, so the fixpoint (the least active id
+// reaching each vertex) is the one of the plain rounds.  Rows of several
+// segments go through partial minima and k_ccb_fix.  Rounds repeat until none
+// lowers a label.
+__global__ void __launch_bounds__(kBlock) k_ccb_mask(const int32_t* mlist, int32_t nm, const int4* runs,
+                                                     const uint64_t* coff, const int64_t* ct, const int64_t* wins,
+                                                     int32_t nq, uint64_t* mask) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const uint64_t all = nq >= 64 ? ~0ull : ((1ull << nq) - 1);
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nm; i += nw) {
+    const int4 run = runs[mlist[i]];
+    const int64_t* tb = ct + coff[run.x];
+    uint64_t bits = 0;
+    for (int32_t b0 = run.y; b0 < run.z && bits != all; b0 += 64) {
+      const int32_t k = b0 + lane;
+      const int64_t t = k < run.z ? tb[k] : kNoTime;
+      for (int q = 0; q < nq; ++q) {
+        if (bits >> q & 1) continue;
+        const int64_t qs = readlane64(ws, q), qe = readlane64(we, q);
+        if (__ballot(t >= qs && t <= qe)) bits |= 1ull << q;
+      }
+    }
+    if (lane == 0) mask[i] = bits;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ccb_init(Window A, int32_t n, int32_t* label) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t v = (int32_t)(i >> 6);
+    label[i] = A.active(v) ? v : -1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ccb_pull(int64_t nseg, const int32_t* seg_row, const uint64_t* seg_beg,
+                                                     const int32_t* seg_len, const uint64_t* seg_off, const int4* runs,
+                                                     const int32_t* midx, const uint64_t* mask, const int64_t* wins,
+                                                     int32_t nq, int32_t* label, int32_t* partial, int32_t* changed) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t s = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); s < nseg; s += nw) {
+    const int32_t r = seg_row[s];
+    const uint64_t sb = seg_beg[s];
+    const int32_t len = seg_len[s];
+    int32_t cur = INT32_MAX;
+    for (int32_t b0 = 0; b0 < len; b0 += 64) {
+      const int32_t j = b0 + lane;
+      const int4 my = j < len ? runs[sb + j] : int4{0, 0, 0, 0};
+      const int32_t mym = j < len ? midx[sb + j] : -1;
+      const int32_t cnt = min(64, len - b0);
+      for (int32_t u0 = 0; u0 < cnt; u0 += kPrbAhead) {
+        int32_t lab[kPrbAhead];
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u)
+          lab[u] = (u0 + u < cnt) ? label[(int64_t)rl32(my.x, u0 + u) * 64 + lane] : INT32_MAX;
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          if (u0 + u >= cnt) break;
+          const int l = u0 + u;
+          const int32_t mi = rl32(mym, l);
+          bool hit;
+          if (mi >= 0) {
+            hit = mask[mi] >> lane & 1;
+          } else {
+            const int32_t t0 = rl32(my.w, l);
+            hit = t0 >= ws && t0 <= we;
+          }
+          if (hit) cur = min(cur, lab[u]);
+        }
+      }
+    }
+    bool lowered = false;
+    if (seg_off[r + 1] - seg_off[r] > 1) {
+      partial[s * 64 + lane] = cur;
+    } else if (on) {
+      int32_t* p = label + (int64_t)r * 64 + lane;
+      if (cur < *p) {
+        *p = cur;
+        lowered = true;
+      }
+    }
+    if (__ballot(lowered) && lane == 0) *changed = 1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ccb_fix(const int32_t* mrows, const uint32_t* nm, const uint64_t* seg_off,
+                                                    const int32_t* partial, int32_t nq, int32_t* label,
+                                                    int32_t* changed) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t cnt = *nm;
+  const uint32_t nw = gridDim.x * (kBlock / 64);
+  for (uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < cnt; i += nw) {
+    const int32_t r = mrows[i];
+    int32_t cur = INT32_MAX;
+    for (uint64_t s = seg_off[r]; s < seg_off[r + 1]; ++s) cur = min(cur, partial[s * 64 + lane]);
+    bool lowered = false;
+    if (lane < nq) {
+      int32_t* p = label + (int64_t)r * 64 + lane;
+      if (cur < *p) {
+        *p = cur;
+        lowered = true;
+      }
+    }
+    if (__ballot(lowered) && lane == 0) *changed = 1;
+  }
+}
+
+// out[q][v] = label[v][q] for q < nq.
+__global__ void __launch_bounds__(kBlock) k_ccb_out(int32_t n, int32_t nq, const int32_t* label, int32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t q = (int32_t)(i & 63);
+    if (q < nq) out[(int64_t)q * n + (i >> 6)] = label[i];
+  }
+}
+
 // Active vertices with more than kPrbLongDeg contacts (unordered).
 __global__ void __launch_bounds__(kBlock) k_long_list(Window A, int32_t n, const int32_t* vdeg, int32_t* list,
                                                       uint32_t* cnt) {
@@ -2102,6 +2230,10 @@ struct tga_graph {
   float *prb_rank = nullptr, *prb_coef = nullptr, *prb_out = nullptr;
   double *prb_acc = nullptr, *prb_partial = nullptr, *prb_part = nullptr, *prb_sums = nullptr;
   int64_t* prb_win = nullptr;
+  // batched CC state (first use)
+  int32_t *ccb_label = nullptr, *ccb_partial = nullptr, *ccb_changed = nullptr, *ccb_out = nullptr;
+  uint64_t* ccb_mask = nullptr;
+  int32_t ccb_out_rows = 0;
   int32_t prb_out_rows = 0;
   int64_t* dcb_win = nullptr;  // batched DC windows (2 x 64)
   int32_t* dcb_out = nullptr;  // batched DC rows when the caller's output is on the host
@@ -2960,6 +3092,76 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
   HIP_TRY(hipGetLastError());
   if (!(flags & TGA_OUT_DEVICE))
     HIP_TRY(hipMemcpyAsync(rank, out, sizeof(float) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
+  return g->check_err();
+}
+
+extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t* label,
+                                              uint32_t flags) {
+  if (!g || !windows || !label || nq < 1 || nq > 64)
+    return fail(TGA_EINVAL, "tga_connected_components_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q])) return fail(TGA_EINVAL, "tga_connected_components_batch: window start > end");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) {
+    // No room for the run structure: one call per window.
+    for (int32_t q = 0; q < nq; ++q)
+      if ((rc = tga_connected_components(g, windows[q], label + q * n, flags))) return rc;
+    return TGA_OK;
+  }
+  if (!g->ccb_label) {
+    if ((rc = g->alloc(g->ccb_label, (size_t)n * 64)) ||
+        (rc = g->alloc(g->ccb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
+        (rc = g->alloc(g->ccb_mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))) ||
+        (rc = g->alloc(g->ccb_changed, 1)))
+      return rc;
+    if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
+  }
+  int64_t hw[2 * 64];
+  for (int32_t q = 0; q < nq; ++q) {
+    hw[2 * q] = windows[q].start;
+    hw[2 * q + 1] = windows[q].end;
+  }
+  hipStream_t st = g->stream;
+  HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
+  int32_t* out = label;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if (g->ccb_out_rows < nq) {
+      if ((rc = g->alloc(g->ccb_out, (size_t)nq * (size_t)n))) return rc;
+      g->ccb_out_rows = nq;
+    }
+    out = g->ccb_out;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  g->kbegin();
+  if (g->prb_nmulti)
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
+                                                                       g->coff, g->ct, g->prb_win, nq, g->ccb_mask);
+  k_ccb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->ccb_label);
+  HIP_TRY(hipGetLastError());
+  for (int64_t round = 0; round < n && g->prb_nseg; ++round) {
+    k_fill<int32_t><<<1, kBlock, 0, st>>>(g->ccb_changed, 1, 0);
+    k_ccb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+        g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
+        g->ccb_mask, g->prb_win, nq, g->ccb_label, g->ccb_partial, g->ccb_changed);
+    if (g->prb_nmrows)
+      k_ccb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(g->prb_mrows, g->prb_nm, g->prb_seg_off,
+                                                                       g->ccb_partial, nq, g->ccb_label,
+                                                                       g->ccb_changed);
+    HIP_TRY(hipGetLastError());
+    int32_t changed = 0;
+    HIP_TRY(hipMemcpyAsync(&changed, g->ccb_changed, sizeof changed, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!changed) break;
+  }
+  k_ccb_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->ccb_label, out);
+  g->kend();
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE))
+    HIP_TRY(hipMemcpyAsync(label, out, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
   return g->check_err();
 }
 

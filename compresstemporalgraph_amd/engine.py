@@ -79,6 +79,7 @@ def lib():
         L.tga_bfs_server_close.restype = None
         L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
+        L.tga_connected_components_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, vp, C.c_uint32]
         L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
         L.tga_pagerank.argtypes = [vp, Window, C.c_int32, C.c_float, C.c_float, vp, p32, C.c_uint32]
         L.tga_pagerank_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, C.c_float, vp, C.c_uint32]
@@ -240,6 +241,20 @@ class TemporalGraph:
         o = _Out(self.n, np.int32, out)
         _check(lib().tga_connected_components(self.h, _win(window), o.ptr, o.flags))
         return o.arr
+
+    def connected_components_batch(self, windows, out=None):
+        """Batched connected components: up to 64 windows in one call.  Returns
+        an (nq, n) int32 array whose row q equals connected_components(windows[q]);
+        `out` may be a caller's (nq, n) device tensor instead."""
+        nq = len(windows)
+        wins = (Window * nq)(*[_win(w) for w in windows])
+        if out is None:
+            arr = np.empty((nq, self.n), np.int32)
+            ptr, flags = arr.ctypes.data_as(C.c_void_p), 0
+        else:
+            arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
+        _check(lib().tga_connected_components_batch(self.h, wins, nq, ptr, flags))
+        return arr
 
     def graph_coloring(self, window=None, rounds: int = 512, priority=None, out=None):
         o = _Out(self.n, np.int32, out)

@@ -168,6 +168,14 @@ int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arriv
  * BaselineRunner.java:553-608; replaces connected_components.cu:2091-2154). */
 int tga_connected_components(tga_graph* g, tga_window w, int32_t* label, uint32_t flags);
 
+/* Batched connected components: nq (1..64) windows.  label receives nq rows
+ * of n: row q is exactly what tga_connected_components(g, windows[q], ...)
+ * returns.  Min-label rounds over the window-independent run structure of
+ * tga_pagerank_batch, the queries as int32 label columns.  No counterpart in
+ * the reference (one window per executable run). */
+int tga_connected_components_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t* label,
+                                   uint32_t flags);
+
 /* Graph coloring: round c colors every uncolored active v that has no
  * uncolored active out-neighbor u over window contacts with priority[u] >
  * priority[v] (ties: u < v); at most `rounds` rounds; INT32_MAX when uncolored.

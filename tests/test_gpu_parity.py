@@ -194,6 +194,32 @@ def test_connected_components(pair):
         np.testing.assert_array_equal(g.connected_components(w), ds.oracle().cc(window=w))
 
 
+def test_connected_components_batch(pair):
+    """Up to 64 windows in one call (full, ranges, single instants, empty):
+    every row equals the oracle's single-window labels; then a smaller batch."""
+    ds, g = pair
+    rng = np.random.default_rng(23)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    wins = []
+    for q in range(64):
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))
+    got = g.connected_components_batch(wins)
+    o = ds.oracle()
+    for q, w in enumerate(wins):
+        np.testing.assert_array_equal(got[q], o.cc(window=w), err_msg=f"window {q}")
+    np.testing.assert_array_equal(g.connected_components_batch(wins[1:4]), got[1:4])
+
+
 @PATHS
 def test_graph_coloring(pair, path, monkeypatch):
     """Both round engines: higher-priority lists (the default) and decoding
