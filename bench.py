@@ -24,6 +24,8 @@ Prints one JSON line with the driver's contract fields plus:
                 calls (results compared, outputs in HBM).
   batched_dc    64 windowed degree-centrality queries in one batched call
                 against 64 plain calls (results compared, outputs in HBM).
+  batched_cc    64 connected-components queries in one batched call against
+                64 plain calls, windowed and full windows.
   batched_pr    64 PageRank queries (20 iterations) in one batched call
                 against 64 plain calls, windowed and full windows.
   served_mix    Poisson arrivals of 40% BFS, 20% CC, 30% PR, 10% DC at 500/s
@@ -222,6 +224,40 @@ def batched_pr(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
     return out
 
 
+def batched_cc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 connected-components queries in one tga_connected_components_batch
+    call against 64 plain calls, outputs in HBM, windows log-uniform in
+    [2^10, 2^24] (the batching mix's CC share) and full windows; every row
+    compared with its plain result."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(13)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    out = {}
+    for name, wins in (("windowed", [(int(a), int(a + b)) for a, b in zip(starts, lens)]), ("full", [None] * nq)):
+        bat = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+        pla = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+        g.connected_components_batch(wins, out=bat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.connected_components_batch(wins, out=bat)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        g.connected_components(wins[0], out=pla[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in range(nq):
+            g.connected_components(wins[q], out=pla[q])
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        out[name] = {"queries": nq, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+                     "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1),
+                     "speedup": round(tp / tb, 2), "identical": bool(torch.equal(bat, pla))}
+    return out
+
+
 def served_bfs(g, t_lo: int, t_hi: int) -> dict:
     """Serving: BFS window queries at Poisson rates of 1k and 10k per second
     (log-uniform window lengths in [2^10, 2^24], random sources), continuous
@@ -247,12 +283,12 @@ def served_bfs(g, t_lo: int, t_hi: int) -> dict:
 MIX = {"DC": 0.1, "PR": 0.3, "CC": 0.2}  # the rest BFS: the batching mix of the at-scale queries
 
 
-def served_mix(g, g_dc, g_cc, t_lo: int, t_hi: int) -> dict:
+def served_mix(g, workers: dict, t_lo: int, t_hi: int) -> dict:
     """Serving a mixed stream of window queries (40% BFS, 20% CC, 30% PR with
     20 iterations, 10% DC) at Poisson rates of 500 and 2000 per second: BFS by
-    continuous batching in lanes; DC and PR in batched calls on a second
-    handle and CC one by one on a third, each in a worker thread; against
-    sequential plain execution.  Results in HBM; at 500/s every result is compared with the
+    continuous batching in lanes; DC, PR and CC in batched calls, each kind on
+    a handle of its own in a worker thread (plain calls when fewer than
+    serving.MIN_BATCH are pending); against sequential plain execution.  Results in HBM; at 500/s every result is compared with the
     plain one (PageRank rows within 1e-6 relative L-inf, the rest exact)."""
     import torch
     from compresstemporalgraph_amd import serving
@@ -260,7 +296,7 @@ def served_mix(g, g_dc, g_cc, t_lo: int, t_hi: int) -> dict:
     for rate, count in ((500, 400), (2000, 800)):
         req = serving.mixed_requests(rate, count, g.info["n"], t_lo, t_hi, seed=rate + 7, shares=MIX)
         kb, kp = ({}, {}) if rate == 1000 else (None, None)
-        sb = serving.serve_batched(g, req, keep=kb, device_out=True, dc_graph=g_dc, cc_graph=g_cc)
+        sb = serving.serve_batched(g, req, keep=kb, device_out=True, workers=workers)
         sp = serving.serve_plain(g, req, keep=kp, device_out=True)
         row = {"batched": sb, "plain": sp}
         if kb is not None:
@@ -296,21 +332,25 @@ def query_legs(args) -> int:
     log(f"[bench] batched DC: {bdc}")
     if not bdc["identical"]:
         raise SystemExit("batched DC results differ from the plain results")
+    bcc = batched_cc(g, t_lo, t_hi)
+    log(f"[bench] batched CC: {bcc}")
+    if not all(v["identical"] for v in bcc.values()):
+        raise SystemExit("batched CC results differ from the plain results")
     bpr = batched_pr(g, t_lo, t_hi)
     log(f"[bench] batched PR: {bpr}")
     if max(v["max_rel_diff"] for v in bpr.values()) > 1e-5:
         raise SystemExit("batched PageRank differs from the plain results beyond 1e-5")
     served = served_bfs(g, t_lo, t_hi)
     log(f"[bench] served BFS: {served}")
-    g_dc = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
-    g_cc = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
-    smix = served_mix(g, g_dc, g_cc, t_lo, t_hi)
-    g_dc.close()
-    g_cc.close()
+    workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+               for k in ("DC", "PR", "CC")}
+    smix = served_mix(g, workers, t_lo, t_hi)
+    for h in workers.values():
+        h.close()
     log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
-    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_pr": bpr, "served_bfs": served,
-                      "served_mix": smix}), flush=True)
+    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc, "batched_pr": bpr,
+                      "served_bfs": served, "served_mix": smix}), flush=True)
     return 0
 
 
@@ -434,7 +474,7 @@ def main() -> int:
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
-    batched = bdc = bpr = served = smix = None
+    batched = bdc = bcc = bpr = served = smix = None
     if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -444,17 +484,17 @@ def main() -> int:
         cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
         if args.data_root:
             cmd += ["--data-root", args.data_root]
-        # Twelve hardware queues: the served mix runs batches and CC queries
-        # on two more handles beside the BFS loop, and with HIP's default of
-        # four the handles' streams share queues, so a long kernel of one
-        # stalls the launches queued behind it on another.
+        # Sixteen hardware queues: the served mix runs its batches on three
+        # more handles beside the BFS loop, and with HIP's default of four the
+        # handles' streams share queues, so a long kernel of one stalls the
+        # launches queued behind it on another.
         env = dict(os.environ)
-        env.setdefault("GPU_MAX_HW_QUEUES", "12")
+        env.setdefault("GPU_MAX_HW_QUEUES", "16")
         child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
-        batched, bdc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_pr"]
+        batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
         served, smix = legs["served_bfs"], legs["served_mix"]
 
     cpu = None
@@ -485,6 +525,7 @@ def main() -> int:
         "workloads_ms": workloads,
         "batched_bfs": batched,
         "batched_dc": bdc,
+        "batched_cc": bcc,
         "batched_pr": bpr,
         "served_mix": smix,
         "served_bfs": served,

@@ -8,9 +8,9 @@ advances all running queries by one BFS level per decode pass, and completes
 a query as soon as its frontier empties -- so a query that arrives while
 others are on the GPU joins them at the next level instead of waiting for the
 batch to drain.  DC and PR requests that have arrived are answered together
-by one batched call per kind (up to 64 windows; PR 20 iterations); CC
-requests one at a time.  With a second handle these run in a worker thread
-beside the BFS loop, else between its steps.  The plain baseline serves the
+by one batched call per kind (up to 64 windows; PR 20 iterations), and so
+are CC requests.  Given more handles of the graph (one per kind, or shared),
+these run in worker threads beside the BFS loop, else between its steps.  The plain baseline serves the
 same arrivals one query at a time (TemporalGraph.bfs, connected_components,
 pagerank, degree_centrality), in arrival order.
 
@@ -123,8 +123,11 @@ def _pr_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
     return [rank[j] for j in range(len(idx))]
 
 
-def _cc_one(g: TemporalGraph, r, device_out: bool):
-    return g.connected_components(r.window, out=_sink(g, device_out)())
+def _cc_batch(g: TemporalGraph, req: list, idx: list, device_out: bool):
+    """One batched CC call for requests idx; returns its rows."""
+    wins = [req[i].window for i in idx]
+    lab = g.connected_components_batch(wins, out=_sink(g, True, len(idx))() if device_out else None)
+    return [lab[j] for j in range(len(idx))]
 
 
 def _plain_one(g: TemporalGraph, r, device_out: bool):
@@ -144,18 +147,18 @@ def _plain_one(g: TemporalGraph, r, device_out: bool):
 
 
 # Fewest pending queries of a kind worth one batched call: a batched call
-# costs about the same whatever its width (one pass over the graph for 64
-# columns), so a few PR queries (a 64-wide batch costs about as much as 16-24
-# windowed plain ones) or one DC query run as plain calls.
-MIN_BATCH = {"DC": 2, "PR": 16}
+# costs about the same whatever its width (passes over the graph for 64
+# columns), so below these counts the plain calls are cheaper (on a 2^25
+# contact graph a windowed batch costs about 24 plain PR, 9 plain CC or 2
+# plain DC queries).
+MIN_BATCH = {"DC": 2, "PR": 16, "CC": 8}
 
 
 def _answer(g: TemporalGraph, req: list, idx: list, device_out: bool) -> list:
-    """Results for the non-BFS requests idx: DC and PR batched per kind (up
-    to 64 each; plain calls below MIN_BATCH), CC one at a time; returns
-    [(i, result)]."""
+    """Results for the non-BFS requests idx, batched per kind (up to 64
+    each; plain calls below MIN_BATCH); returns [(i, result)]."""
     out = []
-    for kind, fn in (("DC", _dc_batch), ("PR", _pr_batch)):
+    for kind, fn in (("DC", _dc_batch), ("PR", _pr_batch), ("CC", _cc_batch)):
         sel = [i for i in idx if req[i].kind == kind]
         for b in range(0, len(sel), 64):
             part = sel[b:b + 64]
@@ -163,7 +166,6 @@ def _answer(g: TemporalGraph, req: list, idx: list, device_out: bool) -> list:
                 out += list(zip(part, fn(g, req, part, device_out)))
             else:
                 out += [(i, _plain_one(g, req[i], device_out)) for i in part]
-    out += [(i, _cc_one(g, req[i], device_out)) for i in idx if req[i].kind == "CC"]
     return out
 
 
@@ -226,13 +228,13 @@ class _BatchWorker:
 
 
 def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep=None,
-                  device_out: bool = False, dc_graph: TemporalGraph = None,
-                  cc_graph: TemporalGraph = None) -> dict:
+                  device_out: bool = False, workers: dict = None) -> dict:
     """Continuous batching.  `keep` (a dict) receives the results per request
-    index when given (for identity checks).  With `dc_graph` (a second handle
-    of the same graph), the DC / PR batches and CC queries run beside the BFS
-    loop in a worker thread instead of between its steps; with `cc_graph` too
-    (a third handle), CC queries get a worker of their own."""
+    index when given (for identity checks).  `workers` maps the kinds "DC",
+    "PR" and "CC" to other handles of the same graph: each distinct handle
+    gets a worker thread that answers its kinds beside the BFS loop (the
+    handles' streams overlap on the device); kinds without one are answered
+    between the BFS steps."""
     lanes = lanes_for_budget(g.n, budget_bytes)
     sink = _sink(g, device_out)
     done_at = [0.0] * len(req)
@@ -241,19 +243,20 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
         lane_of, dc_wait = {}, []
         nxt = 0
         t0 = time.perf_counter()
-        worker = _BatchWorker(dc_graph, req, device_out, t0, done_at, keep) if dc_graph is not None else None
-        ccw = _BatchWorker(cc_graph, req, device_out, t0, done_at, keep) if worker and cc_graph is not None else None
+        by_handle = {}
+        for kind, h in (workers or {}).items():
+            if id(h) not in by_handle:
+                by_handle[id(h)] = _BatchWorker(h, req, device_out, t0, done_at, keep)
+        worker_of = {kind: by_handle[id(h)] for kind, h in (workers or {}).items()}
+        pool = list(by_handle.values())
         try:
-            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or (worker and worker.busy()) or
-                                    (ccw and ccw.busy())):
+            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or any(w.busy() for w in pool)):
                 now = time.perf_counter() - t0
                 while nxt < len(order) and req[order[nxt]].arrival <= now:
                     i = order[nxt]
                     if req[i].kind != "BFS":
-                        if ccw and req[i].kind == "CC":
-                            ccw.put(i)
-                        elif worker:
-                            worker.put(i)
+                        if req[i].kind in worker_of:
+                            worker_of[req[i].kind].put(i)
                         else:
                             dc_wait.append(i)
                     elif len(lane_of) < lanes:
@@ -261,9 +264,9 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
                     else:
                         break  # BFS lanes full: this request waits (arrival order kept)
                     nxt += 1
-                if not lane_of and not dc_wait:  # idle until the next arrival (or the worker)
+                if not lane_of and not dc_wait:  # idle until the next arrival (or the workers)
                     wake = req[order[nxt]].arrival - now if nxt < len(order) else 1e-4
-                    time.sleep(max(0.0, min(wake, 1e-3) if worker else wake))
+                    time.sleep(max(0.0, min(wake, 1e-3) if pool else wake))
                     continue
                 if dc_wait:
                     idx, dc_wait = dc_wait, []
@@ -281,10 +284,8 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
                         if keep is not None:
                             keep[i] = levels
         finally:
-            if worker:
-                worker.close()
-            if ccw:
-                ccw.close()
+            for w in pool:
+                w.close()
     out = _stats(req, done_at)
     out["lanes"] = lanes
     return out

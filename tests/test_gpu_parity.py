@@ -309,8 +309,8 @@ def test_hits(pair, path, monkeypatch):
 @pytest.mark.parametrize("worker", [False, True])
 def test_served_mix(pair, worker):
     """A BFS / CC / PR / DC request stream through the serving loop (BFS in
-    server lanes, DC and PR in batched calls, CC one by one; between the BFS
-    steps or on a second handle in a worker thread): every result equals the
+    server lanes, DC, PR and CC in batched calls; between the BFS steps or on
+    a second handle in a worker thread): every result equals the
     oracle's answer to that query alone (PageRank within its bar)."""
     from compresstemporalgraph_amd import serving
     ds, g = pair
@@ -323,7 +323,7 @@ def test_served_mix(pair, worker):
     keep = {}
     g2 = TemporalGraph(ds.dir, ds.name) if worker else None
     try:
-        serving.serve_batched(g, req, keep=keep, dc_graph=g2)
+        serving.serve_batched(g, req, keep=keep, workers={"DC": g2, "PR": g2, "CC": g2} if g2 else None)
     finally:
         if g2 is not None:
             g2.close()

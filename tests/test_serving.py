@@ -122,6 +122,10 @@ class _FakeGraph:
     def connected_components(self, window=None, out=None):
         return ("cc", window)
 
+    def connected_components_batch(self, wins, out=None):
+        assert serving.MIN_BATCH["CC"] <= len(wins) <= 64
+        return [("cc", w) for w in wins]
+
 
 def test_serve_loop_bookkeeping():
     """Every request is answered exactly once with its own result, in both
@@ -155,7 +159,7 @@ def test_serve_loop_dc_worker():
     req = serving.mixed_requests(rate=1e5, count=300, n=1000, t_lo=0, t_hi=1 << 20, dc_share=0.3, seed=5)
     g, g2 = _FakeGraph(), _FakeGraph()
     kb = {}
-    st = serving.serve_batched(g, req, keep=kb, dc_graph=g2)
+    st = serving.serve_batched(g, req, keep=kb, workers={"DC": g2})
     assert sorted(kb) == list(range(len(req))) and st["queries"] == len(req)
     for i, r in enumerate(req):
         want = (("cnt", r.window), ("cen", r.window)) if r.kind == "DC" else ("bfs", r.src, r.window)
@@ -180,14 +184,14 @@ def test_serve_loop_all_kinds():
                                  shares={"DC": 0.1, "PR": 0.3, "CC": 0.2})
     want = {"BFS": lambda r: ("bfs", r.src, r.window), "CC": lambda r: ("cc", r.window),
             "PR": lambda r: ("pr", r.window), "DC": lambda r: (("cnt", r.window), ("cen", r.window))}
-    for workers in (0, 1, 2):
-        g, g2, g3 = _FakeGraph(), _FakeGraph(), _FakeGraph()
+    for workers in (0, 1, 3):
+        g, g2, g3, g4 = _FakeGraph(), _FakeGraph(), _FakeGraph(), _FakeGraph()
         kb, kp = {}, {}
-        serving.serve_batched(g, req, keep=kb, dc_graph=g2 if workers else None,
-                              cc_graph=g3 if workers == 2 else None)
+        wmap = {0: None, 1: {"DC": g2, "PR": g2, "CC": g2}, 3: {"DC": g2, "PR": g3, "CC": g4}}[workers]
+        serving.serve_batched(g, req, keep=kb, workers=wmap)
         serving.serve_plain(g, req, keep=kp)
         for i, r in enumerate(req):
             assert kb[i] == want[r.kind](r) == kp[i], (workers, i, r.kind)
-        gb = g2 if workers else g
+        gb = {0: g, 1: g2, 3: g3}[workers]
         calls = getattr(gb, "pr_calls", [])
         assert all(serving.MIN_BATCH["PR"] <= len(c) <= 64 for c in calls)
