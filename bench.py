@@ -340,14 +340,16 @@ def query_legs(args) -> int:
     log(f"[bench] batched PR: {bpr}")
     if max(v["max_rel_diff"] for v in bpr.values()) > 1e-5:
         raise SystemExit("batched PageRank differs from the plain results beyond 1e-5")
-    served = served_bfs(g, t_lo, t_hi)
-    log(f"[bench] served BFS: {served}")
-    workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
-               for k in ("DC", "PR", "CC")}
-    smix = served_mix(g, workers, t_lo, t_hi)
-    for h in workers.values():
-        h.close()
-    log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
+    served = smix = None
+    if not args.no_served:
+        served = served_bfs(g, t_lo, t_hi)
+        log(f"[bench] served BFS: {served}")
+        workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+                   for k in ("DC", "PR", "CC")}
+        smix = served_mix(g, workers, t_lo, t_hi)
+        for h in workers.values():
+            h.close()
+        log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
     print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc, "batched_pr": bpr,
                       "served_bfs": served, "served_mix": smix}), flush=True)
@@ -393,6 +395,8 @@ def main() -> int:
                     help="skip the seven-workload table and the batched / served query legs")
     ap.add_argument("--no-query-legs", action="store_true",
                     help="time the seven workloads but skip the batched / served BFS legs")
+    ap.add_argument("--no-served", action="store_true",
+                    help="with --query-legs: the batched legs only (no served streams)")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
