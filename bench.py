@@ -28,6 +28,8 @@ Prints one JSON line with the driver's contract fields plus:
                 64 plain calls, windowed and full windows.
   batched_pr    64 PageRank queries (20 iterations) in one batched call
                 against 64 plain calls, windowed and full windows.
+  batched_hits  64 HITS queries (20 iterations) in one batched call against
+                64 plain calls, windowed and full windows.
   served_mix    Poisson arrivals of 40% BFS, 20% CC, 30% PR, 10% DC at 500/s
                 and 2000/s: continuous batching against sequential plain
                 execution.
@@ -258,6 +260,40 @@ def batched_cc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
     return out
 
 
+def batched_hits(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 HITS queries (20 iterations, FP32) in one tga_hits_batch call
+    against 64 plain calls, outputs in HBM, windowed and full windows; rows
+    compared with the plain results (max relative L-inf over hub and auth)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(21)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    out = {}
+    for name, wins in (("windowed", [(int(a), int(a + b)) for a, b in zip(starts, lens)]), ("full", [None] * nq)):
+        hb, ab, hp, ap = (torch.empty((nq, n), dtype=torch.float32, device="cuda:0") for _ in range(4))
+        g.hits_batch(wins, iters=20, out_hub=hb, out_auth=ab)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.hits_batch(wins, iters=20, out_hub=hb, out_auth=ab)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        g.hits(wins[0], iters=20, out_hub=hp[0], out_auth=ap[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in range(nq):
+            g.hits(wins[q], iters=20, out_hub=hp[q], out_auth=ap[q])
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        rel = max(float(((x.double() - y.double()).abs().amax(dim=1) /
+                         y.double().abs().amax(dim=1).clamp_min(1e-300)).max()) for x, y in ((hb, hp), (ab, ap)))
+        out[name] = {"queries": nq, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+                     "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1),
+                     "speedup": round(tp / tb, 2), "max_rel_diff": rel}
+    return out
+
+
 def served_bfs(g, t_lo: int, t_hi: int) -> dict:
     """Serving: BFS window queries at Poisson rates of 1k and 10k per second
     (log-uniform window lengths in [2^10, 2^24], random sources), continuous
@@ -340,6 +376,10 @@ def query_legs(args) -> int:
     log(f"[bench] batched PR: {bpr}")
     if max(v["max_rel_diff"] for v in bpr.values()) > 1e-5:
         raise SystemExit("batched PageRank differs from the plain results beyond 1e-5")
+    bhits = batched_hits(g, t_lo, t_hi)
+    log(f"[bench] batched HITS: {bhits}")
+    if max(v["max_rel_diff"] for v in bhits.values()) > 1e-5:
+        raise SystemExit("batched HITS differs from the plain results beyond 1e-5")
     served = smix = None
     if not args.no_served:
         served = served_bfs(g, t_lo, t_hi)
@@ -352,7 +392,7 @@ def query_legs(args) -> int:
         log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
     print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc, "batched_pr": bpr,
-                      "served_bfs": served, "served_mix": smix}), flush=True)
+                      "batched_hits": bhits, "served_bfs": served, "served_mix": smix}), flush=True)
     return 0
 
 
@@ -478,7 +518,7 @@ def main() -> int:
         log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
-    batched = bdc = bcc = bpr = served = smix = None
+    batched = bdc = bcc = bpr = bhits = served = smix = None
     if not args.no_workloads and not args.no_query_legs:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -499,6 +539,7 @@ def main() -> int:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
         batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
+        bhits = legs["batched_hits"]
         served, smix = legs["served_bfs"], legs["served_mix"]
 
     cpu = None
@@ -531,6 +572,7 @@ def main() -> int:
         "batched_dc": bdc,
         "batched_cc": bcc,
         "batched_pr": bpr,
+        "batched_hits": bhits,
         "served_mix": smix,
         "served_bfs": served,
     }

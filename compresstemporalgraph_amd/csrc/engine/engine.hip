@@ -1147,6 +1147,172 @@ __global__ void __launch_bounds__(kBlock) k_ccb_out(int32_t n, int32_t nq, const
   }
 }
 
+// ------------------------------------------------------- batched HITS ---
+//
+// Up to 64 HITS queries as FP32 hub and authority columns.  The authority
+// step pulls over the destination-sorted runs (in-runs of r), the hub step
+// over the same runs grouped by source in fill order (out-runs of x: per
+// run its destination, or -1 when inactive, and its index among the sorted
+// runs, which keys the single-contact time and the multi-contact link mask of
+// batched CC).  Rows are cut into segments like the in-rows; both steps are
+// followed by fixed-order L2 column norms.
+__global__ void __launch_bounds__(kBlock) k_run_inv(uint64_t nv, const uint64_t* sv, int32_t* inv) {
+  for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < (int64_t)nv; j += (int64_t)gridDim.x * kBlock)
+    inv[sv[j]] = (int32_t)j;
+}
+
+// Out-run i (fill order): (r or -1, sorted index or -1, t0, 0).
+__global__ void __launch_bounds__(kBlock) k_out_runs(int64_t nr, const int32_t* rr, const int32_t* inv,
+                                                     const int4* runs, int4* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nr; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t j = inv[i];
+    out[i] = j >= 0 ? int4{rr[i], j, runs[j].w, 0} : int4{-1, -1, 0, 0};
+  }
+}
+
+// Out-row of every active x: its items' runs (one range: its chunks are
+// consecutive items), and its segment count.
+__global__ void __launch_bounds__(kBlock) k_out_rows(Window A, int32_t n, const int32_t* vinfo, const Ckpt* ck,
+                                                     const uint64_t* off, uint64_t* obeg, int32_t* olen,
+                                                     uint32_t* nseg) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    uint64_t b = 0, e = 0;
+    if (A.active((int32_t)v)) {
+      const int32_t info = vinfo[v];
+      if (info >= 0) {
+        b = off[n + info];
+        e = off[n + info + ck[info].nchunk];
+      } else {
+        b = off[v];
+        e = off[v + 1];
+      }
+    }
+    obeg[v] = b;
+    olen[v] = (int32_t)(e - b);
+    nseg[v] = (uint32_t)((e - b + kPrbSeg - 1) / kPrbSeg);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_oseg_fill(int32_t n, const uint64_t* obeg, const int32_t* olen,
+                                                      const uint64_t* seg_off, int32_t* seg_row, uint64_t* seg_beg,
+                                                      int32_t* seg_len, int32_t* mrows, uint32_t* nm) {
+  for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < n; r += (int64_t)gridDim.x * kBlock) {
+    const uint64_t s0 = seg_off[r], s1 = seg_off[r + 1];
+    for (uint64_t s = s0; s < s1; ++s) {
+      const int64_t j = (int64_t)(s - s0) * kPrbSeg;
+      seg_row[s] = (int32_t)r;
+      seg_beg[s] = obeg[r] + (uint64_t)j;
+      seg_len[s] = (int32_t)min((int64_t)kPrbSeg, (int64_t)olen[r] - j);
+    }
+    if (s1 - s0 > 1) mrows[atomicAdd(nm, 1u)] = (int32_t)r;
+  }
+}
+
+// One wave per segment: lane q sums (double)vec[y][q] over the segment's
+// runs linked in query q, y the run's other end.  OUT: out-runs (y = r, -1
+// for an inactive r); else in-runs (y = x).  Single-segment rows store the
+// sum in acc, others in partial (k_prb_fix adds them with base 0).
+template <int OUT>
+__global__ void __launch_bounds__(kBlock) k_hitsb_pull(int64_t nseg, const int32_t* seg_row, const uint64_t* seg_beg,
+                                                       const int32_t* seg_len, const uint64_t* seg_off,
+                                                       const int4* rr, const int32_t* midx, const uint64_t* mask,
+                                                       const int64_t* wins, int32_t nq, const float* vec,
+                                                       double* acc, double* partial) {
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t s = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); s < nseg; s += nw) {
+    const int32_t row = seg_row[s];
+    const uint64_t sb = seg_beg[s];
+    const int32_t len = seg_len[s];
+    double sum = 0;
+    for (int32_t b0 = 0; b0 < len; b0 += 64) {
+      const int32_t j = b0 + lane;
+      const int4 my = j < len ? rr[sb + j] : int4{-1, -1, 0, 0};
+      const int32_t y = my.x;
+      const int32_t t0 = OUT ? my.z : my.w;
+      const int32_t si = OUT ? my.y : (j < len ? (int32_t)(sb + j) : -1);
+      const int32_t mym = si >= 0 ? midx[si] : -1;
+      const int32_t cnt = min(64, len - b0);
+      for (int32_t u0 = 0; u0 < cnt; u0 += kPrbAhead) {
+        float vv[kPrbAhead];
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          const int32_t yy = (u0 + u < cnt) ? rl32(y, u0 + u) : -1;
+          vv[u] = yy >= 0 ? vec[(int64_t)yy * 64 + lane] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          if (u0 + u >= cnt) break;
+          const int l = u0 + u;
+          if (rl32(y, l) < 0) continue;
+          const int32_t mi = rl32(mym, l);
+          bool hit;
+          if (mi >= 0) {
+            hit = mask[mi] >> lane & 1;
+          } else {
+            const int32_t tt = rl32(t0, l);
+            hit = tt >= ws && tt <= we;
+          }
+          if (hit) sum += (double)vv[u];
+        }
+      }
+    }
+    if (seg_off[row + 1] - seg_off[row] > 1)
+      partial[s * 64 + lane] = sum;
+    else
+      acc[(int64_t)row * 64 + lane] = sum;
+  }
+}
+
+// Column sums of squares (k_prb_colsum's fixed order) and their square roots.
+__global__ void __launch_bounds__(kBlock) k_colsq(int32_t n, const double* acc, double* part) {
+  __shared__ double sh[kBlock / 64][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double s = 0;
+  for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + wave; v < n; v += (int64_t)gridDim.x * (kBlock / 64)) {
+    const double x = acc[v * 64 + lane];
+    s += x * x;
+  }
+  sh[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0) {
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += sh[w][lane];
+    part[(int64_t)blockIdx.x * 64 + lane] = t;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_colsq2(const double* part, int32_t nb, double* norms) {
+  double t = 0;
+  for (int b = 0; b < nb; ++b) t += part[(int64_t)b * 64 + threadIdx.x];
+  norms[threadIdx.x] = sqrt(t);
+}
+
+__global__ void __launch_bounds__(kBlock) k_colscale(int32_t n, const double* acc, const double* norms, float* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const double s = norms[i & 63];
+    out[i] = (float)(s > 0 ? acc[i] / s : acc[i]);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cols_init(Window A, int32_t n, float v, float* x, float* y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const float a = A.active((int32_t)(i >> 6)) ? v : 0.0f;
+    x[i] = a;
+    y[i] = a;
+  }
+}
+
+// out[q][v] = col[v][q] for q < nq.
+__global__ void __launch_bounds__(kBlock) k_cols_out(int32_t n, int32_t nq, const float* col, float* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const int32_t q = (int32_t)(i & 63);
+    if (q < nq) out[(int64_t)q * n + (i >> 6)] = col[i];
+  }
+}
+
 // Active vertices with more than kPrbLongDeg contacts (unordered).
 __global__ void __launch_bounds__(kBlock) k_long_list(Window A, int32_t n, const int32_t* vdeg, int32_t* list,
                                                       uint32_t* cnt) {
@@ -2230,6 +2396,17 @@ struct tga_graph {
   float *prb_rank = nullptr, *prb_coef = nullptr, *prb_out = nullptr;
   double *prb_acc = nullptr, *prb_partial = nullptr, *prb_part = nullptr, *prb_sums = nullptr;
   int64_t* prb_win = nullptr;
+  // batched HITS: out-runs (built with the run structure) and batch state
+  int4* prb_out_runs = nullptr;
+  uint64_t *prb_obeg = nullptr, *prb_oseg_off = nullptr, *prb_oseg_beg = nullptr;
+  int32_t *prb_olen = nullptr, *prb_oseg_row = nullptr, *prb_oseg_len = nullptr, *prb_omrows = nullptr;
+  uint32_t* prb_onm = nullptr;
+  int64_t prb_noseg = 0;
+  int32_t prb_nomrows = 0;
+  float *hb_hub = nullptr, *hb_auth = nullptr, *hb_out = nullptr;
+  double *hb_partial = nullptr, *hb_norm = nullptr;
+  uint64_t* hb_mask = nullptr;
+  int32_t hb_out_rows = 0;
   // batched CC state (first use)
   int32_t *ccb_label = nullptr, *ccb_partial = nullptr, *ccb_changed = nullptr, *ccb_out = nullptr;
   uint64_t* ccb_mask = nullptr;
@@ -2920,8 +3097,8 @@ static int ensure_prb(tga_graph* g, bool* ok) {
                              g->stream));
   // temporaries 40 B per run + sort storage; kept: 16 B per run, 8 + 8 + 4 B
   // per vertex, batch state 1.8 KB per vertex (64 columns), segments
-  const uint64_t need = cap * (56 + 8) + cap / 2 * 256 + tmp_bytes + (uint64_t)n * (20 + 64 * (4 + 4 + 4 + 8 + 8) + 24) +
-                        64 * 1024;
+  const uint64_t need = cap * (56 + 8 + 16) + cap / 2 * 256 + tmp_bytes +
+                        (uint64_t)n * (20 + 64 * (4 + 4 + 4 + 8 + 8) + 24 + 40) + 64 * 1024;
   if ((rc = list_budget(need, false, ok)) || !*ok) return rc;
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
   hipStream_t st = g->stream;
@@ -2980,6 +3157,36 @@ static int ensure_prb(tga_graph* g, bool* ok) {
     HIP_TRY(hipStreamSynchronize(st));  // before the temporaries go
     g->prb_nmulti = (int32_t)nmulti;
   }
+  {
+    // out-runs for batched HITS: fill order, with the sorted index of each
+    DevTmp t_inv;
+    int32_t* inv;
+    HIP_TRY(t_inv.get(inv, nr));
+    k_fill<int32_t><<<g->grid((int64_t)nr), kBlock, 0, st>>>(inv, (int64_t)nr, -1);
+    k_run_inv<<<g->grid((int64_t)nv), kBlock, 0, st>>>(nv, sv, inv);
+    if ((rc = g->alloc(g->prb_out_runs, (size_t)nr)) || (rc = g->alloc(g->prb_obeg, (size_t)n)) ||
+        (rc = g->alloc(g->prb_olen, (size_t)n)) || (rc = g->alloc(g->prb_oseg_off, (size_t)n + 1)) ||
+        (rc = g->alloc(g->prb_omrows, (size_t)n)) || (rc = g->alloc(g->prb_onm, 1)))
+      return rc;
+    k_out_runs<<<g->grid((int64_t)nr), kBlock, 0, st>>>((int64_t)nr, rr, inv, runs, g->prb_out_runs);
+    k_out_rows<<<g->grid(n), kBlock, 0, st>>>(A, (int32_t)n, g->vinfo, g->ck, g->loff, g->prb_obeg, g->prb_olen, nseg);
+    HIP_TRY(hipGetLastError());
+    if ((rc = scan_counts(g, nseg, n, g->prb_oseg_off))) return rc;
+    uint64_t nos = 0;
+    if ((rc = read_u64(g, g->prb_oseg_off + n, &nos))) return rc;
+    if ((rc = g->alloc(g->prb_oseg_row, (size_t)nos)) || (rc = g->alloc(g->prb_oseg_beg, (size_t)nos)) ||
+        (rc = g->alloc(g->prb_oseg_len, (size_t)nos)))
+      return rc;
+    k_fill<uint32_t><<<1, kBlock, 0, st>>>(g->prb_onm, 1, 0u);
+    k_oseg_fill<<<g->grid(n), kBlock, 0, st>>>((int32_t)n, g->prb_obeg, g->prb_olen, g->prb_oseg_off,
+                                               g->prb_oseg_row, g->prb_oseg_beg, g->prb_oseg_len, g->prb_omrows,
+                                               g->prb_onm);
+    HIP_TRY(hipGetLastError());
+    uint32_t onm = 0;
+    if ((rc = g->read_counts(g->prb_onm, &onm, 1))) return rc;  // synchronizes before the temporaries go
+    g->prb_noseg = (int64_t)nos;
+    g->prb_nomrows = (int32_t)onm;
+  }
   k_seg_count<<<g->grid(n), kBlock, 0, st>>>((int32_t)n, g->prb_in_off, nseg);
   HIP_TRY(hipGetLastError());
   if ((rc = scan_counts(g, nseg, n, g->prb_seg_off))) return rc;
@@ -3036,10 +3243,14 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
   }
   if (!g->prb_du) {
     if ((rc = g->alloc(g->prb_du, (size_t)n * 64)) || (rc = g->alloc(g->prb_rank, (size_t)n * 64)) ||
-        (rc = g->alloc(g->prb_coef, (size_t)n * 64)) || (rc = g->alloc(g->prb_acc, (size_t)n * 64)) ||
+        (rc = g->alloc(g->prb_coef, (size_t)n * 64)) ||
         (rc = g->alloc(g->prb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
-        (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64)) || (rc = g->alloc(g->prb_sums, 64)) ||
-        (rc = g->alloc(g->prb_win, 2 * 64)) || (rc = g->alloc(g->prb_wm, (size_t)g->prb_nmulti * 64)))
+        (rc = g->alloc(g->prb_sums, 64)) || (rc = g->alloc(g->prb_wm, (size_t)g->prb_nmulti * 64)))
+      return rc;
+    // shared with batched HITS, which may have allocated them first
+    if ((!g->prb_acc && (rc = g->alloc(g->prb_acc, (size_t)n * 64))) ||
+        (!g->prb_part && (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64))) ||
+        (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))))
       return rc;
   }
   int64_t hw[2 * 64];
@@ -3162,6 +3373,93 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
   HIP_TRY(hipGetLastError());
   if (!(flags & TGA_OUT_DEVICE))
     HIP_TRY(hipMemcpyAsync(label, out, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
+  return g->check_err();
+}
+
+extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t iters, float* hub,
+                              float* auth, uint32_t flags) {
+  if (!g || !windows || !hub || !auth || nq < 1 || nq > 64 || iters < 0)
+    return fail(TGA_EINVAL, "tga_hits_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q])) return fail(TGA_EINVAL, "tga_hits_batch: window start > end");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) {
+    // No room for the run structure: one call per window.
+    for (int32_t q = 0; q < nq; ++q)
+      if ((rc = tga_hits(g, windows[q], iters, hub + q * n, auth + q * n, flags))) return rc;
+    return TGA_OK;
+  }
+  if (!g->hb_hub) {
+    if ((rc = g->alloc(g->hb_hub, (size_t)n * 64)) || (rc = g->alloc(g->hb_auth, (size_t)n * 64)) ||
+        (rc = g->alloc(g->hb_partial,
+                       (size_t)std::max<int64_t>(std::max(g->prb_nseg, g->prb_noseg), 1) * 64)) ||
+        (rc = g->alloc(g->hb_norm, 64)) || (rc = g->alloc(g->hb_mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))))
+      return rc;
+    if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
+    if (!g->prb_acc && (rc = g->alloc(g->prb_acc, (size_t)n * 64))) return rc;
+    if (!g->prb_part && (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64))) return rc;
+  }
+  int64_t hw[2 * 64];
+  for (int32_t q = 0; q < nq; ++q) {
+    hw[2 * q] = windows[q].start;
+    hw[2 * q + 1] = windows[q].end;
+  }
+  hipStream_t st = g->stream;
+  HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
+  float *oh = hub, *oa = auth;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if (g->hb_out_rows < 2 * nq) {
+      if ((rc = g->alloc(g->hb_out, (size_t)2 * nq * (size_t)n))) return rc;
+      g->hb_out_rows = 2 * nq;
+    }
+    oh = g->hb_out;
+    oa = g->hb_out + (int64_t)nq * n;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  g->kbegin();
+  if (g->prb_nmulti)
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
+                                                                       g->coff, g->ct, g->prb_win, nq, g->hb_mask);
+  k_cols_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, 1.0f, g->hb_hub, g->hb_auth);
+  for (int32_t it = 0; it < iters; ++it) {
+    for (int step = 0; step < 2; ++step) {  // 0: hub from authority over out-runs; 1: authority from hub, in-runs
+      k_fill<double><<<g->grid(n * 64), kBlock, 0, st>>>(g->prb_acc, n * 64, 0.0);
+      if (step == 0) {
+        if (g->prb_noseg)
+          k_hitsb_pull<1><<<g->grid(g->prb_noseg * 64), kBlock, 0, st>>>(
+              g->prb_noseg, g->prb_oseg_row, g->prb_oseg_beg, g->prb_oseg_len, g->prb_oseg_off, g->prb_out_runs,
+              g->prb_midx, g->hb_mask, g->prb_win, nq, g->hb_auth, g->prb_acc, g->hb_partial);
+        if (g->prb_nomrows)
+          k_prb_fix<<<g->grid((int64_t)g->prb_nomrows * 64), kBlock, 0, st>>>(
+              g->prb_omrows, g->prb_onm, g->prb_oseg_off, g->hb_partial, 0.0, g->prb_acc);
+      } else {
+        if (g->prb_nseg)
+          k_hitsb_pull<0><<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+              g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
+              g->hb_mask, g->prb_win, nq, g->hb_hub, g->prb_acc, g->hb_partial);
+        if (g->prb_nmrows)
+          k_prb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(
+              g->prb_mrows, g->prb_nm, g->prb_seg_off, g->hb_partial, 0.0, g->prb_acc);
+      }
+      k_colsq<<<kPrbRedBlocks, kBlock, 0, st>>>((int32_t)n, g->prb_acc, g->prb_part);
+      k_colsq2<<<1, 64, 0, st>>>(g->prb_part, kPrbRedBlocks, g->hb_norm);
+      k_colscale<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, g->prb_acc, g->hb_norm,
+                                                      step == 0 ? g->hb_hub : g->hb_auth);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  k_cols_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->hb_hub, oh);
+  k_cols_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->hb_auth, oa);
+  g->kend();
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE)) {
+    HIP_TRY(hipMemcpyAsync(hub, oh, sizeof(float) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(auth, oa, sizeof(float) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
+  }
   return g->check_err();
 }
 

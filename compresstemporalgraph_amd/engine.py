@@ -84,6 +84,7 @@ def lib():
         L.tga_pagerank.argtypes = [vp, Window, C.c_int32, C.c_float, C.c_float, vp, p32, C.c_uint32]
         L.tga_pagerank_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, C.c_float, vp, C.c_uint32]
         L.tga_hits.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
+        L.tga_hits_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, vp, vp, C.c_uint32]
         L.tga_profile.argtypes = [vp, C.c_int]
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
         _LIB = L
@@ -289,6 +290,22 @@ class TemporalGraph:
             arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
         _check(lib().tga_pagerank_batch(self.h, wins, nq, iters, damping, ptr, flags))
         return arr
+
+    def hits_batch(self, windows, iters: int = 20, out_hub=None, out_auth=None):
+        """Batched HITS: up to 64 windows together (FP32 columns).  Returns
+        (hub, auth), each (nq, n) float32, row q equal to hits(windows[q], iters)
+        up to FP64 summation order; out_hub / out_auth may be (nq, n) device
+        tensors instead."""
+        nq = len(windows)
+        wins = (Window * nq)(*[_win(w) for w in windows])
+        if out_hub is None:
+            hub, auth = np.empty((nq, self.n), np.float32), np.empty((nq, self.n), np.float32)
+            ph, pa, flags = hub.ctypes.data_as(C.c_void_p), auth.ctypes.data_as(C.c_void_p), 0
+        else:
+            hub, auth = out_hub, out_auth
+            ph, pa, flags = C.c_void_p(hub.data_ptr()), C.c_void_p(auth.data_ptr()), TGA_OUT_DEVICE
+        _check(lib().tga_hits_batch(self.h, wins, nq, iters, ph, pa, flags))
+        return hub, auth
 
     def hits(self, window=None, iters: int = 20, out_hub=None, out_auth=None):
         h = _Out(self.n, np.float32, out_hub)

@@ -209,6 +209,15 @@ int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32_t nq, int3
  * (BaselineRunner.java:472-550; replaces hits.cu:1898-1987, :2284-2349). */
 int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, float* auth, uint32_t flags);
 
+/* Batched HITS: nq (1..64) windows, `iters` iterations each; hub and auth
+ * receive nq rows of n: row q equals tga_hits(g, windows[q], iters, ...) up
+ * to FP64 summation order.  FP32 hub and authority columns over the run
+ * structure of tga_pagerank_batch (authority over destination-sorted runs,
+ * hub over the same runs grouped by source), no atomics.  No counterpart in
+ * the reference (one window per executable run). */
+int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t iters, float* hub, float* auth,
+                   uint32_t flags);
+
 /* Kernel timing on the handle's own stream: while enabled, every launch of a
  * decode kernel (the kernels that walk the compressed streams) is bracketed by
  * HIP events, and tga_profile_read returns the summed elapsed milliseconds and

@@ -89,7 +89,10 @@ def test_active_batched(act):
     cnt, cen = g.degree_centrality_batch(wins)
     lab = g.connected_components_batch(wins)
     pr = g.pagerank_batch(wins, iters=20)
+    hub, auth = g.hits_batch(wins, iters=10)
     for q, w in enumerate(wins):
+        h32, a32 = o.hits(10, window=w, active=rng, dtype=np.float32)
+        assert rel_linf(hub[q], h32) <= REL_TOL and rel_linf(auth[q], a32) <= REL_TOL, f"hits {q}"
         np.testing.assert_array_equal(lv[q], o.bfs(srcs[q], window=w, active=rng), err_msg=f"bfs {q}")
         ocnt, ocen = o.degree(window=w, active=rng, centrality=True)
         np.testing.assert_array_equal(cnt[q], ocnt, err_msg=f"dc {q}")

@@ -295,6 +295,36 @@ def test_pagerank_tol(pair, path, monkeypatch):
     assert rel_linf(got, ref) <= REL_TOL * 10
 
 
+def test_hits_batch(pair):
+    """Up to 64 windows together: every row within the HITS bar of the
+    oracle's single-window FP32 result and within 1e-6 of the engine's."""
+    ds, g = pair
+    rng = np.random.default_rng(29)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    wins = []
+    for q in range(64):
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))
+    hub, auth = g.hits_batch(wins, iters=20)
+    o = ds.oracle()
+    for q, w in enumerate(wins):
+        h32, a32 = o.hits(20, window=w, dtype=np.float32)
+        assert rel_linf(hub[q], h32) <= REL_TOL, f"hub, window {q}"
+        assert rel_linf(auth[q], a32) <= REL_TOL, f"auth, window {q}"
+        if q < 8:
+            h1, a1 = g.hits(w, iters=20)
+            assert rel_linf(hub[q], h1) <= 1e-6 and rel_linf(auth[q], a1) <= 1e-6, f"window {q}"
+
+
 @PATHS
 def test_hits(pair, path, monkeypatch):
     ds, g = pair
