@@ -24,6 +24,7 @@
  * in the module docstring of oracle/oracle.py.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -266,7 +267,7 @@ static int decode_list(orc* g, int64_t x, ivec* out, int depth, uint64_t* outdeg
   }
   if (extra > 0) { /* residuals */
     int64_t v = nat2int(rd_zeta(&b, 3)) + x;
-    if (depth == 0 && v == 0 && extra > 1) g->q1_records++;
+    if (depth == 0 && v == 0 && extra > 1) __atomic_fetch_add(&g->q1_records, 1, __ATOMIC_RELAXED);
     iv_push(out, (int32_t)v);
     for (int64_t i = 1; i < extra; i++) {
       v += (int64_t)rd_zeta(&b, 3) + 1;
@@ -370,6 +371,70 @@ int orc_neighbor_access(orc* g, const int32_t* verts, int64_t nverts, int32_t ab
   *checksum = cs;
   *contacts = cnt;
   return 0;
+}
+
+/* orc_neighbor_access over `threads` POSIX threads (blocks of 1024 list
+ * entries dealt round-robin; the checksum is an order-independent sum, so it
+ * equals the single-thread result).  Only for timing the CPU at all cores. */
+typedef struct {
+  orc* g;
+  const int32_t* verts;
+  int64_t nverts, count, ab;
+  int t, threads, rc;
+  uint64_t cs;
+  int64_t cnt;
+} na_job;
+
+static void* na_worker(void* p) {
+  na_job* j = (na_job*)p;
+  ivec lst = {0};
+  for (int64_t b = (int64_t)j->t * 1024; b < j->count; b += (int64_t)j->threads * 1024) {
+    const int64_t e = b + 1024 < j->count ? b + 1024 : j->count;
+    for (int64_t i = b; i < e; i++) {
+      const int64_t x = j->nverts > 0 ? j->verts[i] : j->ab + i;
+      if (decode_list(j->g, x, &lst, 0, NULL)) {
+        j->rc = -1;
+        free(lst.v);
+        return NULL;
+      }
+      uint64_t goff, toff;
+      orc_offsets(j->g, x, &goff, &toff);
+      bitrd tb = {j->g->ts, j->g->ts_len * 8, toff, 0};
+      int64_t prev = j->g->min_ts;
+      for (int64_t k = 0; k < lst.n; k++) {
+        prev += nat2int(rd_zeta(&tb, 2));
+        j->cs += orc_contact_hash((int32_t)x, lst.v[k], (int32_t)prev);
+      }
+      j->cnt += lst.n;
+    }
+  }
+  free(lst.v);
+  return NULL;
+}
+
+int orc_neighbor_access_mt(orc* g, const int32_t* verts, int64_t nverts, int32_t ab, int32_t ae, int threads,
+                           uint64_t* checksum, int64_t* contacts) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  na_job jobs[256];
+  pthread_t tid[256];
+  const int64_t count = nverts > 0 ? nverts : (int64_t)(ae - ab);
+  for (int t = 0; t < threads; t++) {
+    jobs[t] = (na_job){g, verts, nverts, count, ab, t, threads, 0, 0, 0};
+    if (pthread_create(&tid[t], NULL, na_worker, &jobs[t])) return -1;
+  }
+  uint64_t cs = 0;
+  int64_t cnt = 0;
+  int rc = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(tid[t], NULL);
+    cs += jobs[t].cs;
+    cnt += jobs[t].cnt;
+    rc |= jobs[t].rc;
+  }
+  *checksum = cs;
+  *contacts = cnt;
+  return rc;
 }
 
 /* Degree centrality (degree_centrality.cu:1444-1453; BaselineRunner.java:300-333):

@@ -76,6 +76,7 @@ def lib():
         L.orc_contact_hash.argtypes = [C.c_int32, C.c_int32, C.c_int32]
         L.orc_contact_hash.restype = C.c_uint64
         L.orc_neighbor_access.argtypes = [C.c_void_p, p32, C.c_int64, C.c_int32, C.c_int32, pu64, p64]
+        L.orc_neighbor_access_mt.argtypes = [C.c_void_p, p32, C.c_int64, C.c_int32, C.c_int32, C.c_int, pu64, p64]
         Q = C.POINTER(_Q)
         L.orc_degree.argtypes = [C.c_void_p, Q, p32, p32]
         L.orc_bfs.argtypes = [C.c_void_p, Q, C.c_int32, p32]
@@ -176,6 +177,15 @@ class Oracle:
             v = np.ascontiguousarray(verts, np.int32)
             nv = len(v)
         self._chk(lib().orc_neighbor_access(self.h, _p(v, C.c_int32), nv, ab, ae, C.byref(cs), C.byref(cnt)))
+        return cs.value, cnt.value
+
+    def neighbor_access_mt(self, verts, threads, active=None):
+        """neighbor_access over `threads` threads (for timing only)."""
+        ab, ae = active if active is not None else (0, self.n)
+        cs, cnt = C.c_uint64(), C.c_int64()
+        v = np.ascontiguousarray(verts, np.int32)
+        self._chk(lib().orc_neighbor_access_mt(self.h, _p(v, C.c_int32), len(v), ab, ae, threads, C.byref(cs),
+                                               C.byref(cnt)))
         return cs.value, cnt.value
 
     def degree(self, window=None, active=None, centrality=False):

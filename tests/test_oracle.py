@@ -62,6 +62,16 @@ def test_neighbor_access(ds):
     assert keep.any()
 
 
+def test_neighbor_access_threads(ds):
+    """The threaded variant (used only to time the CPU at all cores) returns
+    the single-thread checksum and count, with any thread count."""
+    o = ds.oracle()
+    verts = np.random.default_rng(4).integers(0, ds.n, 3000).astype(np.int32)
+    want = o.neighbor_access(verts)
+    for threads in (1, 3, 8):
+        assert o.neighbor_access_mt(verts, threads) == want
+
+
 def test_algorithms_match_bruteforce(ds):
     o = ds.oracle()
     prio = fmt.java_random_ints(123, 256, ds.n)
