@@ -1915,6 +1915,42 @@ __global__ void __launch_bounds__(kBlock) k_ea_fix(const Ckpt* ck, int32_t nck, 
   }
 }
 
+// Latest departure to a target: a window contact (x, r, t) between active
+// vertices gives x the departure t when t <= ld[r].  Every item pulls the best
+// such t over its contacts; a vertex split into chunks merges them with an
+// atomic maximum.  Values only rise and each is met by a real path, so the
+// rounds reach the same fixpoint whatever order the waves run in.
+struct LdAct {
+  Window W;
+  int32_t* ld;
+  uint32_t* changed;
+  int32_t best = INT32_MIN;
+  __device__ bool begin(int32_t x, bool, Split) {
+    if (!W.active(x)) return false;
+    best = ld[x];
+    return true;
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
+    if (t > best && W.in(t) && W.active(r) && t <= ld[r]) best = t;
+    return true;
+  }
+  __device__ void end(int32_t x, bool chunked) {
+    if (chunked) {
+      if (best > ld[x] && atomicMax(&ld[x], best) < best) *changed = 1u;
+    } else if (best > ld[x]) {
+      ld[x] = best;
+      *changed = 1u;
+    }
+  }
+};
+
+template <int MAXD>
+__global__ void __launch_bounds__(kBlock) k_ld(Streams S, const Ckpt* ck, Items I, Window W, int32_t* ld,
+                                               uint32_t* changed) {
+  LdAct a{W, ld, changed};
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+}
+
 // Window contact count (PageRank divisor).
 struct WdegAct {
   Window W;
@@ -3720,6 +3756,29 @@ extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int
     }
   }
   int rc = g->out_copy(arrival, g->i0, (size_t)g->n * 4, flags);
+  if (rc) return rc;
+  return g->check_err();
+}
+
+extern "C" int tga_latest_departure(tga_graph* g, tga_window w, int32_t target, int32_t* departure,
+                                    int32_t* rounds, uint32_t flags) {
+  if (!g || !departure || bad_window(w) || target < 0 || target >= g->n)
+    return fail(TGA_EINVAL, "tga_latest_departure: bad arguments");
+  HIP_TRY(hipSetDevice(g->device));
+  const Window W = g->window(w);
+  const int32_t d0 = (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(w.end, INT32_MAX));
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, INT32_MIN);
+  HIP_TRY(hipMemcpyAsync(g->i0 + target, &d0, 4, hipMemcpyHostToDevice, g->stream));
+  uint32_t* changed = (uint32_t*)g->u64s;
+  int32_t round = 0;
+  for (uint32_t more = 1; more; ++round) {
+    HIP_TRY(hipMemsetAsync(changed, 0, sizeof(uint32_t), g->stream));
+    LAUNCH_STATIC(k_ld, W, g->i0, changed);
+    HIP_TRY(hipMemcpyAsync(&more, changed, sizeof more, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
+  if (rounds) *rounds = round;
+  int rc = g->out_copy(departure, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
   return g->check_err();
 }

@@ -78,6 +78,7 @@ def lib():
         L.tga_bfs_server_close.argtypes = [vp]
         L.tga_bfs_server_close.restype = None
         L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
+        L.tga_latest_departure.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
         L.tga_connected_components_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, vp, C.c_uint32]
         L.tga_graph_coloring.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
@@ -237,6 +238,16 @@ class TemporalGraph:
         o = _Out(self.n, np.int32, out)
         _check(lib().tga_earliest_arrival(self.h, _win(window), src, o.ptr, o.flags))
         return o.arr
+
+    def latest_departure(self, target: int, window=None, out=None, rounds=False):
+        """Latest departure to `target`: per vertex, the latest first-contact
+        time of a time-respecting path to the target inside the window
+        (INT32_MIN where none exists; the window end at the target itself).
+        With rounds=True also returns the number of pull passes run."""
+        o = _Out(self.n, np.int32, out)
+        nr = C.c_int32(0)
+        _check(lib().tga_latest_departure(self.h, _win(window), target, o.ptr, C.byref(nr), o.flags))
+        return (o.arr, nr.value) if rounds else o.arr
 
     def connected_components(self, window=None, out=None):
         o = _Out(self.n, np.int32, out)

@@ -163,6 +163,17 @@ void tga_bfs_server_close(tga_bfs_server* s);
  * arrival[x] <= t < arrival[r]; INT32_MAX when unreached. */
 int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arrival, uint32_t flags);
 
+/* Latest departure to `target` (the time-reversed counterpart of earliest
+ * arrival; the reference has no such entry point, its temporal traversal is
+ * bfs.cu:1452-1471's forward rule): departure[target] = the window end clamped
+ * to int32; departure[x] = the latest t of a contact x -t-> x1 that starts a
+ * path to the target over window contacts between active vertices with
+ * non-decreasing times; INT32_MIN where the target is unreachable.  `rounds`
+ * (may be NULL) receives the number of pull passes run, the last one changing
+ * nothing. */
+int tga_latest_departure(tga_graph* g, tga_window w, int32_t target, int32_t* departure, int32_t* rounds,
+                         uint32_t flags);
+
 /* Connected components (min-label): label[v] = smallest active u that reaches
  * v over window contacts, -1 for inactive v (equals the component order of
  * BaselineRunner.java:553-608; replaces connected_components.cu:2091-2154). */

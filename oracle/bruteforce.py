@@ -53,6 +53,7 @@ class Raw:
         self.active = np.zeros(n, bool)
         self.active[ab:min(ae, n)] = True
         self.src, self.dst, self.t = src, dst, t
+        self.window = (ws, we)
         self.inw = (t >= ws) & (t <= we)
         # window edges between active vertices
         keep = self.inw & self.active[src] & self.active[dst]
@@ -82,6 +83,26 @@ class Raw:
             level[nxt] = lvl
             frontier = nxt
         return level.astype(np.int32)
+
+    # -- latest departure to target z: the latest t1 of a path x -t1-> x1 -t2->
+    #    ... -> z over window contacts between active vertices with
+    #    t1 <= t2 <= ...; ld[z] = the window end (clamped to int32, also for an
+    #    inactive z, whose in-contacts are dropped); INT32_MIN where z is
+    #    unreachable.  Fixpoint: ld[x] = max over contacts (x, r, t)
+    #    with ld[r] reached and t <= ld[r] of t.
+    def latest_departure(self, z):
+        neg = -(1 << 31)
+        ld = np.full(self.n, neg, np.int64)
+        ld[z] = min(self.window[1], (1 << 31) - 1)
+        while True:
+            lr = ld[self.edst]
+            ok = (lr != neg) & (self.et <= lr)
+            cand = np.full(self.n, neg, np.int64)
+            np.maximum.at(cand, self.esrc[ok], self.et[ok])
+            new = np.maximum(ld, cand)
+            if np.array_equal(new, ld):
+                return ld.astype(np.int32)
+            ld = new
 
     # -- CC min-label by fixpoint propagation
     def cc(self):

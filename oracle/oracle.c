@@ -523,6 +523,32 @@ int orc_earliest_arrival(orc* g, const orc_q* q, int32_t src, int32_t* arrival) 
   return 0;
 }
 
+/* Latest departure to target z (the time-reversed counterpart of earliest
+ * arrival over every window contact, no first-contact rule; not in the
+ * reference's algorithm set): ld[z] = we (clamped to int32); a window contact
+ * (x, r, t) between active vertices gives x the departure t when t <= ld[r];
+ * ld = INT32_MIN where z is unreachable.  Fixpoint of the maximum. */
+int orc_latest_departure(orc* g, const orc_q* q, int32_t z, int32_t* ld) {
+  int64_t n = g->n;
+  for (int64_t v = 0; v < n; v++) ld[v] = INT32_MIN;
+  if (z < 0 || z >= n) return fail("target out of range", z);
+  ld[z] = (int32_t)(q->we > INT32_MAX ? INT32_MAX : q->we < INT32_MIN ? INT32_MIN : q->we);
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (int64_t x = 0; x < n; x++) {
+      if (!act(g, q, x)) continue;
+      int32_t best = ld[x];
+      for (int64_t e = g->start[x]; e < g->start[x + 1]; e++) {
+        int32_t r = g->adj[e], t = g->tim[e];
+        if (in_w(q, t) && act(g, q, r) && t <= ld[r] && t > best) best = t;
+      }
+      if (best > ld[x]) { ld[x] = best; changed = 1; }
+    }
+  }
+  return 0;
+}
+
 /* Connected components as min-label: label[v] = min{u active : u reaches v}
  * over window contacts (equals BaselineRunner.java:553-608's component ids up
  * to the monotone relabelling start-vertex -> component number). */

@@ -81,6 +81,7 @@ def lib():
         L.orc_degree.argtypes = [C.c_void_p, Q, p32, p32]
         L.orc_bfs.argtypes = [C.c_void_p, Q, C.c_int32, p32]
         L.orc_earliest_arrival.argtypes = [C.c_void_p, Q, C.c_int32, p32]
+        L.orc_latest_departure.argtypes = [C.c_void_p, Q, C.c_int32, p32]
         L.orc_cc.argtypes = [C.c_void_p, Q, p32]
         L.orc_gc.argtypes = [C.c_void_p, Q, p32, C.c_int32, p32]
         L.orc_pagerank_f64.argtypes = [C.c_void_p, Q, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_double)]
@@ -206,6 +207,12 @@ class Oracle:
         self.csr()
         out = np.empty(self.n, np.int32)
         self._chk(lib().orc_earliest_arrival(self.h, self._q(window, active), src, _p(out, C.c_int32)))
+        return out
+
+    def latest_departure(self, z, window=None, active=None):
+        self.csr()
+        out = np.empty(self.n, np.int32)
+        self._chk(lib().orc_latest_departure(self.h, self._q(window, active), z, _p(out, C.c_int32)))
         return out
 
     def cc(self, window=None, active=None):

@@ -77,6 +77,7 @@ def test_active_frontier_algorithms(act):
         for s in sources(ds, rng):
             np.testing.assert_array_equal(g.bfs(s, w), o.bfs(s, window=w, active=rng))
             np.testing.assert_array_equal(g.earliest_arrival(s, w), o.earliest_arrival(s, window=w, active=rng))
+            np.testing.assert_array_equal(g.latest_departure(s, w), o.latest_departure(s, window=w, active=rng))
 
 
 def test_active_batched(act):

@@ -188,6 +188,17 @@ def test_earliest_arrival(pair):
         np.testing.assert_array_equal(g.earliest_arrival(s, w), ds.oracle().earliest_arrival(s, window=w))
 
 
+def test_latest_departure(pair):
+    ds, g = pair
+    o = ds.oracle()
+    hub = int(np.bincount(ds.dst).argmax())
+    for w in windows_for(ds):
+        for z in (int(ds.dst[0]), hub, int(ds.src[len(ds.src) // 2])):
+            got, rounds = g.latest_departure(z, w, rounds=True)
+            np.testing.assert_array_equal(got, o.latest_departure(z, window=w), err_msg=f"{w} {z}")
+            assert rounds >= 1
+
+
 def test_connected_components(pair):
     ds, g = pair
     for w in windows_for(ds):

@@ -187,6 +187,52 @@ def test_bruteforce_coloring_priority_invariant():
             assert col[x] == want
 
 
+def test_latest_departure_is_exhaustive_path_search():
+    """Latest departure on tiny random graphs equals the best departure over
+    every simple time-respecting path found by exhaustive search (windows and
+    an active range included).  Simple paths suffice: cutting a cycle out of a
+    time-respecting path leaves one with the same first contact."""
+    rng = np.random.default_rng(31)
+    for trial in range(60):
+        n = 6
+        m = int(rng.integers(4, 22))
+        src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+        t = rng.integers(0, 12, m)
+        win = None if trial % 3 == 0 else tuple(sorted(int(v) for v in rng.integers(0, 12, 2)))
+        act = None if trial % 2 == 0 else (1, 5)
+        R = B.Raw(src, dst, t, n, window=win, active=act)
+        edges = list(zip(R.esrc.tolist(), R.edst.tolist(), R.et.tolist()))
+        z = int(rng.integers(0, n))
+
+        def reach(v, after, seen):  # a simple path v -> z leaving at or after `after`
+            return v == z or any(a == v and tt >= after and b not in seen and reach(b, tt, seen | {b})
+                                 for a, b, tt in edges)
+
+        want = np.full(n, -(1 << 31), np.int64)
+        want[z] = R.window[1]
+        for x in range(n):
+            best = [tt for a, b, tt in edges if a == x != z and b != x and reach(b, tt, {x, b})]
+            if best:
+                want[x] = max(best)
+        np.testing.assert_array_equal(R.latest_departure(z), want.astype(np.int32), err_msg=f"trial {trial}")
+
+
+def test_latest_departure_oracle(tmp_path):
+    """The C oracle's latest departure equals the brute force (itself pinned
+    by exhaustive path search) over windows, an active range and targets."""
+    (s, d, t), n = _small_graph(seed=21, n=300, m=3000)
+    ds = DS(str(tmp_path), "ld", s, d, t, n_min=n)
+    o = ds.oracle()
+    lo, hi = int(t.min()), int(t.max())
+    for w in (None, (lo + (hi - lo) // 4, lo + (hi - lo) * 3 // 4), (lo, lo)):
+        for act in (None, (n // 10, n - n // 7)):
+            for z in (int(d[0]), int(d[len(d) // 2]), n // 10):
+                R = B.Raw(s, d, t, n, window=w, active=act)
+                got = o.latest_departure(z, window=w, active=act)
+                np.testing.assert_array_equal(got, R.latest_departure(z), err_msg=f"{w} {act} {z}")
+                assert (got != -(1 << 31)).sum() > 1 or w == (lo, lo) or act is not None
+
+
 def test_earliest_arrival_small(tmp_path):
     """Earliest arrival by exhaustive relaxation over first window contacts."""
     (s, d, t), n = _small_graph(seed=16, n=200, m=2000)
