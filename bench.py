@@ -18,8 +18,9 @@ Prints one JSON line with the driver's contract fields plus:
   cpu_baseline  the CPU oracle's neighbor access (single thread, the build's
                 C restatement of the reference reader, kind "port") timed on
                 this host over a bounded vertex sample.
-  workloads_ms  the seven entry points (plus earliest arrival) over windows
-                covering 100%, 50% and 10% of the timestamp span.
+  workloads_ms  the seven entry points (plus earliest arrival and latest
+                departure) over windows covering 100%, 50% and 10% of the
+                timestamp span.
   batched_bfs   64 windowed BFS queries in one batched call against 64 plain
                 calls (results compared, outputs in HBM).
   batched_dc    64 windowed degree-centrality queries in one batched call
@@ -87,7 +88,8 @@ def workload_times(g, t_lo: int, t_hi: int, sources: int = 8) -> dict:
     10%, 50% and 100% of the timestamp span (the at-scale plain
     configuration): BFS from `sources` seeded sources (mean), CC, PR (20
     iterations, FP32), HITS (20 iterations, FP32), DC (counts and the Java
-    centrality), GC and RA; plus the earliest-arrival rule.  Each call is
+    centrality), GC and RA; plus the earliest-arrival rule and latest
+    departure (to the same first source).  Each call is
     timed once after one warm call."""
     import numpy as np
     rng = np.random.default_rng(11)
@@ -108,6 +110,7 @@ def workload_times(g, t_lo: int, t_hi: int, sources: int = 8) -> dict:
                "DC": timed(lambda: g.degree_centrality(w, centrality=True)),
                "BFS": round(sum(timed(lambda s=s: g.bfs(s, w)) for s in srcs) / len(srcs), 3),
                "EA": timed(lambda: g.earliest_arrival(srcs[0], w)),
+               "LD": timed(lambda: g.latest_departure(srcs[0], w)),
                "CC": timed(lambda: g.connected_components(w)),
                "GC": timed(lambda: g.graph_coloring(w, rounds=512)),
                "PR": timed(lambda: g.pagerank(w, iters=20)),
