@@ -1915,11 +1915,15 @@ __global__ void __launch_bounds__(kBlock) k_ea_fix(const Ckpt* ck, int32_t nck, 
   }
 }
 
-// Latest departure to a target: a window contact (x, r, t) between active
-// vertices gives x the departure t when t <= ld[r].  Every item pulls the best
-// such t over its contacts; a vertex split into chunks merges them with an
-// atomic maximum.  Values only rise and each is met by a real path, so the
-// rounds reach the same fixpoint whatever order the waves run in.
+// Latest departure to a target (temporal_ligra LatestDeparture-Compute.C:27-51
+// rule over the transposed graph, with instantaneous contacts): a window
+// contact (x, r, t) between active vertices with r reached gives x the
+// departure t when t <= ld[r].  Every item pulls the best such t over its
+// contacts; a vertex split into chunks merges them with an atomic maximum.
+// Values only rise and each is met by a real path, so the rounds reach the
+// same fixpoint as the reference's frontier loop whatever order the waves run
+// in.  Unreached vertices keep the reference's -(INT_MAX / 2) (:116-119).
+constexpr int32_t kLdUnreached = -(INT32_MAX / 2);
 struct LdAct {
   Window W;
   int32_t* ld;
@@ -1931,7 +1935,10 @@ struct LdAct {
     return true;
   }
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
-    if (t > best && W.in(t) && W.active(r) && t <= ld[r]) best = t;
+    if (t > best && W.in(t) && W.active(r)) {
+      const int32_t lr = ld[r];
+      if (lr != kLdUnreached && t <= lr) best = t;
+    }
     return true;
   }
   __device__ void end(int32_t x, bool chunked) {
@@ -3767,7 +3774,7 @@ extern "C" int tga_latest_departure(tga_graph* g, tga_window w, int32_t target, 
   HIP_TRY(hipSetDevice(g->device));
   const Window W = g->window(w);
   const int32_t d0 = (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(w.end, INT32_MAX));
-  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, INT32_MIN);
+  k_fill<int32_t><<<g->grid(g->n), kBlock, 0, g->stream>>>(g->i0, g->n, kLdUnreached);
   HIP_TRY(hipMemcpyAsync(g->i0 + target, &d0, 4, hipMemcpyHostToDevice, g->stream));
   uint32_t* changed = (uint32_t*)g->u64s;
   int32_t round = 0;

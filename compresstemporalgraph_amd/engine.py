@@ -242,7 +242,9 @@ class TemporalGraph:
     def latest_departure(self, target: int, window=None, out=None, rounds=False):
         """Latest departure to `target`: per vertex, the latest first-contact
         time of a time-respecting path to the target inside the window
-        (INT32_MIN where none exists; the window end at the target itself).
+        (-1073741823 = -(INT32_MAX // 2) where none exists, the value temporal_ligra's
+        LatestDeparture-Compute.C:116-119 leaves; the window end at the target
+        itself).
         With rounds=True also returns the number of pull passes run."""
         o = _Out(self.n, np.int32, out)
         nr = C.c_int32(0)

@@ -163,14 +163,15 @@ void tga_bfs_server_close(tga_bfs_server* s);
  * arrival[x] <= t < arrival[r]; INT32_MAX when unreached. */
 int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arrival, uint32_t flags);
 
-/* Latest departure to `target` (the time-reversed counterpart of earliest
- * arrival; the reference has no such entry point, its temporal traversal is
- * bfs.cu:1452-1471's forward rule): departure[target] = the window end clamped
- * to int32; departure[x] = the latest t of a contact x -t-> x1 that starts a
- * path to the target over window contacts between active vertices with
- * non-decreasing times; INT32_MIN where the target is unreachable.  `rounds`
- * (may be NULL) receives the number of pull passes run, the last one changing
- * nothing. */
+/* Latest departure to `target` (replaces the reference's CPU comparison app
+ * third_party/temporal_ligra/apps/temporal/LatestDeparture-Compute.C:86-135,
+ * rule :27-51, with instantaneous contacts, start = end = t):
+ * departure[target] = the window end clamped to int32; departure[x] = the
+ * latest t of a contact x -t-> x1 that starts a path to the target over window
+ * contacts between active vertices with non-decreasing times; -(INT32_MAX / 2)
+ * (the reference's unset value, :116-119) where the target is unreachable.
+ * `rounds` (may be NULL) receives the number of pull passes run, the last one
+ * changing nothing. */
 int tga_latest_departure(tga_graph* g, tga_window w, int32_t target, int32_t* departure, int32_t* rounds,
                          uint32_t flags);
 

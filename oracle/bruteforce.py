@@ -12,6 +12,7 @@ import numpy as np
 
 INF32 = 0x7FFFFFFF
 FULL_WINDOW = (-(1 << 31), (1 << 31) - 1)
+LD_UNREACHED = -(((1 << 31) - 1) // 2)  # -1073741823
 
 _K1, _K2, _K3 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9
 
@@ -87,11 +88,12 @@ class Raw:
     # -- latest departure to target z: the latest t1 of a path x -t1-> x1 -t2->
     #    ... -> z over window contacts between active vertices with
     #    t1 <= t2 <= ...; ld[z] = the window end (clamped to int32, also for an
-    #    inactive z, whose in-contacts are dropped); INT32_MIN where z is
-    #    unreachable.  Fixpoint: ld[x] = max over contacts (x, r, t)
-    #    with ld[r] reached and t <= ld[r] of t.
+    #    inactive z, whose in-contacts are dropped); LD_UNREACHED where z is
+    #    unreachable (-1073741823, temporal_ligra LatestDeparture-Compute.C:116-119).
+    #    Fixpoint: ld[x] = max over contacts (x, r, t) with r reached and
+    #    t <= ld[r] of t.
     def latest_departure(self, z):
-        neg = -(1 << 31)
+        neg = LD_UNREACHED
         ld = np.full(self.n, neg, np.int64)
         ld[z] = min(self.window[1], (1 << 31) - 1)
         while True:

@@ -523,14 +523,17 @@ int orc_earliest_arrival(orc* g, const orc_q* q, int32_t src, int32_t* arrival) 
   return 0;
 }
 
-/* Latest departure to target z (the time-reversed counterpart of earliest
- * arrival over every window contact, no first-contact rule; not in the
- * reference's algorithm set): ld[z] = we (clamped to int32); a window contact
- * (x, r, t) between active vertices gives x the departure t when t <= ld[r];
- * ld = INT32_MIN where z is unreachable.  Fixpoint of the maximum. */
+/* Latest departure to target z (temporal_ligra
+ * apps/temporal/LatestDeparture-Compute.C:27-51 over the transposed graph,
+ * driver :99-135, with instantaneous contacts): ld[z] = we (clamped to
+ * int32); a window contact (x, r, t) between active vertices with r reached
+ * gives x the departure t when t <= ld[r]; unreached vertices keep the
+ * reference's -(INT_MAX / 2) (:116-119).  Fixpoint of the maximum, which the
+ * reference's frontier loop also reaches. */
+#define ORC_LD_UNREACHED (-(INT32_MAX / 2))
 int orc_latest_departure(orc* g, const orc_q* q, int32_t z, int32_t* ld) {
   int64_t n = g->n;
-  for (int64_t v = 0; v < n; v++) ld[v] = INT32_MIN;
+  for (int64_t v = 0; v < n; v++) ld[v] = ORC_LD_UNREACHED;
   if (z < 0 || z >= n) return fail("target out of range", z);
   ld[z] = (int32_t)(q->we > INT32_MAX ? INT32_MAX : q->we < INT32_MIN ? INT32_MIN : q->we);
   int changed = 1;
@@ -541,7 +544,7 @@ int orc_latest_departure(orc* g, const orc_q* q, int32_t z, int32_t* ld) {
       int32_t best = ld[x];
       for (int64_t e = g->start[x]; e < g->start[x + 1]; e++) {
         int32_t r = g->adj[e], t = g->tim[e];
-        if (in_w(q, t) && act(g, q, r) && t <= ld[r] && t > best) best = t;
+        if (in_w(q, t) && act(g, q, r) && ld[r] != ORC_LD_UNREACHED && t <= ld[r] && t > best) best = t;
       }
       if (best > ld[x]) { ld[x] = best; changed = 1; }
     }

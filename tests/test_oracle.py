@@ -208,7 +208,7 @@ def test_latest_departure_is_exhaustive_path_search():
             return v == z or any(a == v and tt >= after and b not in seen and reach(b, tt, seen | {b})
                                  for a, b, tt in edges)
 
-        want = np.full(n, -(1 << 31), np.int64)
+        want = np.full(n, B.LD_UNREACHED, np.int64)
         want[z] = R.window[1]
         for x in range(n):
             best = [tt for a, b, tt in edges if a == x != z and b != x and reach(b, tt, {x, b})]
@@ -230,7 +230,7 @@ def test_latest_departure_oracle(tmp_path):
                 R = B.Raw(s, d, t, n, window=w, active=act)
                 got = o.latest_departure(z, window=w, active=act)
                 np.testing.assert_array_equal(got, R.latest_departure(z), err_msg=f"{w} {act} {z}")
-                assert (got != -(1 << 31)).sum() > 1 or w == (lo, lo) or act is not None
+                assert (got != B.LD_UNREACHED).sum() > 1 or w == (lo, lo) or act is not None
 
 
 def test_earliest_arrival_small(tmp_path):
