@@ -5,6 +5,10 @@ contact of the workload, with the compressed graph resident in HBM.  The
 workload is the plain at-scale configuration: Chung-Lu, exponent 2.1, 2^21
 vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks (``chunglu21``),
 generated and encoded by the format kit before the timed region.
+``--workload netflow25`` runs the large single-GPU configuration instead
+(2^25 vertices, 2^30 contacts, streams of about 2.7 GB, beyond the 256 MB
+Infinity Cache): the RA pass, then plain BFS, CC and PR over the full and a 1%
+window; it takes a few minutes to generate and encode.
 
 Prints one JSON line with the driver's contract fields plus:
   roofline      the decode pass against MI355X HBM bandwidth (8 TB/s):
@@ -15,6 +19,9 @@ Prints one JSON line with the driver's contract fields plus:
                 `traffic` is the rocprof FETCH_SIZE + WRITE_SIZE per pass, scaled
                 by a copy-kernel calibration, from profiles/ when that
                 measurement was taken on the current engine sources (else null).
+  checksum      the decoded neighbor-access checksum, and whether it equals
+                the checksum of the raw contact list the dataset was encoded
+                from (a mismatch ends the run with an error).
   cpu_baseline  the CPU oracle's neighbor access (single thread, the build's
                 C restatement of the reference reader, kind "port") timed on
                 this host over a bounded vertex sample.
@@ -56,7 +63,17 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_ra_chunglu21.traffic.json")
+def traffic_file(workload: str) -> str:
+    return os.path.join(ROOT, "profiles", f"k_ra_{workload}.traffic.json")
+
+
+WORKLOAD_DESC = {
+    "rmat16": "R-MAT (0.57, 0.19, 0.19), 2^16 vertices, 2^20 contacts, Geometric contacts per pair, "
+              "uniform timestamps over 2^20 ticks",
+    "chunglu21": "Chung-Lu exponent 2.1, 2^21 vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks",
+    "netflow25": "netflow-like, 2^25 vertices, 2^30 contacts, power-law endpoints (exponent 2.5), Pareto "
+                 "contacts per pair (mean ~20), bursty Pareto(1.5) inter-arrivals over 2^30 ticks",
+}
 
 
 def engine_source_digest() -> str:
@@ -74,7 +91,7 @@ def measured_traffic(workload: str):
     """HBM bytes per pass from the committed rocprof FETCH_SIZE / WRITE_SIZE
     measurement, when it was taken on this workload with these engine sources."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(workload)) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
@@ -116,6 +133,33 @@ def workload_times(g, t_lo: int, t_hi: int, sources: int = 8) -> dict:
                "PR": timed(lambda: g.pagerank(w, iters=20)),
                "HITS": timed(lambda: g.hits(w, iters=20))}
         out[name] = row
+    return out
+
+
+def large_workload_times(g, t_lo: int, t_hi: int, sources: int = 4) -> dict:
+    """Milliseconds per call of plain BFS (mean over `sources` seeded
+    sources), CC and PR (20 iterations, FP32) over the full window and a 1%
+    window in the middle of the span, plus the RA pass (the large single-GPU
+    configuration).  Each call is timed once after one warm call."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    span = t_hi - t_lo
+    windows = {"100%": (t_lo, t_hi), "1%": (t_lo + span // 2, t_lo + span // 2 + span // 100)}
+    srcs = [int(v) for v in rng.integers(0, g.info["n"], sources)]
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        f()
+        return round((time.perf_counter() - t0) * 1e3, 3)
+
+    out = {}
+    for name, w in windows.items():
+        out[name] = {"RA": timed(lambda: g.neighbor_access()),
+                     "BFS": round(sum(timed(lambda s=s: g.bfs(s, w)) for s in srcs) / len(srcs), 3),
+                     "CC": timed(lambda: g.connected_components(w)),
+                     "PR": timed(lambda: g.pagerank(w, iters=20))}
+        log(f"[bench] {name} window: {out[name]}")
     return out
 
 
@@ -482,6 +526,7 @@ def main() -> int:
 
     root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
     t0 = time.perf_counter()
+    os.environ.setdefault("TGAF_PROGRESS", "1")  # the large graph encodes for minutes
     desc = datasets.ensure(root, args.workload)
     log(f"[bench] dataset {args.workload}: n={desc['max_vertex']} contacts={desc['contacts']} "
         f"ready in {time.perf_counter() - t0:.1f} s")
@@ -507,6 +552,11 @@ def main() -> int:
     assert cnt == contacts
     log(f"[bench] {args.steps} steps in {elapsed:.3f} s; decode kernels {kernel_ms / args.steps:.3f} ms per pass "
         f"({launches} launches); checksum {cs:#018x}")
+    # Size-independent parity: the decoded checksum equals the one of the raw
+    # contact list the dataset was encoded from.
+    raw_cs = desc.get("checksum")
+    if raw_cs is not None and cs != raw_cs:
+        raise SystemExit(f"decoded checksum {cs:#018x} differs from the raw list's {raw_cs:#018x}")
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = contacts * args.steps / elapsed
@@ -523,14 +573,16 @@ def main() -> int:
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
     workloads = None
+    large = args.workload == "netflow25"
     if not args.no_workloads:
         t0 = time.perf_counter()
-        workloads = workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
-        log(f"[bench] seven workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
+        times = large_workload_times if large else workload_times
+        workloads = times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
+        log(f"[bench] workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
 
     batched = bdc = bcc = bpr = bhits = served = smix = None
-    if not args.no_workloads and not args.no_query_legs:
+    if not args.no_workloads and not args.no_query_legs and not large:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
         # process, and whose streams share the hardware queues), so they run
@@ -571,12 +623,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: neighbor-access (RA) full decode pass; Chung-Lu exponent 2.1, "
-                               f"2^21 vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks",
+        "config": {"workload": f"{args.workload}: neighbor-access (RA) full decode pass; "
+                               f"{WORKLOAD_DESC.get(args.workload, args.workload)}",
                    "vertices": n, "contacts": contacts,
                    "graph_bytes": info["graph_bytes"], "timestamps_bytes": info["timestamps_bytes"],
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        "checksum": {"value": f"{cs:#018x}", "equals_raw_list": raw_cs is not None and cs == raw_cs},
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
         "batched_bfs": batched,

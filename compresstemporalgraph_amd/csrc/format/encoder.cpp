@@ -25,6 +25,7 @@
 #include <thread>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -293,6 +294,12 @@ static int write_indexes(const char* dir, const std::vector<uint64_t>& goff, uin
 
 using namespace tgaf;
 
+// Progress lines on stderr for long generations and encodings.
+static bool progress_on() {
+  const char* e = std::getenv("TGAF_PROGRESS");
+  return e && *e && *e != '0';
+}
+
 extern "C" const char* tgaf_last_error(void) { return g_err.c_str(); }
 
 extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, int64_t n_min,
@@ -323,7 +330,10 @@ extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t
   Plan cand, best;
   tgaf_stats s{};
   uint64_t prev_bits = 0;
+  const bool talk = progress_on();
   for (int64_t x = 0; x < n; ++x) {
+    if (talk && (x & ((1 << 22) - 1)) == 0 && x > 0)
+      std::fprintf(stderr, "[encode] graph: %lld/%lld vertices\n", (long long)x, (long long)n);
     goff[(size_t)x] = gw.bits();
     ow.gamma(gw.bits() - prev_bits);
     prev_bits = gw.bits();
@@ -376,6 +386,8 @@ extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t
   tw.put((uint64_t)tmin, 64);
   std::vector<uint64_t> toff((size_t)n + 1);
   for (int64_t x = 0; x < n; ++x) {
+    if (talk && (x & ((1 << 22) - 1)) == 0 && x > 0)
+      std::fprintf(stderr, "[encode] timestamps: %lld/%lld vertices\n", (long long)x, (long long)n);
     toff[(size_t)x] = tw.bits();
     int64_t prev = tmin;
     for (int64_t i = start[(size_t)x]; i < start[(size_t)x + 1]; ++i) {
@@ -546,14 +558,11 @@ extern "C" int tgaf_gen_rmat(int32_t scale, double a, double b, double c, double
   return TGAF_OK;
 }
 
-extern "C" int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int64_t period, uint64_t seed,
-                                 int64_t m, int32_t* src, int32_t* dst, int64_t* t) {
-  if (n < 2 || n > INT32_MAX || exponent <= 1.0 || t_span <= 0 || period <= 0 || m <= 0 || !src || !dst || !t)
-    return fail(TGAF_EINVAL, "tgaf_gen_chung_lu: bad arguments");
-  // Alias table (Vose) over w_i = (i+1)^(-1/(exponent-1)).
+// Alias table (Vose) over the power-law weights w_i = (i+1)^(-1/(exponent-1)).
+static void power_law_alias(int64_t n, double exponent, std::vector<double>& prob, std::vector<int32_t>& alias) {
   const double beta = 1.0 / (exponent - 1.0);
-  std::vector<double> prob((size_t)n);
-  std::vector<int32_t> alias((size_t)n, 0);
+  prob.assign((size_t)n, 0.0);
+  alias.assign((size_t)n, 0);
   {
     double total = 0.0;
     for (int64_t i = 0; i < n; ++i) total += (prob[(size_t)i] = std::pow((double)(i + 1), -beta));
@@ -575,6 +584,15 @@ extern "C" int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int
     for (int32_t i : large) prob[(size_t)i] = 1.0;
     for (int32_t i : small) prob[(size_t)i] = 1.0;
   }
+}
+
+extern "C" int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int64_t period, uint64_t seed,
+                                 int64_t m, int32_t* src, int32_t* dst, int64_t* t) {
+  if (n < 2 || n > INT32_MAX || exponent <= 1.0 || t_span <= 0 || period <= 0 || m <= 0 || !src || !dst || !t)
+    return fail(TGAF_EINVAL, "tgaf_gen_chung_lu: bad arguments");
+  std::vector<double> prob;
+  std::vector<int32_t> alias;
+  power_law_alias(n, exponent, prob, alias);
   // Fixed chunking with one generator per chunk: the output does not depend
   // on the number of threads.
   constexpr int64_t kChunks = 64;
@@ -613,6 +631,132 @@ extern "C" int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int
   for (unsigned i = 0; i < nth; ++i) pool.emplace_back(work);
   for (auto& th : pool) th.join();
   sort_contacts(m, src, dst, t);
+  return TGAF_OK;
+}
+
+extern "C" int tgaf_gen_netflow(int64_t n, double exponent, double count_min, double gap_min, int64_t t_span,
+                                uint64_t seed, int64_t m, int32_t* src, int32_t* dst, int64_t* t) {
+  if (n < 2 || n > INT32_MAX || exponent <= 1.0 || count_min < 1.0 || gap_min <= 0.0 || t_span <= 0 ||
+      t_span > ((int64_t)1 << 31) || m <= 0 || !src || !dst || !t)
+    return fail(TGAF_EINVAL, "tgaf_gen_netflow: bad arguments");
+  const bool talk = progress_on();
+  std::vector<double> prob;
+  std::vector<int32_t> alias;
+  power_law_alias(n, exponent, prob, alias);
+  // Pairs, drawn serially until their contact counts reach m (the last one
+  // cut to fit): both ends from the power-law weights, the count a Pareto
+  // (alpha 2) variate round(count_min * U^(-1/2)), mean about 2 count_min.
+  struct Pair {
+    uint64_t key;  // src << 32 | dst
+    int64_t cnt;
+  };
+  std::vector<Pair> pairs;
+  pairs.reserve((size_t)(m / (2.0 * count_min) * 1.1) + 16);
+  {
+    std::mt19937_64 rng(splitmix64(seed));
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    auto draw = [&]() -> uint32_t {
+      const double u = U(rng) * (double)n;
+      int64_t i = (int64_t)u;
+      if (i >= n) i = n - 1;
+      return (uint32_t)((U(rng) < prob[(size_t)i]) ? (int32_t)i : alias[(size_t)i]);
+    };
+    int64_t total = 0;
+    while (total < m) {
+      const uint32_t a = draw(), b = draw();
+      int64_t c = std::max<int64_t>(1, std::llround(count_min / std::sqrt(1.0 - U(rng))));
+      if (c > m - total) c = m - total;
+      pairs.push_back(Pair{(uint64_t)a << 32 | b, c});
+      total += c;
+    }
+  }
+  std::sort(pairs.begin(), pairs.end(), [](const Pair& x, const Pair& y) { return x.key < y.key; });
+  size_t np = 0;  // merge repeated pairs
+  for (size_t i = 0; i < pairs.size(); ++i) {
+    if (np > 0 && pairs[np - 1].key == pairs[i].key)
+      pairs[np - 1].cnt += pairs[i].cnt;
+    else
+      pairs[np++] = pairs[i];
+  }
+  pairs.resize(np);
+  std::vector<int64_t> off(np + 1, 0);
+  for (size_t i = 0; i < np; ++i) off[i + 1] = off[i] + pairs[i].cnt;
+  if (talk) std::fprintf(stderr, "[gen] %zu distinct pairs, %lld contacts\n", np, (long long)off[np]);
+  // Contacts of a pair: a burst from a uniform start with Pareto (alpha 1.5)
+  // gaps of at least gap_min, folded into [0, t_span) and sorted.  Fixed
+  // chunks with one generator each, so the output does not depend on the
+  // number of threads.
+  constexpr int64_t kChunks = 256;
+  std::atomic<int64_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int64_t c = next.fetch_add(1);
+      if (c >= kChunks) break;
+      std::mt19937_64 rng(splitmix64(seed * 0x100000001B3ULL + 0x5bd1e995ULL + (uint64_t)c));
+      std::uniform_real_distribution<double> U(0.0, 1.0);
+      const size_t p0 = np * (size_t)c / kChunks, p1 = np * (size_t)(c + 1) / kChunks;
+      for (size_t p = p0; p < p1; ++p) {
+        const int32_t a = (int32_t)(pairs[p].key >> 32), b = (int32_t)(pairs[p].key & 0xffffffffu);
+        int64_t tt = std::min<int64_t>((int64_t)(U(rng) * (double)t_span), t_span - 1);
+        bool wrapped = false;
+        for (int64_t k = off[p]; k < off[p + 1]; ++k) {
+          src[k] = a;
+          dst[k] = b;
+          t[k] = tt;
+          const double gap = gap_min * std::pow(1.0 - U(rng), -1.0 / 1.5);
+          tt += (int64_t)std::min(gap, (double)t_span);
+          if (tt >= t_span) {
+            tt %= t_span;
+            wrapped = true;
+          }
+        }
+        if (wrapped) std::sort(t + off[p], t + off[p + 1]);
+      }
+      if (talk && (c & 31) == 31) std::fprintf(stderr, "[gen] contact chunks %lld/%lld\n", (long long)c + 1, (long long)kChunks);
+    }
+  };
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned i = 0; i < nth; ++i) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  return TGAF_OK;
+}
+
+// The neighbor-access checksum of a raw contact list: the wrapping sum of
+// fmix64(x * K1 ^ r * K2 ^ t * K3) over its contacts, with x, r and t taken
+// as unsigned 32-bit values (the sum the decoding pass returns).
+extern "C" int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m,
+                                     uint64_t* out) {
+  if (m < 0 || (m > 0 && (!src || !dst || !t)) || !out) return fail(TGAF_EINVAL, "tgaf_contact_checksum: bad arguments");
+  constexpr int64_t kChunks = 64;
+  std::vector<uint64_t> part(kChunks, 0);
+  std::atomic<int64_t> next{0};
+  auto mix = [](uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+  };
+  auto work = [&]() {
+    for (;;) {
+      const int64_t c = next.fetch_add(1);
+      if (c >= kChunks) break;
+      uint64_t sum = 0;
+      for (int64_t i = m * c / kChunks; i < m * (c + 1) / kChunks; ++i)
+        sum += mix((uint64_t)(uint32_t)src[i] * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(uint32_t)dst[i] * 0xC2B2AE3D27D4EB4FULL ^
+                   (uint64_t)(uint32_t)(int32_t)t[i] * 0x165667B19E3779F9ULL);
+      part[(size_t)c] = sum;
+    }
+  };
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned i = 0; i < nth; ++i) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  uint64_t sum = 0;
+  for (uint64_t v : part) sum += v;
+  *out = sum;
   return TGAF_OK;
 }
 

@@ -16,6 +16,12 @@ Workloads (seeded, deterministic):
 * ``chunglu21`` — Chung-Lu, exponent 2.1, 2^21 vertices, 2^25 contacts,
   timestamps i.i.d. from the diurnal intensity 1 + 0.5 sin(2 pi t / 2^18) over
   2^24 ticks, seed 2.  The plain at-scale case.
+* ``netflow25`` — netflow-like, 2^25 vertices, 2^30 contacts: pairs from
+  power-law weights (exponent 2.5 on both ends, so the largest out-degree is
+  about 10^6 contacts), a Pareto (alpha 2) number of contacts per pair with
+  mean about 20, bursty Pareto (alpha 1.5) inter-arrival gaps of at least 256
+  ticks over 2^30 ticks, seed 4.  The large single-GPU case: its streams
+  (about 2.5 GB) exceed the 256 MB Infinity Cache, so a pass reads HBM.
 """
 from __future__ import annotations
 
@@ -42,6 +48,7 @@ class Workload:
 WORKLOADS = {
     "rmat16": Workload("rmat16", "rmat", 1 << 16, 1 << 20, 1, 1 << 20, True),
     "chunglu21": Workload("chunglu21", "chung_lu", 1 << 21, 1 << 25, 2, 1 << 24, False),
+    "netflow25": Workload("netflow25", "netflow", 1 << 25, 1 << 30, 4, 1 << 30, False),
 }
 
 
@@ -59,6 +66,8 @@ def generate(w: Workload):
         return fmt.gen_rmat(scale, w.m, t_span=w.t_span, seed=w.seed)
     if w.kind == "chung_lu":
         return fmt.gen_chung_lu(w.n, w.m, exponent=2.1, t_span=w.t_span, period=w.t_span // 64, seed=w.seed)
+    if w.kind == "netflow":
+        return fmt.gen_netflow(w.n, w.m, exponent=2.5, count_min=10.0, gap_min=256.0, t_span=w.t_span, seed=w.seed)
     raise ValueError(w.kind)
 
 
@@ -80,6 +89,7 @@ def write_dataset(root: str, name: str, src, dst, t, n_min: int = 0, raw: bool =
         "contacts": st.arcs,
         "t_min": int(np.min(t)) if len(t) else 0,
         "t_max": int(np.max(t)) if len(t) else 0,
+        "checksum": fmt.contact_checksum(src, dst, t),
         "stats": asdict(st),
     }
     with open(os.path.join(d, name + ".dataset.json"), "w") as f:

@@ -65,6 +65,9 @@ def _lib():
                                       C.c_uint64, C.c_int64, p32, p32, p64]
         lib.tgaf_gen_chung_lu.argtypes = [C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_uint64, C.c_int64,
                                           p32, p32, p64]
+        lib.tgaf_gen_netflow.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_uint64,
+                                         C.c_int64, p32, p32, p64]
+        lib.tgaf_contact_checksum.argtypes = [p32, p32, p64, C.c_int64, C.POINTER(C.c_uint64)]
         lib.tgaf_write_raw_gz.argtypes = [C.c_char_p, p32, p32, p64, C.c_int64]
         lib.tgaf_read_raw_gz.argtypes = [C.c_char_p, C.c_int, p64, p32, p32, p64]
         lib.tgaf_decode_table.argtypes = [C.c_int, p32]
@@ -124,6 +127,27 @@ def gen_chung_lu(n: int, m: int, exponent=2.1, t_span=1 << 24, period=(1 << 24) 
     _check(_lib().tgaf_gen_chung_lu(n, exponent, t_span, period, seed, m, _p(src, C.c_int32), _p(dst, C.c_int32),
                                     _p(t, C.c_int64)))
     return src, dst, t
+
+
+def gen_netflow(n: int, m: int, exponent=2.5, count_min=10.0, gap_min=256.0, t_span=1 << 30, seed=4):
+    src = np.empty(m, np.int32)
+    dst = np.empty(m, np.int32)
+    t = np.empty(m, np.int64)
+    _check(_lib().tgaf_gen_netflow(n, exponent, count_min, gap_min, t_span, seed, m, _p(src, C.c_int32),
+                                   _p(dst, C.c_int32), _p(t, C.c_int64)))
+    return src, dst, t
+
+
+def contact_checksum(src, dst, t) -> int:
+    """The neighbor-access checksum of a raw contact list (what the decoding
+    pass returns for its encoding)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    t = np.ascontiguousarray(t, dtype=np.int64)
+    out = C.c_uint64()
+    _check(_lib().tgaf_contact_checksum(_p(src, C.c_int32), _p(dst, C.c_int32), _p(t, C.c_int64), len(src),
+                                        C.byref(out)))
+    return out.value
 
 
 def write_raw_gz(path: str, src, dst, t) -> None:

@@ -78,6 +78,20 @@ int tgaf_gen_rmat(int32_t scale, double a, double b, double c, double p_geo, int
  *  1 + 0.5 sin(2 pi t / period) on [0, t_span). */
 int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int64_t period, uint64_t seed,
                       int64_t m, int32_t* src, int32_t* dst, int64_t* t);
+/*  Netflow-like: pairs with both endpoints from the same power-law weights,
+ *  a Pareto (alpha 2) number of contacts per pair, round(count_min *
+ *  U^(-1/2)) with mean about 2 count_min, repeated pairs merged, the last
+ *  pair cut so the total is exactly m; each pair's contacts a burst from a
+ *  uniform start with Pareto (alpha 1.5) gaps of at least gap_min ticks,
+ *  folded into [0, t_span).  With TGAF_PROGRESS set in the environment,
+ *  progress lines go to stderr (also for tgaf_encode). */
+int tgaf_gen_netflow(int64_t n, double exponent, double count_min, double gap_min, int64_t t_span, uint64_t seed,
+                     int64_t m, int32_t* src, int32_t* dst, int64_t* t);
+
+/* The wrapping 64-bit sum over contacts of fmix64(x * 0x9E3779B97F4A7C15 ^
+ * r * 0xC2B2AE3D27D4EB4F ^ t * 0x165667B19E3779F9) (x, r, t as unsigned 32-bit
+ * values): the checksum tga_neighbor_access returns for the encoded list. */
+int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, uint64_t* out);
 
 /* Raw 4-column text ("src dst 1 t" per line), gzip-compressed. */
 int tgaf_write_raw_gz(const char* path, const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m);

@@ -92,3 +92,30 @@ def test_workload_generators_are_deterministic():
         np.testing.assert_array_equal(u, v)
     s, d, t = c
     assert len(s) == 20000 and (np.diff(s) >= 0).all()
+
+
+def test_netflow_generator(tmp_path):
+    """The netflow-like generator: exactly m contacts sorted by (src, dst, t),
+    deterministic, contacts per pair with mean near 20, times inside the span;
+    the raw checksum stored with the dataset equals the oracle's decoded
+    neighbor-access checksum of its encoding."""
+    import sys
+    w = datasets.Workload("nf", "netflow", 1 << 12, 200000, 4, 1 << 20, False)
+    s, d, t = datasets.generate(w)
+    s2, d2, t2 = datasets.generate(w)
+    for u, v in ((s, s2), (d, d2), (t, t2)):
+        np.testing.assert_array_equal(u, v)
+    assert len(s) == 200000 and t.min() >= 0 and t.max() < 1 << 20
+    key = s.astype(np.int64) << 32 | d
+    assert (np.diff(key) >= 0).all()
+    same = np.diff(key) == 0
+    assert (np.diff(t)[same] >= 0).all()
+    per_pair = len(s) / (1 + (~same).sum())
+    assert 15 < per_pair < 25, per_pair
+    desc = datasets.write_dataset(str(tmp_path), "nf", s, d, t, n_min=w.n)
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+    import oracle as O
+    o = O.Oracle(datasets.dataset_dir(str(tmp_path), "nf"), "nf", w.n)
+    cs, cnt = o.neighbor_access()
+    o.close()
+    assert cnt == len(s) and cs == desc["checksum"]
