@@ -128,6 +128,15 @@ def ds_rmat(data_root):
 
 
 @pytest.fixture(scope="session")
+def ds_nf(data_root):
+    """Netflow-like: every contact in a run of one repeated neighbor, bursty
+    times whose deltas turn negative between pairs, hub records of ~10^4
+    contacts (chunked)."""
+    s, d, t = fmt.gen_netflow(1 << 14, 1 << 19, t_span=1 << 20, gap_min=16.0, seed=6)
+    return DS(data_root, "nf14", s, d, t, n_min=1 << 14)
+
+
+@pytest.fixture(scope="session")
 def ds_cl(data_root):
     s, d, t = fmt.gen_chung_lu(1 << 14, 1 << 18, exponent=2.1, t_span=1 << 20, period=(1 << 20) // 64, seed=5)
     return DS(data_root, "cl14", s, d, t, n_min=1 << 14)

@@ -21,9 +21,9 @@ def rel_linf(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.fixture(scope="module", params=["adv", "cl14"])
+@pytest.fixture(scope="module", params=["adv", "cl14", "nf14"])
 def act(request):
-    ds = request.getfixturevalue({"adv": "ds_adv", "cl14": "ds_cl"}[request.param])
+    ds = request.getfixturevalue({"adv": "ds_adv", "cl14": "ds_cl", "nf14": "ds_nf"}[request.param])
     rng = (ds.n // 10, ds.n - ds.n // 7)
     print(f"\n[gpu] dataset {ds.name} n={ds.n} active={rng}", flush=True)
     g = TemporalGraph(ds.dir, ds.name, active=rng)

@@ -23,10 +23,10 @@ def rel_linf(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.fixture(scope="module", params=["sample", "adv", "rmat16", "cl14"])
+@pytest.fixture(scope="module", params=["sample", "adv", "rmat16", "cl14", "nf14"])
 def pair(request):
     ds = request.getfixturevalue({"sample": "ds_sample", "adv": "ds_adv", "rmat16": "ds_rmat",
-                                  "cl14": "ds_cl"}[request.param])
+                                  "cl14": "ds_cl", "nf14": "ds_nf"}[request.param])
     print(f"\n[gpu] dataset {ds.name} n={ds.n} m={len(ds.src)}", flush=True)
     g = TemporalGraph(ds.dir, ds.name)
     yield ds, g
