@@ -140,8 +140,14 @@ def large_workload_times(g, t_lo: int, t_hi: int, sources: int = 4) -> dict:
     """Milliseconds per call of plain BFS (mean over `sources` seeded
     sources), CC and PR (20 iterations, FP32) over the full window and a 1%
     window in the middle of the span, plus the RA pass (the large single-GPU
-    configuration).  Each call is timed once after one warm call."""
+    configuration).  Results go to device buffers (HBM), as the reference's
+    kernels leave them: a host copy of one 2^25-entry result over PCIe would
+    take longer than most of these calls.  Each call is timed once after one
+    warm call, synchronised on the device."""
     import numpy as np
+    import torch
+    lvl = torch.empty(g.info["n"], dtype=torch.int32, device="cuda:0")
+    rank = torch.empty(g.info["n"], dtype=torch.float32, device="cuda:0")
     rng = np.random.default_rng(11)
     span = t_hi - t_lo
     windows = {"100%": (t_lo, t_hi), "1%": (t_lo + span // 2, t_lo + span // 2 + span // 100)}
@@ -156,11 +162,28 @@ def large_workload_times(g, t_lo: int, t_hi: int, sources: int = 4) -> dict:
     out = {}
     for name, w in windows.items():
         out[name] = {"RA": timed(lambda: g.neighbor_access()),
-                     "BFS": round(sum(timed(lambda s=s: g.bfs(s, w)) for s in srcs) / len(srcs), 3),
-                     "CC": timed(lambda: g.connected_components(w)),
-                     "PR": timed(lambda: g.pagerank(w, iters=20))}
+                     "BFS": round(sum(timed(lambda s=s: (g.bfs(s, w, out=lvl), torch.cuda.synchronize()))
+                                      for s in srcs) / len(srcs), 3),
+                     "CC": timed(lambda: (g.connected_components(w, out=lvl), torch.cuda.synchronize())),
+                     "PR": timed(lambda: (g.pagerank(w, iters=20, out=rank), torch.cuda.synchronize()))}
         log(f"[bench] {name} window: {out[name]}")
     return out
+
+
+def large_legs(args) -> int:
+    """The large configuration's plain workloads with results in HBM (child
+    process: torch's HIP runtime must initialise before the engine's)."""
+    import torch
+    torch.zeros(1, device="cuda:0")
+    from compresstemporalgraph_amd import datasets
+    from compresstemporalgraph_amd.engine import TemporalGraph
+    root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
+    desc = datasets.ensure(root, args.workload)
+    g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    out = large_workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
+    g.close()
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def batched_bfs(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
@@ -492,6 +515,8 @@ def main() -> int:
                     help="time the seven workloads but skip the batched / served BFS legs")
     ap.add_argument("--no-served", action="store_true",
                     help="with --query-legs: the batched legs only (no served streams)")
+    ap.add_argument("--large-legs", action="store_true",
+                    help="only time the large configuration's plain workloads and print their JSON (child process)")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -520,6 +545,8 @@ def main() -> int:
 
     if args.query_legs:
         return query_legs(args)
+    if args.large_legs:
+        return large_legs(args)
 
     from compresstemporalgraph_amd import datasets
     from compresstemporalgraph_amd.engine import TemporalGraph
@@ -574,12 +601,21 @@ def main() -> int:
                 "algorithmic_bytes": int(alg_bytes)}
     workloads = None
     large = args.workload == "netflow25"
-    if not args.no_workloads:
+    if not args.no_workloads and not large:
         t0 = time.perf_counter()
-        times = large_workload_times if large else workload_times
-        workloads = times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
+        workloads = workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
         log(f"[bench] workloads timed in {time.perf_counter() - t0:.1f} s: {workloads}")
     g.close()
+    if not args.no_workloads and large:
+        import subprocess
+        cmd = [sys.executable, os.path.abspath(__file__), "--large-legs", "--workload", args.workload]
+        if args.data_root:
+            cmd += ["--data-root", args.data_root]
+        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        if child.returncode != 0:
+            raise SystemExit(f"large workload legs failed (exit {child.returncode})")
+        workloads = json.loads(child.stdout.strip().splitlines()[-1])
+        log(f"[bench] workloads (results in HBM): {workloads}")
 
     batched = bdc = bcc = bpr = bhits = served = smix = None
     if not args.no_workloads and not args.no_query_legs and not large:

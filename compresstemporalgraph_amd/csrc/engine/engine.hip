@@ -231,6 +231,7 @@ struct Sel {
 };
 
 constexpr int kExpandBlocks = 1024;  // grid of k_expand (one queue reservation per group per block)
+constexpr uint32_t kExpandCoop = 16;  // chunk lists longer than this are written by a whole wave
 
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -334,15 +335,34 @@ __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t
     for (int g = 0; g < kGroups; ++g) v[g] = g == grp ? items : 0u;
     v[kGroups] = (items && info >= 0) ? 1u : 0u;
     block_excl_scan(v, pre, tot);
+    // Queue offset of this thread's items (static indices only: a dynamically
+    // indexed register array spills).
+    const uint32_t qoff = grp == 0 ? run[0] + pre[0] : (grp == 1 ? run[1] + pre[1] : run[2] + pre[2]);
+    bool coop = false;
     if (items) {
-      // (static indices only: a dynamically indexed register array spills)
-      int32_t* q = grp == 0 ? Q.q[0] + run[0] + pre[0] : (grp == 1 ? Q.q[1] + run[1] + pre[1] : Q.q[2] + run[2] + pre[2]);
+      int32_t* q = (grp == 0 ? Q.q[0] : (grp == 1 ? Q.q[1] : Q.q[2])) + qoff;
       if (info < 0) {
         q[0] = x;
       } else {
-        for (uint32_t j = 0; j < items; ++j) q[j] = -(info + (int32_t)j) - 1;
+        if (items <= kExpandCoop)
+          for (uint32_t j = 0; j < items; ++j) q[j] = -(info + (int32_t)j) - 1;
+        else
+          coop = true;
         if (bigq) bigq[run[kGroups] + pre[kGroups]] = x;
       }
+    }
+    // Long chunk lists are written by the whole wave, one record at a time,
+    // as coalesced stores (one lane writing thousands of entries would hold
+    // the block back).
+    const int lane = threadIdx.x & 63;
+    for (uint64_t todo = __ballot(coop); todo; todo &= todo - 1) {
+      const int l = __ffsll((unsigned long long)todo) - 1;
+      const int lg = __shfl(grp, l, 64);
+      const uint32_t lo_ = (uint32_t)__shfl((int32_t)qoff, l, 64);
+      const int32_t li = __shfl(info, l, 64);
+      const uint32_t ln = (uint32_t)__shfl((int32_t)items, l, 64);
+      int32_t* q = (lg == 0 ? Q.q[0] : (lg == 1 ? Q.q[1] : Q.q[2])) + lo_;
+      for (uint32_t j = (uint32_t)lane; j < ln; j += 64) q[j] = -(li + (int32_t)j) - 1;
     }
 #pragma unroll
     for (int g = 0; g <= kGroups; ++g) run[g] += tot[g];
