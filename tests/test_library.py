@@ -119,3 +119,45 @@ def test_netflow_generator(tmp_path):
     cs, cnt = o.neighbor_access()
     o.close()
     assert cnt == len(s) and cs == desc["checksum"]
+
+
+def test_entry_points_reject_null_handles():
+    """Every compute entry point returns TGA_EINVAL for a NULL handle (or
+    server) and names itself in tga_last_error(); none exits or touches a
+    device."""
+    L = engine.lib()
+    W = engine.Window
+    w = W(0, 10)
+    buf = (C.c_int32 * 4)()
+    fbuf = (C.c_float * 4)()
+    wins = (W * 1)(w)
+    src = (C.c_int32 * 1)(0)
+    u64 = C.c_uint64()
+    i64 = C.c_int64()
+    i32 = C.c_int32()
+    calls = {
+        "tga_neighbor_access": lambda: L.tga_neighbor_access(None, None, 0, C.byref(u64), C.byref(i64)),
+        "tga_degree_centrality": lambda: L.tga_degree_centrality(None, w, buf, None, 0),
+        "tga_degree_centrality_batch": lambda: L.tga_degree_centrality_batch(None, wins, 1, buf, None, 0),
+        "tga_bfs": lambda: L.tga_bfs(None, w, 0, buf, 0),
+        "tga_bfs_batch": lambda: L.tga_bfs_batch(None, wins, src, 1, buf, 0),
+        "tga_bfs_server_open": lambda: L.tga_bfs_server_open(None, C.byref(C.c_void_p())),
+        "tga_bfs_server_admit": lambda: L.tga_bfs_server_admit(None, 0, w, C.byref(i32)),
+        "tga_bfs_server_step": lambda: L.tga_bfs_server_step(None, C.byref(u64)),
+        "tga_bfs_server_result": lambda: L.tga_bfs_server_result(None, 0, buf, 0),
+        "tga_earliest_arrival": lambda: L.tga_earliest_arrival(None, w, 0, buf, 0),
+        "tga_latest_departure": lambda: L.tga_latest_departure(None, w, 0, buf, None, 0),
+        "tga_connected_components": lambda: L.tga_connected_components(None, w, buf, 0),
+        "tga_connected_components_batch": lambda: L.tga_connected_components_batch(None, wins, 1, buf, 0),
+        "tga_graph_coloring": lambda: L.tga_graph_coloring(None, w, 4, None, buf, 0),
+        "tga_pagerank": lambda: L.tga_pagerank(None, w, 1, 0.85, 0.0, fbuf, None, 0),
+        "tga_pagerank_batch": lambda: L.tga_pagerank_batch(None, wins, 1, 1, 0.85, fbuf, 0),
+        "tga_hits": lambda: L.tga_hits(None, w, 1, fbuf, fbuf, 0),
+        "tga_hits_batch": lambda: L.tga_hits_batch(None, wins, 1, 1, fbuf, fbuf, 0),
+        "tga_profile": lambda: L.tga_profile(None, 1),
+        "tga_profile_read": lambda: L.tga_profile_read(None, None, None, 0),
+        "tga_info": lambda: L.tga_info(None, None),
+    }
+    for name, call in calls.items():
+        assert call() == -1, name  # TGA_EINVAL
+        assert name in L.tga_last_error().decode(), name
