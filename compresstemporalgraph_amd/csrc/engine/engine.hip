@@ -24,12 +24,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -2343,9 +2345,24 @@ bool read_file(const std::string& path, std::vector<uint8_t>& out) {
 // Stream words plus kPadWords zero words (bit buffers read up to five words
 // from any position inside the stream).
 constexpr size_t kPadWords = 5;
+// Large streams are converted by up to 16 threads, one slice of words each
+// (a little-endian load of 8 bytes, byte-swapped, puts the first byte on top).
 std::vector<uint64_t> be_words(const std::vector<uint8_t>& b) {
-  std::vector<uint64_t> w((b.size() + 7) / 8 + kPadWords, 0);
-  for (size_t i = 0; i < b.size(); ++i) w[i >> 3] |= (uint64_t)b[i] << (56 - 8 * (i & 7));
+  const size_t nw = (b.size() + 7) / 8;
+  std::vector<uint64_t> w(nw + kPadWords, 0);
+  auto slice = [&](size_t lo, size_t hi) {
+    const size_t bytes = std::min(b.size(), hi * 8) - lo * 8;
+    std::memcpy(w.data() + lo, b.data() + lo * 8, bytes);
+    for (size_t i = lo; i < hi; ++i) w[i] = __builtin_bswap64(w[i]);
+  };
+  const size_t nth = nw >= ((size_t)1 << 22) ? std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())) : 1;
+  if (nth == 1) {
+    slice(0, nw);
+  } else {
+    std::vector<std::thread> pool;
+    for (size_t k = 0; k < nth; ++k) pool.emplace_back(slice, nw * k / nth, nw * (k + 1) / nth);
+    for (auto& th : pool) th.join();
+  }
   return w;
 }
 
@@ -2749,10 +2766,14 @@ static int build_schedule(tga_graph* g) {
       else
         chain[group_of(c)].emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
     }
-    std::stable_sort(small.begin(), small.end(), [&](int32_t a, int32_t b) {
-      return msb64_host((uint64_t)deg[(size_t)a]) > msb64_host((uint64_t)deg[(size_t)b]);
-    });
-    items[0].insert(items[0].end(), small.begin(), small.end());
+    // Longest length class (msb of the degree) first, id order within a
+    // class: a stable counting sort over the 64 classes.
+    size_t at[65] = {};
+    for (const int32_t x : small) ++at[64 - msb64_host((uint64_t)deg[(size_t)x])];
+    for (int k = 1; k <= 64; ++k) at[k] += at[k - 1];
+    const size_t first = items[0].size();
+    items[0].resize(first + small.size());
+    for (const int32_t x : small) items[0][first + at[63 - msb64_host((uint64_t)deg[(size_t)x])]++] = x;
     for (int c = 1; c < kGroups; ++c) {
       std::stable_sort(chain[c].begin(), chain[c].end(),
                        [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {
@@ -2792,6 +2813,19 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   }
   if (max_label < 1 || max_label >= INT32_MAX) return fail(TGA_ERANGE, "MAX_LABEL must be in [1, 2^31 - 2]");
   const int64_t n = max_label + 1;
+  // Phase times on stderr when TGA_OPEN_TIMES is set.
+  const char* tenv = std::getenv("TGA_OPEN_TIMES");
+  const bool talk = tenv && *tenv && *tenv != '0';
+  auto tick = std::chrono::steady_clock::now();
+  std::string phases;
+  auto phase = [&](const char* what) {
+    const auto now = std::chrono::steady_clock::now();
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%s%s %.3f s", phases.empty() ? "" : ", ", what,
+                  std::chrono::duration<double>(now - tick).count());
+    phases += buf;
+    tick = now;
+  };
 
   std::vector<uint8_t> graph, ts, glo, gup, tlo, tup;
   const std::pair<std::string, std::vector<uint8_t>*> files[] = {
@@ -2804,6 +2838,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   for (const auto& f : files)
     if (!read_file(f.first, *f.second)) return fail(TGA_EIO, "cannot read required file: " + f.first);
   if (ts.size() < 8) return fail(TGA_EFORMAT, "timestamps file shorter than its 64-bit header");
+  phase("read");
 
   // Lower-bit widths, computed from the stream file sizes (bfs.cu:1785-1786).
   const uint64_t qg = (uint64_t)graph.size() * 8 / (uint64_t)max_label;
@@ -2819,12 +2854,14 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
 
   const std::vector<uint64_t> gw = be_words(graph), tw = be_words(ts), glw = be_words(glo), tlw = be_words(tlo);
   const std::vector<uint64_t> guw = le_words(gup), tuw = le_words(tup);
+  phase("words");
   uint64_t gones = 0, tones = 0;
   std::vector<uint64_t> gsamp, gspill, tsamp, tspill;
   build_select(guw, (gup.size() + 7) / 8, gones, gsamp, gspill);
   build_select(tuw, (tup.size() + 7) / 8, tones, tsamp, tspill);
   if (gones < (uint64_t)n || tones < (uint64_t)n)
     return fail(TGA_EFORMAT, "GPU index holds fewer entries than MAX_LABEL + 1");
+  phase("select");
   if ((uint64_t)n * (uint64_t)lg > (uint64_t)glo.size() * 8 || (uint64_t)n * (uint64_t)lt > (uint64_t)tlo.size() * 8)
     return fail(TGA_EFORMAT, "GPU index lower-bit file too short");
 
@@ -2866,8 +2903,11 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   if ((rc = g->alloc(g->d_err, 1))) return bail(rc);
   S.err = g->d_err;
   if (hipMemset(g->d_err, 0, sizeof(int32_t)) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipMemset failed"));
+  phase("device+upload");
 
   if ((rc = build_schedule(g))) return bail(rc);
+  if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipStreamSynchronize failed"));
+  phase("schedule");
 
   const size_t qn = (size_t)n + (size_t)g->nck;
   if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
@@ -2889,6 +2929,8 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   I.active_begin = g->ab;
   I.active_end = g->ae;
   I.device_bytes = g->device_bytes;
+  phase("buffers");
+  if (talk) std::fprintf(stderr, "[tga_open] %s: %s\n", name.c_str(), phases.c_str());
   *out = g;
   return TGA_OK;
 }
