@@ -10,7 +10,7 @@ generated and encoded by the format kit before the timed region.
 Infinity Cache): the RA pass, then plain BFS, CC and PR over the full and a 1%
 window; it takes a few minutes to generate and encode.
 
-Prints one JSON line with the driver's contract fields plus:
+Prints one JSON line with the benchmark record's fields plus:
   roofline      the decode pass against MI355X HBM bandwidth (8 TB/s):
                 algorithmic bytes per pass (|.graph| + |.timestamps| + the
                 Elias-Fano index bits (n+1)(l_g+2 + l_t+2)/8) divided by the

@@ -4,7 +4,7 @@ The reference runs one workload per executable (gpu_cuda/<Dataset>/<algo>.cu)
 and its CPU driver names them ``RA BFS CC PR HITS DC GC``
 (cpu_java/.../BaselineRunner.java:50-88).  ``TemporalGraph`` opens one dataset
 on one GPU and exposes the seven entry points of ``include/tga.h`` with the
-same names; ``run(workload)`` accepts the driver's workload spellings.
+same names; ``run(workload)`` accepts that CPU driver's workload spellings.
 
 Outputs are numpy arrays by default.  Passing ``out=`` a CUDA tensor (or any
 object with ``data_ptr()`` on the graph's device) writes the result into
