@@ -150,8 +150,10 @@ def _plain_one(g: TemporalGraph, r, device_out: bool):
 # costs about the same whatever its width (passes over the graph for 64
 # columns), so below these counts the plain calls are cheaper (on a 2^25
 # contact graph a windowed batch costs about 24 plain PR, 9 plain CC or 2
-# plain DC queries).
-MIN_BATCH = {"DC": 2, "PR": 16, "CC": 8}
+# plain DC queries).  CC waits for 16: a batch made while the other kinds
+# run takes several times its time alone and holds back every queue, so CC
+# batches only under a real backlog.
+MIN_BATCH = {"DC": 2, "PR": 16, "CC": 16}
 
 
 def _answer(g: TemporalGraph, req: list, idx: list, device_out: bool) -> list:
