@@ -28,6 +28,9 @@ Prints one JSON line with the benchmark record's fields plus:
   workloads_ms  the seven entry points (plus earliest arrival and latest
                 departure) over windows covering 100%, 50% and 10% of the
                 timestamp span.
+  batched_mix   256 window queries of the batching mix (40% BFS, 20% CC, 30%
+                PR, 10% DC) as batched calls of up to 64 per kind against 256
+                plain calls, on this graph and on rmat16 (results compared).
   batched_bfs   64 windowed BFS queries in one batched call against 64 plain
                 calls (results compared, outputs in HBM).
   batched_dc    64 windowed degree-centrality queries in one batched call
@@ -428,6 +431,70 @@ def served_mix(g, workers: dict, t_lo: int, t_hi: int) -> dict:
     return out
 
 
+def batched_mix(g, t_lo: int, t_hi: int, nq: int = 256) -> dict:
+    """The batching configuration's query set: nq window queries (lengths
+    log-uniform in [2^10, 2^24], uniform starts; 40% BFS from random sources,
+    20% CC, 30% PR with 20 iterations, 10% DC with centrality), answered as
+    batched calls of up to 64 queries of a kind, against nq plain calls.
+    Results in HBM; every batched result is compared with its plain one
+    (PageRank within 1e-6 relative L-inf, the rest exact).  One untimed
+    batched pass and one plain call per kind come first (caches and list
+    buffers)."""
+    import torch
+    from compresstemporalgraph_amd import serving
+    n = g.info["n"]
+    req = serving.mixed_requests(1.0, nq, n, t_lo, t_hi, seed=77, shares=MIX)
+    calls = {"CC": serving._cc_batch, "PR": serving._pr_batch, "DC": serving._dc_batch}
+
+    def run_batched():
+        res = {}
+        for kind in ("BFS", "CC", "PR", "DC"):
+            idx = [i for i, r in enumerate(req) if r.kind == kind]
+            for b in range(0, len(idx), 64):
+                part = idx[b:b + 64]
+                if kind == "BFS":
+                    out = torch.empty((len(part), n), dtype=torch.int32, device="cuda:0")
+                    g.bfs_batch([req[i].src for i in part], [req[i].window for i in part], out=out)
+                    rows = [out[j] for j in range(len(part))]
+                else:
+                    rows = calls[kind](g, req, part, True)
+                res.update(zip(part, rows))
+        torch.cuda.synchronize()
+        return res
+
+    def run_plain():
+        res = {i: serving._plain_one(g, r, True) for i, r in enumerate(req)}
+        torch.cuda.synchronize()
+        return res
+
+    run_batched()
+    for kind in ("BFS", "CC", "PR", "DC"):
+        first = [r for r in req if r.kind == kind]
+        if first:
+            serving._plain_one(g, first[0], True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rb = run_batched()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rp = run_plain()
+    tp = time.perf_counter() - t0
+
+    def same(a, b):
+        if isinstance(a, tuple):
+            return all(torch.equal(x, y) for x, y in zip(a, b))
+        if a.dtype == torch.float32:
+            d = (a.double() - b.double()).abs().max() / b.double().abs().max().clamp_min(1e-300)
+            return bool(d <= 1e-6)
+        return torch.equal(a, b)
+
+    identical = len(rb) == nq and all(same(rb[i], rp[i]) for i in range(nq))
+    kinds = {k: sum(r.kind == k for r in req) for k in ("BFS", "CC", "PR", "DC")}
+    return {"queries": nq, "kinds": kinds, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+            "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1), "speedup": round(tp / tb, 2),
+            "identical": bool(identical)}
+
+
 def query_legs(args) -> int:
     """Batched and served BFS legs (child process of the default run)."""
     import torch
@@ -438,6 +505,14 @@ def query_legs(args) -> int:
     desc = datasets.ensure(root, args.workload)
     g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
     t_lo, t_hi = desc.get("t_min", 0), desc.get("t_max", 1 << 24)
+    bmix = batched_mix(g, t_lo, t_hi)
+    log(f"[bench] batched 256-query mix on {args.workload}: {bmix}")
+    if not bmix["identical"]:
+        raise SystemExit("batched mix results differ from the plain results")
+    if args.mix_only:
+        g.close()
+        print(json.dumps({"batched_mix": bmix}), flush=True)
+        return 0
     batched = batched_bfs(g, t_lo, t_hi)
     log(f"[bench] batched BFS: {batched}")
     if not batched["identical"]:
@@ -469,8 +544,9 @@ def query_legs(args) -> int:
             h.close()
         log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
-    print(json.dumps({"batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc, "batched_pr": bpr,
-                      "batched_hits": bhits, "served_bfs": served, "served_mix": smix}), flush=True)
+    print(json.dumps({"batched_mix": bmix, "batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc,
+                      "batched_pr": bpr, "batched_hits": bhits, "served_bfs": served, "served_mix": smix}),
+          flush=True)
     return 0
 
 
@@ -524,6 +600,8 @@ def main() -> int:
                     help="with --query-legs: the batched legs only (no served streams)")
     ap.add_argument("--large-legs", action="store_true",
                     help="only time the large configuration's plain workloads and print their JSON (child process)")
+    ap.add_argument("--mix-only", action="store_true",
+                    help="with --query-legs: only the 256-query batched mix")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -624,7 +702,7 @@ def main() -> int:
         workloads = json.loads(child.stdout.strip().splitlines()[-1])
         log(f"[bench] workloads (results in HBM): {workloads}")
 
-    batched = bdc = bcc = bpr = bhits = served = smix = None
+    batched = bdc = bcc = bpr = bhits = served = smix = bmix = None
     if not args.no_workloads and not args.no_query_legs and not large:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -644,6 +722,16 @@ def main() -> int:
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
+        bmix = {args.workload: legs["batched_mix"]}
+        # The same 256-query mix on the small R-MAT graph (2^16 vertices,
+        # 2^20 contacts), the graph the batching configuration names.
+        cmd_c1 = [sys.executable, os.path.abspath(__file__), "--query-legs", "--mix-only", "--workload", "rmat16"]
+        if args.data_root:
+            cmd_c1 += ["--data-root", args.data_root]
+        child = subprocess.run(cmd_c1, stdout=subprocess.PIPE, text=True, env=env)
+        if child.returncode != 0:
+            raise SystemExit(f"rmat16 mix leg failed (exit {child.returncode})")
+        bmix["rmat16"] = json.loads(child.stdout.strip().splitlines()[-1])["batched_mix"]
         batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
         bhits = legs["batched_hits"]
         served, smix = legs["served_bfs"], legs["served_mix"]
@@ -675,6 +763,7 @@ def main() -> int:
         "checksum": {"value": f"{cs:#018x}", "equals_raw_list": raw_cs is not None and cs == raw_cs},
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
+        "batched_mix": bmix,
         "batched_bfs": batched,
         "batched_dc": bdc,
         "batched_cc": bcc,
