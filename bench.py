@@ -651,6 +651,42 @@ def main() -> int:
         f"ready in {time.perf_counter() - t0:.1f} s")
     ds_dir = datasets.dataset_dir(root, args.workload)
 
+    large = args.workload == "netflow25"
+    batched = bdc = bcc = bpr = bhits = served = smix = bmix = None
+    if not args.no_workloads and not args.no_query_legs and not large:
+        # The query legs keep results in torch device tensors.  torch brings
+        # its own HIP runtime (which must initialise before the engine's in a
+        # process, and whose streams share the hardware queues), so they run
+        # in a child process, before this process touches the GPU: a live
+        # context here would hold hardware queues the served mix needs.
+        import subprocess
+        cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
+        if args.data_root:
+            cmd += ["--data-root", args.data_root]
+        # Sixteen hardware queues: the served mix runs its batches on three
+        # more handles beside the BFS loop, and with HIP's default of four the
+        # handles' streams share queues, so a long kernel of one stalls the
+        # launches queued behind it on another.
+        env = dict(os.environ)
+        env.setdefault("GPU_MAX_HW_QUEUES", "16")
+        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
+        if child.returncode != 0:
+            raise SystemExit(f"query legs failed (exit {child.returncode})")
+        legs = json.loads(child.stdout.strip().splitlines()[-1])
+        bmix = {args.workload: legs["batched_mix"]}
+        # The same 256-query mix on the small R-MAT graph (2^16 vertices,
+        # 2^20 contacts), the graph the batching configuration names.
+        cmd_c1 = [sys.executable, os.path.abspath(__file__), "--query-legs", "--mix-only", "--workload", "rmat16"]
+        if args.data_root:
+            cmd_c1 += ["--data-root", args.data_root]
+        child = subprocess.run(cmd_c1, stdout=subprocess.PIPE, text=True, env=env)
+        if child.returncode != 0:
+            raise SystemExit(f"rmat16 mix leg failed (exit {child.returncode})")
+        bmix["rmat16"] = json.loads(child.stdout.strip().splitlines()[-1])["batched_mix"]
+        batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
+        bhits = legs["batched_hits"]
+        served, smix = legs["served_bfs"], legs["served_mix"]
+
     t0 = time.perf_counter()
     g = TemporalGraph(ds_dir, args.workload, device=0)
     log(f"[bench] opened on GPU in {time.perf_counter() - t0:.2f} s: {g.info}")
@@ -692,7 +728,6 @@ def main() -> int:
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
                 "algorithmic_bytes": int(alg_bytes)}
     workloads = None
-    large = args.workload == "netflow25"
     if not args.no_workloads and not large:
         t0 = time.perf_counter()
         workloads = workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
@@ -708,40 +743,6 @@ def main() -> int:
             raise SystemExit(f"large workload legs failed (exit {child.returncode})")
         workloads = json.loads(child.stdout.strip().splitlines()[-1])
         log(f"[bench] workloads (results in HBM): {workloads}")
-
-    batched = bdc = bcc = bpr = bhits = served = smix = bmix = None
-    if not args.no_workloads and not args.no_query_legs and not large:
-        # The query legs keep results in torch device tensors.  torch brings
-        # its own HIP runtime (which must initialise before the engine's in a
-        # process, and whose streams share the hardware queues), so they run
-        # in a child process and leave the pass timing above untouched.
-        import subprocess
-        cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
-        if args.data_root:
-            cmd += ["--data-root", args.data_root]
-        # Sixteen hardware queues: the served mix runs its batches on three
-        # more handles beside the BFS loop, and with HIP's default of four the
-        # handles' streams share queues, so a long kernel of one stalls the
-        # launches queued behind it on another.
-        env = dict(os.environ)
-        env.setdefault("GPU_MAX_HW_QUEUES", "16")
-        child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
-        if child.returncode != 0:
-            raise SystemExit(f"query legs failed (exit {child.returncode})")
-        legs = json.loads(child.stdout.strip().splitlines()[-1])
-        bmix = {args.workload: legs["batched_mix"]}
-        # The same 256-query mix on the small R-MAT graph (2^16 vertices,
-        # 2^20 contacts), the graph the batching configuration names.
-        cmd_c1 = [sys.executable, os.path.abspath(__file__), "--query-legs", "--mix-only", "--workload", "rmat16"]
-        if args.data_root:
-            cmd_c1 += ["--data-root", args.data_root]
-        child = subprocess.run(cmd_c1, stdout=subprocess.PIPE, text=True, env=env)
-        if child.returncode != 0:
-            raise SystemExit(f"rmat16 mix leg failed (exit {child.returncode})")
-        bmix["rmat16"] = json.loads(child.stdout.strip().splitlines()[-1])["batched_mix"]
-        batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
-        bhits = legs["batched_hits"]
-        served, smix = legs["served_bfs"], legs["served_mix"]
 
     cpu = None
     if not args.no_cpu_baseline:
