@@ -667,8 +667,9 @@ def main() -> int:
         # more handles beside the BFS loop, and with HIP's default of four the
         # handles' streams share queues, so a long kernel of one stalls the
         # launches queued behind it on another.
-        env = dict(os.environ)
-        env.setdefault("GPU_MAX_HW_QUEUES", "16")
+        # Set, not defaulted: the environment may already carry HIP's default
+        # of four.
+        env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
         child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
