@@ -44,6 +44,7 @@ Prints one JSON line with the benchmark record's fields plus:
   served_mix    Poisson arrivals of 40% BFS, 20% CC, 30% PR, 10% DC at 500/s
                 and 2000/s: continuous batching against sequential plain
                 execution.
+  served_mix_rmat16  the same mix on rmat16 at 1k/s and 10k/s.
   served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
                 against sequential plain BFS, queries/s and p50/p99 latency.
 """
@@ -395,16 +396,17 @@ def served_bfs(g, t_lo: int, t_hi: int) -> dict:
 MIX = {"DC": 0.1, "PR": 0.3, "CC": 0.2}  # the rest BFS: the batching mix of the at-scale queries
 
 
-def served_mix(g, workers: dict, t_lo: int, t_hi: int) -> dict:
+def served_mix(g, workers: dict, t_lo: int, t_hi: int, rates=((500, 400), (2000, 800))) -> dict:
     """Serving a mixed stream of window queries (40% BFS, 20% CC, 30% PR with
-    20 iterations, 10% DC) at Poisson rates of 500 and 2000 per second: BFS by
+    20 iterations, 10% DC) at Poisson rates (default 500 and 2000 per second,
+    `rates` holds (rate, queries) pairs): BFS by
     continuous batching in lanes; DC, PR and CC in batched calls, each kind on
     a handle of its own in a worker thread (plain calls when fewer than
     serving.MIN_BATCH are pending); against sequential plain execution.
     Results in HBM.  The timed runs drop each result once answered (as a
-    server handing results on would); at 500/s a further batched and plain
-    run keep every result, and each is compared with the plain one (PageRank
-    rows within 1e-6 relative L-inf, the rest exact)."""
+    server handing results on would); at the first rate a further batched and
+    plain run keep every result, and each is compared with the plain one
+    (PageRank rows within 1e-6 relative L-inf, the rest exact)."""
     import torch
     from compresstemporalgraph_amd import serving
     # Untimed: each worker handle builds its per-handle caches (run structure,
@@ -415,12 +417,12 @@ def served_mix(g, workers: dict, t_lo: int, t_hi: int) -> dict:
     workers["CC"].connected_components_batch(wins)
     workers["DC"].degree_centrality_batch(wins)
     out = {}
-    for rate, count in ((500, 400), (2000, 800)):
+    for rate, count in rates:
         req = serving.mixed_requests(rate, count, g.info["n"], t_lo, t_hi, seed=rate + 7, shares=MIX)
         sb = serving.serve_batched(g, req, device_out=True, workers=workers)
         sp = serving.serve_plain(g, req, device_out=True)
         row = {"batched": sb, "plain": sp}
-        if rate == 500:
+        if rate == rates[0][0]:
             kb, kp = {}, {}
             serving.serve_batched(g, req, keep=kb, device_out=True, workers=workers)
             serving.serve_plain(g, req, keep=kp, device_out=True)
@@ -517,8 +519,16 @@ def query_legs(args) -> int:
     if not bmix["identical"]:
         raise SystemExit("batched mix results differ from the plain results")
     if args.mix_only:
+        # The serving configuration on this graph: the same mix at Poisson
+        # 1k/s and 10k/s.
+        workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+                   for k in ("DC", "PR", "CC")}
+        smix = served_mix(g, workers, t_lo, t_hi, rates=((1000, 1000), (10000, 4000)))
+        for h in workers.values():
+            h.close()
+        log(f"[bench] served mix on {args.workload}: {smix}")
         g.close()
-        print(json.dumps({"batched_mix": bmix}), flush=True)
+        print(json.dumps({"batched_mix": bmix, "served_mix": smix}), flush=True)
         return 0
     batched = batched_bfs(g, t_lo, t_hi)
     log(f"[bench] batched BFS: {batched}")
@@ -608,7 +618,7 @@ def main() -> int:
     ap.add_argument("--large-legs", action="store_true",
                     help="only time the large configuration's plain workloads and print their JSON (child process)")
     ap.add_argument("--mix-only", action="store_true",
-                    help="with --query-legs: only the 256-query batched mix")
+                    help="with --query-legs: only the 256-query batched mix and the served mix at 1k/s and 10k/s")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -652,7 +662,7 @@ def main() -> int:
     ds_dir = datasets.dataset_dir(root, args.workload)
 
     large = args.workload == "netflow25"
-    batched = bdc = bcc = bpr = bhits = served = smix = bmix = None
+    batched = bdc = bcc = bpr = bhits = served = smix = bmix = smix_c1 = None
     if not args.no_workloads and not args.no_query_legs and not large:
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
@@ -683,7 +693,8 @@ def main() -> int:
         child = subprocess.run(cmd_c1, stdout=subprocess.PIPE, text=True, env=env)
         if child.returncode != 0:
             raise SystemExit(f"rmat16 mix leg failed (exit {child.returncode})")
-        bmix["rmat16"] = json.loads(child.stdout.strip().splitlines()[-1])["batched_mix"]
+        c1 = json.loads(child.stdout.strip().splitlines()[-1])
+        bmix["rmat16"], smix_c1 = c1["batched_mix"], c1["served_mix"]
         batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
         bhits = legs["batched_hits"]
         served, smix = legs["served_bfs"], legs["served_mix"]
@@ -779,6 +790,7 @@ def main() -> int:
         "batched_pr": bpr,
         "batched_hits": bhits,
         "served_mix": smix,
+        "served_mix_rmat16": smix_c1,
         "served_bfs": served,
     }
     print(json.dumps(line), flush=True)
