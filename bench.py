@@ -44,7 +44,7 @@ Prints one JSON line with the benchmark record's fields plus:
   served_mix    Poisson arrivals of 40% BFS, 20% CC, 30% PR, 10% DC at 500/s
                 and 2000/s: continuous batching against sequential plain
                 execution.
-  served_mix_rmat16  the same mix on rmat16 at 1k/s and 10k/s.
+  served_mix_rmat16  the same mix on rmat16 at 100/s, 1k/s and 10k/s.
   served_bfs    Poisson BFS arrivals at 1k/s and 10k/s: continuous batching
                 against sequential plain BFS, queries/s and p50/p99 latency.
 """
@@ -520,10 +520,10 @@ def query_legs(args) -> int:
         raise SystemExit("batched mix results differ from the plain results")
     if args.mix_only:
         # The serving configuration on this graph: the same mix at Poisson
-        # 1k/s and 10k/s.
+        # 100/s, 1k/s and 10k/s.
         workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
                    for k in ("DC", "PR", "CC")}
-        smix = served_mix(g, workers, t_lo, t_hi, rates=((1000, 1000), (10000, 4000)))
+        smix = served_mix(g, workers, t_lo, t_hi, rates=((100, 200), (1000, 1000), (10000, 4000)))
         for h in workers.values():
             h.close()
         log(f"[bench] served mix on {args.workload}: {smix}")
@@ -618,7 +618,8 @@ def main() -> int:
     ap.add_argument("--large-legs", action="store_true",
                     help="only time the large configuration's plain workloads and print their JSON (child process)")
     ap.add_argument("--mix-only", action="store_true",
-                    help="with --query-legs: only the 256-query batched mix and the served mix at 1k/s and 10k/s")
+                    help="with --query-legs: only the 256-query batched mix and the served mix at 100/s, 1k/s "
+                         "and 10k/s")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
