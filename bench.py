@@ -444,7 +444,10 @@ def batched_mix(g, t_lo: int, t_hi: int, nq: int = 256) -> dict:
     """The batching configuration's query set: nq window queries (lengths
     log-uniform in [2^10, 2^24], uniform starts; 40% BFS from random sources,
     20% CC, 30% PR with 20 iterations, 10% DC with centrality), answered as
-    batched calls of up to 64 queries of a kind, against nq plain calls.
+    batched calls of up to 64 queries of a kind (a remainder below
+    serving.MIN_BATCH of its kind, 8 for BFS, as plain calls: the serving
+    loop's rule, since a batched call costs about the same at any width),
+    against nq plain calls.
     Results in HBM; every batched result is compared with its plain one
     (PageRank within 1e-6 relative L-inf, the rest exact).  One untimed
     batched pass and one plain call per kind come first (caches and list
@@ -461,6 +464,9 @@ def batched_mix(g, t_lo: int, t_hi: int, nq: int = 256) -> dict:
             idx = [i for i, r in enumerate(req) if r.kind == kind]
             for b in range(0, len(idx), 64):
                 part = idx[b:b + 64]
+                if len(part) < serving.MIN_BATCH.get(kind, 8):
+                    res.update((i, serving._plain_one(g, req[i], True)) for i in part)
+                    continue
                 if kind == "BFS":
                     out = torch.empty((len(part), n), dtype=torch.int32, device="cuda:0")
                     g.bfs_batch([req[i].src for i in part], [req[i].window for i in part], out=out)
