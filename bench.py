@@ -2,13 +2,16 @@
 
 One step = one full neighbor-access (RA) pass of the GPU engine over every
 contact of the workload, with the compressed graph resident in HBM.  The
-workload is the plain at-scale configuration: Chung-Lu, exponent 2.1, 2^21
-vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks (``chunglu21``),
-generated and encoded by the format kit before the timed region.
-``--workload netflow25`` runs the large single-GPU configuration instead
-(2^25 vertices, 2^30 contacts, streams of about 2.7 GB, beyond the 256 MB
-Infinity Cache): the RA pass, then plain BFS, CC and PR over the full and a 1%
-window; it takes a few minutes to generate and encode.
+headline workload is the large single-GPU configuration (``netflow25``):
+netflow-like, 2^25 vertices, 2^30 contacts, streams of about 2.7 GB (beyond
+the 256 MB Infinity Cache, so a pass reads HBM), generated and encoded by the
+format kit before the timed region (about a minute); its line also carries
+plain BFS, CC and PR over the full and a 1% window.  Beside it, under the key
+``chunglu21``, the plain at-scale configuration (Chung-Lu, exponent 2.1,
+2^21 vertices, 2^25 contacts, diurnal timestamps over 2^24 ticks): its RA
+pass and the seven-workload table, and the batched and served query legs run
+on that graph and on rmat16.  ``--workload chunglu21`` makes that graph the
+headline instead (``--no-side`` drops the side measurement).
 
 Prints one JSON line with the benchmark record's fields plus:
   roofline      the decode pass against MI355X HBM bandwidth (8 TB/s):
@@ -611,7 +614,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="chunglu21")
+    ap.add_argument("--workload", default="netflow25")
+    ap.add_argument("--no-side", action="store_true",
+                    help="with the netflow25 headline: skip the chunglu21 pass, workload table and query legs")
     ap.add_argument("--data-root", default=None)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -661,23 +666,22 @@ def main() -> int:
     from compresstemporalgraph_amd.engine import TemporalGraph
 
     root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
-    t0 = time.perf_counter()
     os.environ.setdefault("TGAF_PROGRESS", "1")  # the large graph encodes for minutes
-    desc = datasets.ensure(root, args.workload)
-    log(f"[bench] dataset {args.workload}: n={desc['max_vertex']} contacts={desc['contacts']} "
-        f"ready in {time.perf_counter() - t0:.1f} s")
-    ds_dir = datasets.dataset_dir(root, args.workload)
-
-    large = args.workload == "netflow25"
+    # The at-scale plain graph (chunglu21) carries the seven-workload table
+    # and the batched / served query legs; with the large graph as the
+    # headline it is measured first, as extra keys of the same line.
+    side = "chunglu21" if args.workload == "netflow25" and not args.no_side else None
+    legs_on = args.workload if args.workload != "netflow25" else side
     batched = bdc = bcc = bpr = bhits = served = smix = bmix = smix_c1 = None
-    if not args.no_workloads and not args.no_query_legs and not large:
+    if not args.no_workloads and not args.no_query_legs and legs_on:
+        datasets.ensure(root, legs_on)
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
         # process, and whose streams share the hardware queues), so they run
         # in a child process, before this process touches the GPU: a live
         # context here would hold hardware queues the served mix needs.
         import subprocess
-        cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", args.workload]
+        cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", legs_on]
         if args.data_root:
             cmd += ["--data-root", args.data_root]
         # Sixteen hardware queues: the served mix runs its batches on three
@@ -691,7 +695,7 @@ def main() -> int:
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")
         legs = json.loads(child.stdout.strip().splitlines()[-1])
-        bmix = {args.workload: legs["batched_mix"]}
+        bmix = {legs_on: legs["batched_mix"]}
         # The same 256-query mix on the small R-MAT graph (2^16 vertices,
         # 2^20 contacts), the graph the batching configuration names.
         cmd_c1 = [sys.executable, os.path.abspath(__file__), "--query-legs", "--mix-only", "--workload", "rmat16"]
@@ -706,46 +710,33 @@ def main() -> int:
         bhits = legs["batched_hits"]
         served, smix = legs["served_bfs"], legs["served_mix"]
 
+    side_rec = None
+    if side:
+        sdesc = datasets.ensure(root, side)
+        g = TemporalGraph(datasets.dataset_dir(root, side), side, device=0)
+        rec = ra_pass(g, sdesc, args.steps, args.warmup, None)
+        if not args.no_workloads:
+            t0 = time.perf_counter()
+            rec["workloads_ms"] = workload_times(g, sdesc.get("t_min", 0), sdesc.get("t_max", 1 << 24))
+            log(f"[bench] {side} workloads timed in {time.perf_counter() - t0:.1f} s: {rec['workloads_ms']}")
+        g.close()
+        rec["workload"] = f"{side}: {WORKLOAD_DESC[side]}"
+        side_rec = rec
+
+    t0 = time.perf_counter()
+    desc = datasets.ensure(root, args.workload)
+    log(f"[bench] dataset {args.workload}: n={desc['max_vertex']} contacts={desc['contacts']} "
+        f"ready in {time.perf_counter() - t0:.1f} s")
+    ds_dir = datasets.dataset_dir(root, args.workload)
+    large = args.workload == "netflow25"
+
     t0 = time.perf_counter()
     g = TemporalGraph(ds_dir, args.workload, device=0)
     log(f"[bench] opened on GPU in {time.perf_counter() - t0:.2f} s: {g.info}")
     info = g.info
     n = info["n"]
     contacts = info["contacts"]
-
-    for _ in range(args.warmup):
-        g.neighbor_access()
-    g.profile(True)
-    g.profile_read(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        cs, cnt = g.neighbor_access()
-    elapsed = time.perf_counter() - t0
-    kernel_ms, launches = g.profile_read(reset=True)
-    g.profile(False)
-    assert cnt == contacts
-    log(f"[bench] {args.steps} steps in {elapsed:.3f} s; decode kernels {kernel_ms / args.steps:.3f} ms per pass "
-        f"({launches} launches); checksum {cs:#018x}")
-    # Size-independent parity: the decoded checksum equals the one of the raw
-    # contact list the dataset was encoded from.
-    raw_cs = desc.get("checksum")
-    if raw_cs is not None and cs != raw_cs:
-        raise SystemExit(f"decoded checksum {cs:#018x} differs from the raw list's {raw_cs:#018x}")
-
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = contacts * args.steps / elapsed
-    # One pass = the concurrent launches of the three item groups (flat
-    # records, depth-1 chains, deeper chains), timed as one interval.
-    avg_kernel_s = kernel_ms / args.steps / 1e3
-    index_bits = (n + 1) * (info["l_graph"] + 2 + info["l_timestamps"] + 2)
-    alg_bytes = info["graph_bytes"] + info["timestamps_bytes"] + index_bits / 8.0
-    achieved = alg_bytes / avg_kernel_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": args.traffic_bytes if args.traffic_bytes is not None else measured_traffic(args.workload),
-                "kernel": "k_ra (the three concurrent group launches of one pass)",
-                "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // args.steps,
-                "algorithmic_bytes": int(alg_bytes)}
+    rec = ra_pass(g, desc, args.steps, args.warmup, args.traffic_bytes)
     workloads = None
     if not args.no_workloads and not large:
         t0 = time.perf_counter()
@@ -770,12 +761,12 @@ def main() -> int:
 
     line = {
         "metric": "decoded temporal edges/s",
-        "value": value,
+        "value": rec["value"],
         "unit": "edges/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": rec["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -786,10 +777,11 @@ def main() -> int:
                    "vertices": n, "contacts": contacts,
                    "graph_bytes": info["graph_bytes"], "timestamps_bytes": info["timestamps_bytes"],
                    "parallelism": "single GPU"},
-        "roofline": roofline,
-        "checksum": {"value": f"{cs:#018x}", "equals_raw_list": raw_cs is not None and cs == raw_cs},
+        "roofline": rec["roofline"],
+        "checksum": rec["checksum"],
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
+        "chunglu21": side_rec,
         "batched_mix": bmix,
         "batched_bfs": batched,
         "batched_dc": bdc,
@@ -802,6 +794,45 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def ra_pass(g, desc: dict, steps: int, warmup: int, traffic_bytes) -> dict:
+    """Times `steps` full neighbor-access passes (after `warmup` untimed ones)
+    with the graph resident in HBM; the pass's kernels are also timed with HIP
+    events on the engine's stream.  The decoded checksum must equal the raw
+    contact list's (size-independent parity at full size)."""
+    info = g.info
+    n, contacts, name = info["n"], info["contacts"], desc["name"]
+    for _ in range(warmup):
+        g.neighbor_access()
+    g.profile(True)
+    g.profile_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cs, cnt = g.neighbor_access()
+    elapsed = time.perf_counter() - t0
+    kernel_ms, launches = g.profile_read(reset=True)
+    g.profile(False)
+    assert cnt == contacts
+    log(f"[bench] {name}: {steps} steps in {elapsed:.3f} s; decode kernels {kernel_ms / steps:.3f} ms per pass "
+        f"({launches} launches); checksum {cs:#018x}")
+    raw_cs = desc.get("checksum")
+    if raw_cs is not None and cs != raw_cs:
+        raise SystemExit(f"{name}: decoded checksum {cs:#018x} differs from the raw list's {raw_cs:#018x}")
+    # One pass = the concurrent launches of the item groups, timed as one
+    # interval.
+    avg_kernel_s = kernel_ms / steps / 1e3
+    index_bits = (n + 1) * (info["l_graph"] + 2 + info["l_timestamps"] + 2)
+    alg_bytes = info["graph_bytes"] + info["timestamps_bytes"] + index_bits / 8.0
+    achieved = alg_bytes / avg_kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic_bytes if traffic_bytes is not None else measured_traffic(name),
+                "kernel": "k_ra (the concurrent group launches of one pass)",
+                "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // steps,
+                "algorithmic_bytes": int(alg_bytes)}
+    return {"value": contacts * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "roofline": roofline,
+            "checksum": {"value": f"{cs:#018x}", "equals_raw_list": raw_cs is not None and cs == raw_cs}}
 
 
 if __name__ == "__main__":

@@ -2422,6 +2422,9 @@ struct tga_graph {
   tga_graph_info info{};
   Streams S{};
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_bytes;
+  std::vector<void**> alloc_owner;
+  int64_t fail_alloc_at = 0;  // tga_debug_fail_alloc: this allocation from now fails (0: off)
   int32_t* d_err = nullptr;
   // schedule
   Ckpt* ck = nullptr;
@@ -2543,15 +2546,60 @@ struct tga_graph {
     ev_used = 0;
   }
 
+  // Device allocations of the handle.  Each records its size and, when the
+  // pointer it was stored into is a member of this handle, that member, so
+  // that a failed group of first-use allocations can be undone (AllocTxn)
+  // and a staging buffer can be released before it grows.
   template <class T>
   int alloc(T*& p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (fail_alloc_at > 0 && --fail_alloc_at == 0)
+      return fail(TGA_ENOMEM, "hipMalloc: injected failure (tga_debug_fail_alloc)");
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(TGA_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    const char* at = (const char*)&p;
+    const char* self = (const char*)this;
     allocs.push_back(q);
-    device_bytes += (int64_t)(std::max<size_t>(count, 1) * sizeof(T));
+    alloc_bytes.push_back(bytes);
+    alloc_owner.push_back(at >= self && at < self + sizeof(*this) ? (void**)&p : nullptr);
+    device_bytes += (int64_t)bytes;
     p = (T*)q;
     return TGA_OK;
+  }
+  void free_at(size_t i) {
+    (void)hipFree(allocs[i]);  // waits for work still using it
+    if (alloc_owner[i]) *alloc_owner[i] = nullptr;
+    device_bytes -= (int64_t)alloc_bytes[i];
+    allocs.erase(allocs.begin() + (std::ptrdiff_t)i);
+    alloc_bytes.erase(alloc_bytes.begin() + (std::ptrdiff_t)i);
+    alloc_owner.erase(alloc_owner.begin() + (std::ptrdiff_t)i);
+  }
+  // Frees one allocation of the handle and resets the pointer.
+  template <class T>
+  void release(T*& p) {
+    for (size_t i = allocs.size(); p && i-- > 0;)
+      if (allocs[i] == (void*)p) {
+        free_at(i);
+        break;
+      }
+    p = nullptr;
+  }
+  // Frees every allocation made after `mark` (members are reset to null).
+  void rollback(size_t mark) {
+    while (allocs.size() > mark) free_at(allocs.size() - 1);
+  }
+  // A host staging buffer of `rows` rows of n entries: grown by releasing
+  // the old one first, so a long-running caller holds one buffer of the
+  // largest width it used.
+  template <class T>
+  int rows_buffer(T*& p, int32_t& have, int32_t rows) {
+    if (have >= rows && p) return TGA_OK;
+    release(p);
+    have = 0;
+    int rc = alloc(p, (size_t)rows * (size_t)n);
+    if (!rc) have = rows;
+    return rc;
   }
   template <class T>
   int upload(T*& dst, const std::vector<T>& src) {
@@ -2630,6 +2678,23 @@ struct tga_graph {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
+};
+
+// A group of first-use allocations that commits together or not at all:
+// unless commit() is reached, every allocation made since construction is
+// freed and its handle member reset, so a later call never finds a group
+// half allocated (its first pointer set, the others null).
+struct AllocTxn {
+  tga_graph* g;
+  size_t mark;
+  bool done = false;
+  explicit AllocTxn(tga_graph* g_) : g(g_), mark(g_->allocs.size()) {}
+  void commit() { done = true; }
+  ~AllocTxn() {
+    if (!done) g->rollback(mark);
+  }
+  AllocTxn(const AllocTxn&) = delete;
+  AllocTxn& operator=(const AllocTxn&) = delete;
 };
 
 extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
@@ -2960,6 +3025,7 @@ extern "C" void tga_close(tga_graph* g) { delete g; }
 extern "C" int tga_info(const tga_graph* g, tga_graph_info* info) {
   if (!g || !info) return fail(TGA_EINVAL, "tga_info: NULL argument");
   *info = g->info;
+  info->device_bytes = g->device_bytes;  // what the handle holds now
   return TGA_OK;
 }
 
@@ -3061,11 +3127,13 @@ static int ensure_ctimes(tga_graph* g, bool* ok) {
   const uint64_t contacts = (uint64_t)std::max<int64_t>(g->info.contacts, 0);
   int rc = list_budget(contacts * 8 + (uint64_t)(g->n + 1) * 8 + item_scan_bytes(g), g->ct != nullptr, ok);
   if (rc || !*ok || g->ct) return rc;
+  AllocTxn txn(g);
   if ((rc = item_scan_alloc(g)) || (rc = g->alloc(g->coff, (size_t)g->n + 1)) ||
       (rc = scan_counts(g, (const uint32_t*)g->vdeg, g->n, g->coff)) || (rc = g->alloc(g->ct, (size_t)contacts)))
     return rc;
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
   LAUNCH_STATIC(k_ctimes, A, g->ct, (const uint64_t*)g->coff);
+  txn.commit();
   return TGA_OK;
 }
 
@@ -3138,10 +3206,7 @@ extern "C" int tga_degree_centrality_batch(tga_graph* g, const tga_window* windo
   int32_t *cnt = count, *cen = centrality;
   if (!(flags & TGA_OUT_DEVICE)) {
     const int32_t rows = centrality ? 2 * nq : nq;
-    if (g->dcb_rows < rows) {
-      if ((rc = g->alloc(g->dcb_out, (size_t)rows * (size_t)n))) return rc;
-      g->dcb_rows = rows;
-    }
+    if ((rc = g->rows_buffer(g->dcb_out, g->dcb_rows, rows))) return rc;
     cnt = g->dcb_out;
     cen = centrality ? g->dcb_out + (int64_t)nq * n : nullptr;
   }
@@ -3207,6 +3272,7 @@ static int ensure_prb(tga_graph* g, bool* ok) {
   if ((rc = list_budget(need, false, ok)) || !*ok) return rc;
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
   hipStream_t st = g->stream;
+  AllocTxn txn(g);
   if ((rc = item_scan_alloc(g))) return rc;
   k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->lcnt, m, 0u);
   LAUNCH_STATIC(k_runs, A, g->lcnt, (const uint64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
@@ -3324,6 +3390,7 @@ static int ensure_prb(tga_graph* g, bool* ok) {
     g->prb_ndu_items = (int32_t)ni;
   }
   g->prb_runs = runs;  // built
+  txn.commit();
   return TGA_OK;
 }
 
@@ -3347,6 +3414,7 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
     return TGA_OK;
   }
   if (!g->prb_du) {
+    AllocTxn txn(g);
     if ((rc = g->alloc(g->prb_du, (size_t)n * 64)) || (rc = g->alloc(g->prb_rank, (size_t)n * 64)) ||
         (rc = g->alloc(g->prb_coef, (size_t)n * 64)) ||
         (rc = g->alloc(g->prb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
@@ -3357,6 +3425,7 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
         (!g->prb_part && (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64))) ||
         (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))))
       return rc;
+    txn.commit();
   }
   int64_t hw[2 * 64];
   for (int32_t q = 0; q < nq; ++q) {
@@ -3367,10 +3436,7 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
   HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
   float* out = rank;
   if (!(flags & TGA_OUT_DEVICE)) {
-    if (g->prb_out_rows < nq) {
-      if ((rc = g->alloc(g->prb_out, (size_t)nq * (size_t)n))) return rc;
-      g->prb_out_rows = nq;
-    }
+    if ((rc = g->rows_buffer(g->prb_out, g->prb_out_rows, nq))) return rc;
     out = g->prb_out;
   }
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
@@ -3429,12 +3495,14 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
     return TGA_OK;
   }
   if (!g->ccb_label) {
+    AllocTxn txn(g);
     if ((rc = g->alloc(g->ccb_label, (size_t)n * 64)) ||
         (rc = g->alloc(g->ccb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
         (rc = g->alloc(g->ccb_mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))) ||
         (rc = g->alloc(g->ccb_changed, 1)))
       return rc;
     if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
+    txn.commit();
   }
   int64_t hw[2 * 64];
   for (int32_t q = 0; q < nq; ++q) {
@@ -3445,10 +3513,7 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
   HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
   int32_t* out = label;
   if (!(flags & TGA_OUT_DEVICE)) {
-    if (g->ccb_out_rows < nq) {
-      if ((rc = g->alloc(g->ccb_out, (size_t)nq * (size_t)n))) return rc;
-      g->ccb_out_rows = nq;
-    }
+    if ((rc = g->rows_buffer(g->ccb_out, g->ccb_out_rows, nq))) return rc;
     out = g->ccb_out;
   }
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
@@ -3499,6 +3564,7 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
     return TGA_OK;
   }
   if (!g->hb_hub) {
+    AllocTxn txn(g);
     if ((rc = g->alloc(g->hb_hub, (size_t)n * 64)) || (rc = g->alloc(g->hb_auth, (size_t)n * 64)) ||
         (rc = g->alloc(g->hb_partial,
                        (size_t)std::max<int64_t>(std::max(g->prb_nseg, g->prb_noseg), 1) * 64)) ||
@@ -3507,6 +3573,7 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
     if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
     if (!g->prb_acc && (rc = g->alloc(g->prb_acc, (size_t)n * 64))) return rc;
     if (!g->prb_part && (rc = g->alloc(g->prb_part, (size_t)kPrbRedBlocks * 64))) return rc;
+    txn.commit();
   }
   int64_t hw[2 * 64];
   for (int32_t q = 0; q < nq; ++q) {
@@ -3517,10 +3584,7 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
   HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
   float *oh = hub, *oa = auth;
   if (!(flags & TGA_OUT_DEVICE)) {
-    if (g->hb_out_rows < 2 * nq) {
-      if ((rc = g->alloc(g->hb_out, (size_t)2 * nq * (size_t)n))) return rc;
-      g->hb_out_rows = 2 * nq;
-    }
+    if ((rc = g->rows_buffer(g->hb_out, g->hb_out_rows, 2 * nq))) return rc;
     oh = g->hb_out;
     oa = g->hb_out + (int64_t)nq * n;
   }
@@ -3604,17 +3668,16 @@ extern "C" int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int3
   const int64_t n = g->n;
   int rc;
   if (!g->bq_front) {
+    AllocTxn txn(g);
     if ((rc = g->alloc(g->bq_front, (size_t)n)) || (rc = g->alloc(g->bq_vis, (size_t)n)) ||
         (rc = g->alloc(g->bq_next, (size_t)n)) || (rc = g->alloc(g->bq_bound, kMaxBounds)) ||
         (rc = g->alloc(g->bq_mask, kMaxBounds + 1)) || (rc = g->alloc(g->bq_src, kMaxQueries)))
       return rc;
+    txn.commit();
   }
   int32_t* lv = levels;
   if (!(flags & TGA_OUT_DEVICE)) {
-    if (g->bq_levels_rows < nq) {
-      if ((rc = g->alloc(g->bq_levels, (size_t)nq * (size_t)n))) return rc;
-      g->bq_levels_rows = nq;
-    }
+    if ((rc = g->rows_buffer(g->bq_levels, g->bq_levels_rows, nq))) return rc;
     lv = g->bq_levels;
   }
   // Window table: sorted boundaries (each window [s, e] contributes s and
@@ -3929,9 +3992,11 @@ static int item_scan_alloc(tga_graph* g) {
   if (g->lcnt) return TGA_OK;
   const int64_t m = g->n + g->nck;
   int rc;
+  AllocTxn txn(g);
   if ((rc = g->alloc(g->lcnt, (size_t)m)) || (rc = g->alloc(g->loff, (size_t)m + 1)) ||
       (rc = g->alloc(g->ltsum, (size_t)scan_tiles(m))))
     return rc;
+  txn.commit();
   return TGA_OK;
 }
 
@@ -3953,11 +4018,13 @@ static int gc_lists(tga_graph* g, const Window& W, const int32_t* prio, int32_t 
   if (!ok) return 1;
   if ((rc = item_scan_alloc(g))) return rc;
   if (!g->gc_hp) {
+    AllocTxn txn(g);
     if ((rc = g->alloc(g->gc_hp, (size_t)entries)) || (rc = g->alloc(g->gc_lbeg, (size_t)n)) ||
         (rc = g->alloc(g->gc_llen, (size_t)n)) || (rc = g->alloc(g->gc_ctr, 2)))
       return rc;
     for (auto& l : g->gc_list)
       if ((rc = g->alloc(l, (size_t)n))) return rc;
+    txn.commit();
   }
   hipStream_t st = g->stream;
   k_fill<uint32_t><<<g->grid(n + g->nck), kBlock, 0, st>>>(g->lcnt, n + g->nck, 0u);
@@ -4002,8 +4069,10 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
     if (!g->prio) {
       std::vector<int32_t> h((size_t)g->n);
       java_random_ints(123, 256, g->n, h.data());
+      AllocTxn txn(g);
       int rc = g->upload(g->prio, h);
       if (rc) return rc;
+      txn.commit();
     }
     prio = g->prio;
   }
@@ -4077,6 +4146,7 @@ static int build_pairs(tga_graph* g, const Window& W, uint64_t* extra, uint64_t*
   if (!ok) return 1;
   if ((rc = item_scan_alloc(g))) return rc;
   if (!g->pk) {
+    AllocTxn txn(g);
     uint8_t* tmp = nullptr;
     if ((rc = g->alloc(g->pk, (size_t)cap)) || (rc = g->alloc(g->pv, (size_t)cap)) ||
         (rc = g->alloc(g->sk, (size_t)cap)) || (rc = g->alloc(g->sv, (size_t)cap)) ||
@@ -4085,6 +4155,7 @@ static int build_pairs(tga_graph* g, const Window& W, uint64_t* extra, uint64_t*
       return rc;
     g->sort_tmp = tmp;
     g->sort_tmp_bytes = tmp_bytes;
+    txn.commit();
   }
   hipStream_t st = g->stream;
   k_fill<uint32_t><<<g->grid(m), kBlock, 0, st>>>(g->lcnt, m, 0u);
@@ -4220,6 +4291,12 @@ extern "C" int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, f
   if (!rc) rc = g->out_copy(auth, g->f1, (size_t)g->n * 4, flags);
   if (rc) return rc;
   return g->check_err();
+}
+
+extern "C" int tga_debug_fail_alloc(tga_graph* g, int64_t nth) {
+  if (!g || nth < 0) return fail(TGA_EINVAL, "tga_debug_fail_alloc: bad arguments");
+  g->fail_alloc_at = nth;
+  return TGA_OK;
 }
 
 extern "C" int tga_profile(tga_graph* g, int enable) {

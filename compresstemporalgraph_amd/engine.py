@@ -88,6 +88,7 @@ def lib():
         L.tga_hits_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, vp, vp, C.c_uint32]
         L.tga_profile.argtypes = [vp, C.c_int]
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
+        L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
         _LIB = L
     return _LIB
 
@@ -174,6 +175,17 @@ class TemporalGraph:
         ms, n = C.c_double(), C.c_int64()
         _check(lib().tga_profile_read(self.h, C.byref(ms), C.byref(n), int(reset)))
         return ms.value, n.value
+
+    def device_bytes(self) -> int:
+        """Device memory the handle holds now (grows with first-use state)."""
+        info = GraphInfo()
+        _check(lib().tga_info(self.h, C.byref(info)))
+        return int(info.device_bytes)
+
+    def debug_fail_alloc(self, nth: int) -> None:
+        """Error-path test hook: the nth device allocation from now fails with
+        TGA_ENOMEM (0 turns it off)."""
+        _check(lib().tga_debug_fail_alloc(self.h, int(nth)))
 
     # -- the seven entry points -------------------------------------------
     def neighbor_access(self, verts=None):

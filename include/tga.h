@@ -238,6 +238,13 @@ int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t 
 int tga_profile(tga_graph* g, int enable);
 int tga_profile_read(tga_graph* g, double* kernel_ms, int64_t* launches, int reset);
 
+/* Error-path test hook: the nth device allocation the handle makes from now
+ * on fails with TGA_ENOMEM, as if hipMalloc had (nth = 0 turns it off).  A
+ * call that fails this way leaves no group of first-use buffers half
+ * allocated, so the next call succeeds or fails cleanly.  No counterpart in
+ * the reference, which exits on any allocation failure (bfs.cu:23-32). */
+int tga_debug_fail_alloc(tga_graph* g, int64_t nth);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,29 @@ class Raw:
                 return ld.astype(np.int32)
             ld = new
 
+    # -- earliest arrival from s (the GPU BFS rule of bfs.cu:1601-1619 as a
+    #    deterministic fixpoint): arr[s] = max(window start, 0); each distinct
+    #    window pair (x, r) between active vertices is used with its FIRST
+    #    window contact t only (the reference skips repeated neighbors,
+    #    bfs.cu:1452), and gives r the arrival t when arr[x] <= t < arr[r].
+    def earliest_arrival(self, s):
+        arr = np.full(self.n, INF32, np.int64)
+        arr[s] = max(self.window[0], 0)
+        if not self.active[s]:
+            return arr.astype(np.int32)
+        es, ed, et = self.esrc, self.edst, self.et  # sorted by (src, dst, t)
+        first = np.ones(len(es), bool)
+        first[1:] = (es[1:] != es[:-1]) | (ed[1:] != ed[:-1])
+        ps, pd, pt = es[first], ed[first], et[first]
+        while True:
+            ok = (arr[ps] <= pt) & (pt < arr[pd])
+            cand = np.full(self.n, INF32, np.int64)
+            np.minimum.at(cand, pd[ok], pt[ok])
+            new = np.minimum(arr, cand)
+            if np.array_equal(new, arr):
+                return arr.astype(np.int32)
+            arr = new
+
     # -- CC min-label by fixpoint propagation
     def cc(self):
         label = np.where(self.active, np.arange(self.n), -1).astype(np.int64)

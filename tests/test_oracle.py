@@ -233,6 +233,61 @@ def test_latest_departure_oracle(tmp_path):
                 assert (got != B.LD_UNREACHED).sum() > 1 or w == (lo, lo) or act is not None
 
 
+def test_earliest_arrival_is_exhaustive_path_search():
+    """Earliest arrival on tiny random graphs equals the earliest last hop over
+    every simple path src -> ... -> v whose hops use each pair's first window
+    contact, with non-decreasing times starting at or after max(window start,
+    0) (windows and an active range included).  Simple paths suffice: cutting
+    a cycle out of such a path keeps its times non-decreasing and its last
+    hop.  Plain Python over the raw contacts; independent of the fixpoint."""
+    rng = np.random.default_rng(37)
+    for trial in range(80):
+        n = 6
+        m = int(rng.integers(4, 24))
+        src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+        t = rng.integers(0, 12, m)
+        src, dst, t = [np.asarray(a) for a in sort_contacts(src, dst, t)]
+        win = None if trial % 3 == 0 else tuple(sorted(int(v) for v in rng.integers(0, 12, 2)))
+        act = None if trial % 2 == 0 else (1, 5)
+        ws, we = win if win is not None else B.FULL_WINDOW
+        ab, ae = act if act is not None else (0, n)
+        pair_t = {}
+        for a, b, tt in zip(src.tolist(), dst.tolist(), t.tolist()):
+            if ws <= tt <= we and ab <= a < ae and ab <= b < ae:
+                pair_t[(a, b)] = min(pair_t.get((a, b), tt), tt)
+        s = int(rng.integers(0, n))
+        a0 = max(ws, 0)
+        want = [INF32] * n
+        want[s] = a0
+
+        def walk(v, now, seen):
+            for (a, b), tt in pair_t.items():
+                if a == v and tt >= now and b not in seen:
+                    want[b] = min(want[b], tt)
+                    walk(b, tt, seen | {b})
+
+        if ab <= s < ae:
+            walk(s, a0, {s})
+        R = B.Raw(src, dst, t, n, window=win, active=act)
+        np.testing.assert_array_equal(R.earliest_arrival(s), np.array(want, np.int32), err_msg=f"trial {trial}")
+
+
+def test_earliest_arrival_oracle(tmp_path):
+    """The C oracle's earliest arrival equals the brute force (itself pinned
+    by exhaustive path search) over windows, an active range and sources."""
+    (s, d, t), n = _small_graph(seed=22, n=300, m=3000)
+    ds = DS(str(tmp_path), "ea2", s, d, t, n_min=n)
+    o = ds.oracle()
+    lo, hi = int(t.min()), int(t.max())
+    for w in (None, (lo + (hi - lo) // 4, lo + (hi - lo) * 3 // 4), (lo, lo), (lo + (hi - lo) // 10, hi)):
+        for act in (None, (n // 10, n - n // 7)):
+            for src in (int(s[0]), int(s[len(s) // 2]), n // 10 + 3):
+                R = B.Raw(s, d, t, n, window=w, active=act)
+                got = o.earliest_arrival(src, window=w, active=act)
+                np.testing.assert_array_equal(got, R.earliest_arrival(src), err_msg=f"{w} {act} {src}")
+                assert (got != INF32).sum() > 1 or w == (lo, lo) or act is not None
+
+
 def test_earliest_arrival_small(tmp_path):
     """Earliest arrival by exhaustive relaxation over first window contacts."""
     (s, d, t), n = _small_graph(seed=16, n=200, m=2000)
