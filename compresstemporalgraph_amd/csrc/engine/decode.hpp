@@ -22,11 +22,22 @@
 // every level) are read through register bit buffers that load one 64-bit
 // word per 64 bits consumed, ahead of use; the rare ones (intervals, copy
 // blocks) use two-word peeks.
+//
+// Everything here is __host__ __device__: the kernels use it on the GPU, and
+// a host build of the same code walks whole graphs on the CPU under address
+// and undefined-behaviour sanitizers (tests/host_decode).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define TGA_HD __host__ __device__ __forceinline__
+
 namespace tga {
+
+TGA_HD uint32_t clz64(uint64_t x) { return x ? (uint32_t)__builtin_clzll(x) : 64u; }
+TGA_HD uint32_t popc64(uint64_t x) { return (uint32_t)__builtin_popcountll(x); }
+TGA_HD int32_t min32(int32_t a, int32_t b) { return a < b ? a : b; }
+TGA_HD uint32_t minu32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 constexpr int32_t kInf = 0x7fffffff;
 constexpr int kMaxDepth = 3;          // maxrefcount (BVMultiGraph.java:163)
@@ -34,8 +45,36 @@ constexpr int kWindow = 7;            // windowsize (BVMultiGraph.java:169)
 constexpr uint32_t kSpanBits = 1024;  // select blocks wider than this are stored exactly
 // Contacts between consecutive decode checkpoints of a long record, by the
 // record's reference-chain depth: deeper records cost more per contact (their
-// copies walk the referenced lists), so their chunks are shorter.
-__host__ __device__ constexpr int32_t chunk_stride(int depth) { return depth == 0 ? 128 : (depth == 1 ? 16 : 8); }
+// copies walk the referenced lists), so their chunks are shorter.  The stride
+// of records without references (`flat`) is chosen per graph at open: short
+// chunks balance a small graph's pass, long ones cut the per-chunk reads of a
+// large graph's.
+constexpr int32_t kFlatStrideMin = 128, kFlatStrideMax = 1024;
+TGA_HD constexpr int32_t chunk_stride(int depth, int32_t flat) { return depth == 0 ? flat : (depth == 1 ? 16 : 8); }
+// About 2^20 flat items per pass: contacts / 2^20, a power of two in
+// [kFlatStrideMin, kFlatStrideMax] (128 up to 2^27 contacts, 1024 from 2^30).
+inline int32_t flat_stride_for(int64_t contacts) {
+  int32_t s = kFlatStrideMin;
+  while (s < kFlatStrideMax && ((int64_t)s << 21) <= contacts) s <<= 1;
+  return s;
+}
+
+// Neighbor-access checksum term of one contact (x, r, t): the checksum of a
+// pass is the sum of the terms of its contacts modulo 2^64, so it does not
+// depend on the order contacts are visited in.  The term is a 32-bit mix
+// (one multiply per contact beyond the per-record x * K: the decode, not the
+// checksum, is what the pass measures):
+//   h = t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77;  h ^= h >> 16;
+//   h *= 0x7FEB352D;  h ^= h >> 15          (all uint32)
+TGA_HD uint32_t contact_key_x(int32_t x) { return (uint32_t)x * 0x85EBCA77u; }
+TGA_HD uint64_t contact_term(uint32_t kx, int32_t r, int32_t t) {
+  uint32_t h = (uint32_t)t ^ ((uint32_t)r * 0x9E3779B1u) ^ kx;
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  return h;
+}
+TGA_HD uint64_t contact_hash(int32_t x, int32_t r, int32_t t) { return contact_term(contact_key_x(x), r, t); }
 
 // Error bits reported through the per-handle device error word.
 enum : int32_t {
@@ -66,6 +105,7 @@ struct Streams {
   EFDir gi, ti;
   int32_t n;       // vertices; ids are [0, n)
   int32_t min_ts;  // .timestamps header value
+  int32_t flat_stride;  // checkpoint stride of records without references (chunk_stride)
   int32_t* err;    // sticky error word
   // Decode checkpoints of long records: vinfo[x] >= 0 is the first
   // checkpoint of x, checkpoint j covering contacts [j * stride, (j + 1) *
@@ -79,10 +119,16 @@ struct Streams {
   const uint64_t* vpos;
 };
 
-__device__ __forceinline__ void raise_err(const Streams& S, int32_t bit) { atomicOr(S.err, bit); }
+TGA_HD void raise_err(const Streams& S, int32_t bit) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicOr(S.err, bit);
+#else
+  __atomic_fetch_or(S.err, bit, __ATOMIC_RELAXED);
+#endif
+}
 
 // 64 stream bits starting at bit `pos` (MSB-first).
-__device__ __forceinline__ uint64_t peek64(const uint64_t* w, uint64_t pos) {
+TGA_HD uint64_t peek64(const uint64_t* w, uint64_t pos) {
   const uint64_t i = pos >> 6;
   const uint32_t s = (uint32_t)pos & 63u;
   const uint64_t a = w[i];
@@ -90,13 +136,13 @@ __device__ __forceinline__ uint64_t peek64(const uint64_t* w, uint64_t pos) {
   return s ? (a << s) | (b >> (64u - s)) : a;
 }
 
-__device__ __forceinline__ int64_t nat2int(uint64_t v) { return (v & 1) ? -(int64_t)(v >> 1) - 1 : (int64_t)(v >> 1); }
+TGA_HD int64_t nat2int(uint64_t v) { return (v & 1) ? -(int64_t)(v >> 1) - 1 : (int64_t)(v >> 1); }
 
 // Codeword decoders on a 64-bit window `p`; `len` receives the codeword
 // length.  A window too short for the code (corrupt stream) flags kErrCode
 // and yields 0 with length 1.
-__device__ __forceinline__ uint64_t dec_gamma(const Streams& S, uint64_t p, uint32_t& len) {
-  const uint32_t z = (uint32_t)__clzll(p);
+TGA_HD uint64_t dec_gamma(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t z = clz64(p);
   if (z > 31) {
     raise_err(S, kErrCode);
     len = 1;
@@ -106,8 +152,8 @@ __device__ __forceinline__ uint64_t dec_gamma(const Streams& S, uint64_t p, uint
   return ((p << z) >> (63 - z)) - 1;
 }
 
-__device__ __forceinline__ uint32_t dec_unary(const Streams& S, uint64_t p, uint32_t& len) {
-  const uint32_t z = (uint32_t)__clzll(p);
+TGA_HD uint32_t dec_unary(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t z = clz64(p);
   if (z > 62) {
     raise_err(S, kErrCode);
     len = 1;
@@ -118,8 +164,8 @@ __device__ __forceinline__ uint32_t dec_unary(const Streams& S, uint64_t p, uint
 }
 
 template <int K>
-__device__ __forceinline__ uint64_t dec_zeta(const Streams& S, uint64_t p, uint32_t& len) {
-  const uint32_t h = (uint32_t)__clzll(p);
+TGA_HD uint64_t dec_zeta(const Streams& S, uint64_t p, uint32_t& len) {
+  const uint32_t h = clz64(p);
   const uint32_t nb = h * K + K - 1;  // bits of the short form
   if (h + nb + 2 > 64) {
     raise_err(S, kErrCode);
@@ -142,7 +188,7 @@ struct Rd {
   uint64_t bits;
 };
 
-__device__ __forceinline__ uint64_t safe_peek(const Streams& S, Rd r, uint64_t pos) {
+TGA_HD uint64_t safe_peek(const Streams& S, Rd r, uint64_t pos) {
   if (pos >= r.bits) {
     raise_err(S, kErrOverrun);
     return ~0ull;  // all ones: every code decodes to 0 with length 1
@@ -150,14 +196,14 @@ __device__ __forceinline__ uint64_t safe_peek(const Streams& S, Rd r, uint64_t p
   return peek64(r.w, pos);
 }
 
-__device__ __forceinline__ uint32_t rd_unary(const Streams& S, Rd r, uint64_t& pos) {
+TGA_HD uint32_t rd_unary(const Streams& S, Rd r, uint64_t& pos) {
   uint32_t len;
   const uint32_t v = dec_unary(S, safe_peek(S, r, pos), len);
   pos += len;
   return v;
 }
 
-__device__ __forceinline__ uint64_t rd_gamma(const Streams& S, Rd r, uint64_t& pos) {
+TGA_HD uint64_t rd_gamma(const Streams& S, Rd r, uint64_t& pos) {
   uint32_t len;
   const uint64_t v = dec_gamma(S, safe_peek(S, r, pos), len);
   pos += len;
@@ -165,7 +211,7 @@ __device__ __forceinline__ uint64_t rd_gamma(const Streams& S, Rd r, uint64_t& p
 }
 
 template <int K>
-__device__ __forceinline__ uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& pos) {
+TGA_HD uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& pos) {
   uint32_t len;
   const uint64_t v = dec_zeta<K>(S, safe_peek(S, r, pos), len);
   pos += len;
@@ -185,7 +231,7 @@ struct BitBuf {
   uint32_t idx;  // word index of a
   uint32_t sh;   // bit offset into a
 
-  __device__ __forceinline__ void init(const uint64_t* w, uint32_t last, uint64_t pos) {
+  TGA_HD void init(const uint64_t* w, uint32_t last, uint64_t pos) {
     uint64_t i = pos >> 6;
     if (i + 2 > last) i = last - 2;
     idx = (uint32_t)i;
@@ -195,12 +241,12 @@ struct BitBuf {
     c = w[idx + 2];
   }
   // A buffer that will not be read (its position is recorded as 0).
-  __device__ __forceinline__ void clear() {
+  TGA_HD void clear() {
     a = b = c = 0;
     idx = sh = 0;
   }
-  __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
-  __device__ __forceinline__ void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
+  TGA_HD uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
+  TGA_HD void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
     sh += n;
     if (sh >= 64) {
       sh -= 64;
@@ -210,7 +256,7 @@ struct BitBuf {
       c = (idx + 2 <= last) ? w[idx + 2] : 0;
     }
   }
-  __device__ __forceinline__ uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
+  TGA_HD uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
 };
 
 // Pipelined bit buffer for the sub-streams read about one codeword per
@@ -232,7 +278,7 @@ struct PBuf {
   uint32_t pfi;   // index of the word in pf (~0: none)
   uint32_t nv;    // look-ahead words held (0..2)
 
-  __device__ __forceinline__ void init(const uint64_t* w, uint32_t last, uint64_t pos) {
+  TGA_HD void init(const uint64_t* w, uint32_t last, uint64_t pos) {
     uint64_t i = pos >> 6;
     if (i + 4 > last) i = last - 4;
     idx = (uint32_t)i;
@@ -248,14 +294,14 @@ struct PBuf {
   }
   // A buffer that will not be read: its look-ahead counts as full, so steps
   // request nothing.
-  __device__ __forceinline__ void clear() {
+  TGA_HD void clear() {
     a = b = c = d = e = pf = 0;
     idx = sh = 0;
     nv = 2;
     pfi = ~0u;
   }
-  __device__ __forceinline__ uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
-  __device__ __forceinline__ void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
+  TGA_HD uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
+  TGA_HD void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
     sh += n;
     if (sh >= 64) {
       sh -= 64;
@@ -266,57 +312,57 @@ struct PBuf {
         d = e;
         --nv;
       } else {
-        c = w[min(idx + 3, last)];
+        c = w[minu32(idx + 3, last)];
       }
       ++idx;
     }
   }
-  __device__ __forceinline__ void step(const uint64_t* w, uint32_t last) {
+  TGA_HD void step(const uint64_t* w, uint32_t last) {
     // Merge, branch-free: pf fills the first missing look-ahead slot.
     const uint32_t m = (pfi == idx + 3 + nv);
     d = (m & (nv == 0)) ? pf : d;
     e = (m & (nv == 1)) ? pf : e;
     nv += m;
     pfi = nv < 2 ? idx + 3 + nv : ~0u;
-    if (nv < 2) pf = w[min(pfi, last)];
+    if (nv < 2) pf = w[minu32(pfi, last)];
   }
-  __device__ __forceinline__ uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
+  TGA_HD uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
 };
 
 // Buffered readers of the graph stream (g_*) and the timestamp stream (t_*).
 template <class B>
-__device__ __forceinline__ void g_init(const Streams& S, B& b, uint64_t pos) {
+TGA_HD void g_init(const Streams& S, B& b, uint64_t pos) {
   if (pos >= S.g_bits) raise_err(S, kErrOverrun);
   b.init(S.g, S.g_last, pos);
 }
 template <class B>
-__device__ __forceinline__ uint64_t g_gamma(const Streams& S, B& b) {
+TGA_HD uint64_t g_gamma(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_gamma(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
 template <class B>
-__device__ __forceinline__ uint32_t g_unary(const Streams& S, B& b) {
+TGA_HD uint32_t g_unary(const Streams& S, B& b) {
   uint32_t len;
   const uint32_t v = dec_unary(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
 template <int K, class B>
-__device__ __forceinline__ uint64_t g_zeta(const Streams& S, B& b) {
+TGA_HD uint64_t g_zeta(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<K>(S, b.peek(), len);
   b.skip(len, S.g, S.g_last);
   return v;
 }
 template <class B>
-__device__ __forceinline__ void t_init(const Streams& S, B& b, uint64_t pos) {
+TGA_HD void t_init(const Streams& S, B& b, uint64_t pos) {
   if (pos >= S.t_bits) raise_err(S, kErrOverrun);
   b.init(S.t, S.t_last, pos);
 }
 template <class B>
-__device__ __forceinline__ uint64_t t_zeta2(const Streams& S, B& b) {
+TGA_HD uint64_t t_zeta2(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<2>(S, b.peek(), len);
   b.skip(len, S.t, S.t_last);
@@ -324,17 +370,17 @@ __device__ __forceinline__ uint64_t t_zeta2(const Streams& S, B& b) {
 }
 
 // Position of the r-th (0-based) set bit of w (r < popcount(w)).
-__device__ __forceinline__ uint32_t select_in_word(uint64_t w, uint32_t r) {
+TGA_HD uint32_t select_in_word(uint64_t w, uint32_t r) {
   uint32_t pos = 0;
-  uint32_t c = (uint32_t)__popcll(w & 0xffffffffull);
+  uint32_t c = popc64(w & 0xffffffffull);
   if (r >= c) { r -= c; w >>= 32; pos += 32; }
-  c = (uint32_t)__popcll(w & 0xffffull);
+  c = popc64(w & 0xffffull);
   if (r >= c) { r -= c; w >>= 16; pos += 16; }
-  c = (uint32_t)__popcll(w & 0xffull);
+  c = popc64(w & 0xffull);
   if (r >= c) { r -= c; w >>= 8; pos += 8; }
-  c = (uint32_t)__popcll(w & 0xfull);
+  c = popc64(w & 0xfull);
   if (r >= c) { r -= c; w >>= 4; pos += 4; }
-  c = (uint32_t)__popcll(w & 0x3ull);
+  c = popc64(w & 0x3ull);
   if (r >= c) { r -= c; w >>= 2; pos += 2; }
   c = (uint32_t)(w & 1ull);
   if (r >= c) { pos += 1; }
@@ -342,14 +388,14 @@ __device__ __forceinline__ uint32_t select_in_word(uint64_t w, uint32_t r) {
 }
 
 // select1(i): position of the i-th one of the upper bits.
-__device__ __forceinline__ uint64_t ef_select(const Streams& S, const EFDir& d, uint64_t i) {
+TGA_HD uint64_t ef_select(const Streams& S, const EFDir& d, uint64_t i) {
   const uint64_t s = d.samp[i >> 6];
   if (s >> 63) return d.spill[(s & ~(1ull << 63)) + (i & 63)];
   uint32_t r = (uint32_t)(i & 63);
   uint64_t w = s >> 6;
   uint64_t word = d.upper[w] & (~0ull << (s & 63));
   for (uint32_t guard = 0;; ++guard) {
-    const uint32_t c = (uint32_t)__popcll(word);
+    const uint32_t c = popc64(word);
     if (r < c) break;
     r -= c;
     ++w;
@@ -363,15 +409,20 @@ __device__ __forceinline__ uint64_t ef_select(const Streams& S, const EFDir& d, 
 }
 
 // getOffset (bfs.cu:543-565): ((select1(i) - i) << l) | low_i.
-__device__ __forceinline__ uint64_t ef_get(const Streams& S, const EFDir& d, uint64_t i) {
+TGA_HD uint64_t ef_get(const Streams& S, const EFDir& d, uint64_t i) {
   const uint64_t hi = ef_select(S, d, i) - i;
   uint64_t lo = 0;
   if (d.l > 0) lo = peek64(d.lower, i * (uint64_t)d.l) >> (64 - d.l);
   return (hi << d.l) | lo;
 }
 
+// End of the timestamps of x: where those of x + 1 start.
+TGA_HD uint64_t rec_tend(const Streams& S, int32_t x) {
+  return x + 1 < S.n ? S.vpos[2 * (size_t)x + 3] : S.t_bits;
+}
+
 // Start offsets of the record of x in both streams.
-__device__ __forceinline__ void rec_pos(const Streams& S, int32_t x, uint64_t& gpos, uint64_t& tpos) {
+TGA_HD void rec_pos(const Streams& S, int32_t x, uint64_t& gpos, uint64_t& tpos) {
   const ulonglong2 p = reinterpret_cast<const ulonglong2*>(S.vpos)[x];
   gpos = p.x;
   tpos = p.y;
@@ -410,33 +461,26 @@ struct Level {
 template <int MAXD>
 struct Dec {
   Level lv[MAXD + 1];
-  PBuf rb0, mb0;                         // level 0: residuals, multiples (and the header while parsed)
+  BitBuf rb0, mb0;                       // level 0: residuals, multiples (and the header while parsed)
   BitBuf rb[MAXD > 0 ? MAXD : 1];        // level L >= 1 at [L - 1]
   BitBuf mb[MAXD > 0 ? MAXD : 1];
 };
 
 template <int L, int MAXD>
-__device__ __forceinline__ auto& rbuf(Dec<MAXD>& D) {
+TGA_HD auto& rbuf(Dec<MAXD>& D) {
   if constexpr (L == 0) return D.rb0; else return D.rb[L - 1];
 }
 template <int L, int MAXD>
-__device__ __forceinline__ auto& mbuf(Dec<MAXD>& D) {
+TGA_HD auto& mbuf(Dec<MAXD>& D) {
   if constexpr (L == 0) return D.mb0; else return D.mb[L - 1];
 }
 template <int L, int MAXD>
-__device__ __forceinline__ const auto& rbuf(const Dec<MAXD>& D) {
+TGA_HD const auto& rbuf(const Dec<MAXD>& D) {
   if constexpr (L == 0) return D.rb0; else return D.rb[L - 1];
 }
 template <int L, int MAXD>
-__device__ __forceinline__ const auto& mbuf(const Dec<MAXD>& D) {
+TGA_HD const auto& mbuf(const Dec<MAXD>& D) {
   if constexpr (L == 0) return D.mb0; else return D.mb[L - 1];
-}
-
-// Advances the pipelined buffers of the top level by one contact.
-template <int MAXD>
-__device__ __forceinline__ void dec_step(const Streams& S, Dec<MAXD>& D) {
-  D.rb0.step(S.g, S.g_last);
-  D.mb0.step(S.g, S.g_last);
 }
 
 // Saved decoder state at a chunk boundary of a long list (see engine.hip).
@@ -468,7 +512,7 @@ struct Ckpt {
 struct Split {
   int32_t cont_slot, cont_r, tail_slot, tail_r;
   // Slot that contact (r) of this item is deferred to, or -1.
-  __device__ __forceinline__ int32_t slot_of(int32_t r) const {
+  TGA_HD int32_t slot_of(int32_t r) const {
     if (r == cont_r && cont_slot >= 0) return cont_slot;
     if (r == tail_r && tail_slot >= 0) return tail_slot;
     return -1;
@@ -476,12 +520,12 @@ struct Split {
 };
 
 template <int L, int MAXD>
-__device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D);
+TGA_HD int32_t dec_next(const Streams& S, Dec<MAXD>& D);
 
 // Restores one level from a saved copy: the buffers of the sub-streams that
 // still have codes to read are refilled at their saved positions.
 template <class RB, class MB>
-__device__ __forceinline__ void lv_restore(const Streams& S, Level& v, RB& rb, MB& mb, const Level& src) {
+TGA_HD void lv_restore(const Streams& S, Level& v, RB& rb, MB& mb, const Level& src) {
   v = src;
   if (v.rleft > 0) g_init(S, rb, v.rpos); else rb.clear();
   if (v.mleft > 0) g_init(S, mb, v.mpos); else mb.clear();
@@ -489,7 +533,7 @@ __device__ __forceinline__ void lv_restore(const Streams& S, Level& v, RB& rb, M
 
 // Copies one level, with its sub-stream positions, into a checkpoint.
 template <class RB, class MB>
-__device__ __forceinline__ void lv_save(Level& dst, const Level& v, const RB& rb, const MB& mb) {
+TGA_HD void lv_save(Level& dst, const Level& v, const RB& rb, const MB& mb) {
   dst = v;
   dst.rpos = rb.pos();
   dst.mpos = mb.pos();
@@ -498,13 +542,13 @@ __device__ __forceinline__ void lv_save(Level& dst, const Level& v, const RB& rb
 // Restores levels L..MAXD of D from saved levels src[L - LL ..] (those within
 // the saved chain depth).
 template <int LL, int L, int MAXD>
-__device__ __forceinline__ void restore_levels(const Streams& S, Dec<MAXD>& D, const Level* src, int32_t depth) {
+TGA_HD void restore_levels(const Streams& S, Dec<MAXD>& D, const Level* src, int32_t depth) {
   if (L - LL <= depth) lv_restore(S, D.lv[L], rbuf<L>(D), mbuf<L>(D), src[L - LL]);
   if constexpr (L < MAXD) restore_levels<LL, L + 1, MAXD>(S, D, src, depth);
 }
 
 template <int L, int MAXD>
-__device__ __forceinline__ void save_levels(Level* dst, const Dec<MAXD>& D) {
+TGA_HD void save_levels(Level* dst, const Dec<MAXD>& D) {
   lv_save(dst[L], D.lv[L], rbuf<L>(D), mbuf<L>(D));
   if constexpr (L < MAXD) save_levels<L + 1, MAXD>(dst, D);
 }
@@ -513,7 +557,7 @@ __device__ __forceinline__ void save_levels(Level* dst, const Dec<MAXD>& D) {
 // referenced record at or before it, when that record is long enough to have
 // checkpoints.  Returns how many values the jump passed over.
 template <int LL, int MAXD>
-__device__ __forceinline__ int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int32_t target) {
+TGA_HD int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int32_t target) {
   if (!S.vinfo) return 0;
   const int32_t first = S.vinfo[D.lv[LL].x];
   if (first < 0) return 0;
@@ -530,7 +574,7 @@ __device__ __forceinline__ int32_t dec_jump(const Streams& S, Dec<MAXD>& D, int3
 
 // Pulls the next value of level L's copy source from level L+1.
 template <int L, int MAXD>
-__device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
+TGA_HD int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
   if constexpr (L >= MAXD) {
     return kInf;
   } else {
@@ -571,11 +615,11 @@ __device__ __forceinline__ int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
 }
 
 template <int L, int MAXD>
-__device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
+TGA_HD int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   Level& v = D.lv[L];
   const Rd g{S.g, S.g_bits};
   // The deepest level has no copy source.
-  const int32_t m = (L < MAXD) ? min(min(v.hm, v.hi), min(v.hr, v.hc)) : min(min(v.hm, v.hi), v.hr);
+  const int32_t m = (L < MAXD) ? min32(min32(v.hm, v.hi), min32(v.hr, v.hc)) : min32(min32(v.hm, v.hi), v.hr);
   if (m == kInf) return kInf;
   ++v.kpos;
   if (m == v.hr) {
@@ -615,16 +659,16 @@ __device__ __forceinline__ int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
 
 // Next successor without consuming it.
 template <int MAXD>
-__device__ __forceinline__ int32_t dec_peek(const Dec<MAXD>& D) {
+TGA_HD int32_t dec_peek(const Dec<MAXD>& D) {
   const Level& v = D.lv[0];
-  return (MAXD > 0) ? min(min(v.hm, v.hi), min(v.hr, v.hc)) : min(min(v.hm, v.hi), v.hr);
+  return (MAXD > 0) ? min32(min32(v.hm, v.hi), min32(v.hr, v.hc)) : min32(min32(v.hm, v.hi), v.hr);
 }
 
 // Parses the record of x at bit offset `gpos` into level L (and the levels of
 // its reference chain).  Returns the outdegree d; `depth` receives the
 // deepest level used.
 template <int L, int MAXD>
-__device__ __forceinline__ int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos,
+TGA_HD int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos,
                                             int32_t& depth) {
   Level& v = D.lv[L];
   // The header is parsed through the residual buffer, which ends up at the
@@ -728,14 +772,14 @@ struct TsCursor {
   int32_t prev;
 };
 
-__device__ __forceinline__ int32_t ts_next(const Streams& S, TsCursor& c) {
+TGA_HD int32_t ts_next(const Streams& S, TsCursor& c) {
   c.prev = (int32_t)((uint32_t)c.prev + (uint32_t)nat2int(t_zeta2(S, c.b)));
   return c.prev;
 }
 
 // Saves the decoder at a chunk boundary.
 template <int MAXD>
-__device__ __forceinline__ void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t depth, const TsCursor& tc, int32_t x,
+TGA_HD void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t depth, const TsCursor& tc, int32_t x,
                                           int32_t k0) {
   c->x = x;
   c->k0 = k0;
@@ -745,46 +789,203 @@ __device__ __forceinline__ void ckpt_save(Ckpt* c, const Dec<MAXD>& D, int32_t d
   save_levels<0, MAXD>(c->lv, D);
 }
 
+// Next run of the top level's successor sequence: a value and how many
+// consecutive contacts carry it (cnt).  Only multiples repeat a value (they
+// are pulled out before copying, so no other source yields a multiple's
+// value), so a run is the rest of a multiple, or one value of the other
+// sources.  kInf when the list is exhausted.
 template <int MAXD>
-__device__ __forceinline__ void ckpt_load(const Streams& S, const Ckpt* c, Dec<MAXD>& D, TsCursor& tc) {
-  restore_levels<0, 0, MAXD>(S, D, c->lv, c->depth);
-  t_init(S, tc.b, c->tpos);
-  tc.prev = c->tprev;
+TGA_HD int32_t dec_next_run(const Streams& S, Dec<MAXD>& D, int32_t& cnt) {
+  Level& v = D.lv[0];
+  const int32_t o = (MAXD > 0) ? min32(min32(v.hi, v.hr), v.hc) : min32(v.hi, v.hr);
+  if (v.hm < o) {
+    const int32_t m = v.hm;
+    cnt = v.mrem;
+    v.kpos += cnt;
+    if (v.mleft > 0) {
+      v.hm += (int32_t)g_gamma(S, mbuf<0>(D)) + 1;
+      v.mrem = (int32_t)g_gamma(S, mbuf<0>(D)) + 2;
+      --v.mleft;
+    } else {
+      v.hm = kInf;
+      v.mrem = 0;
+    }
+    return m;
+  }
+  cnt = 1;
+  return dec_next<0, MAXD>(S, D);
 }
+
+// Per-lane LDS window over the timestamp stream, read by the walk.
+//
+// A lane's timestamps are one sequential stream of zeta_2 codes (about 14 to
+// 28 bits a contact).  Read eight bytes at a time from HBM, each lane would
+// touch each 128-byte line of its stream a dozen times over hundreds of
+// contacts, and with ~10^5 lanes in flight the L2 evicts the line between
+// touches (a pass fetched ~9x its stream bytes).  Instead the window is
+// topped up once per burst of contacts in whole 16-byte units (a
+// global_load_dwordx4 each, the units of one top-up back to back, so a line is
+// fetched once), and the codes are decoded from LDS.
+//
+// Layout: kStageDw stream dwords per lane in stream order at slot
+// (dword index mod kStageDw), slots kStageDw and kStageDw + 1 mirroring slots
+// 0 and 1 so a 96-bit read never wraps; slot i of a lane at s[i * stride]
+// (lanes interleaved: every lane of a wave hits its own bank).
+constexpr int32_t kStageDw = 16;  // staged dwords per lane (power of two, multiple of 4)
+struct TsStage {
+  uint32_t* s;
+  int32_t stride;
+  uint64_t base;  // stream dword index of relative dword 0 (a multiple of 4)
+  uint64_t lim;   // stream dword index one past the item's last code
+  uint32_t pos;   // bit offset of the next code from dword `base`
+  uint32_t wend;  // relative dword index one past the last staged dword
+  int32_t prev;   // the timestamp before the next code
+};
+
+// Stages the 16-byte unit at relative dword wend (stream words 2u, 2u + 1).
+TGA_HD void stage_unit(const Streams& S, TsStage& st) {
+  const uint64_t u = (st.base + st.wend) >> 2;
+  const uint64_t lastw = (uint64_t)S.t_last;
+  const uint64_t w0 = S.t[2 * u <= lastw ? 2 * u : lastw];
+  const uint64_t w1 = S.t[2 * u + 1 <= lastw ? 2 * u + 1 : lastw];
+  const uint32_t slot = st.wend & (kStageDw - 1);
+  uint32_t* p = st.s + (size_t)slot * st.stride;
+  p[0] = (uint32_t)(w0 >> 32);
+  p[st.stride] = (uint32_t)w0;
+  p[2 * st.stride] = (uint32_t)(w1 >> 32);
+  p[3 * st.stride] = (uint32_t)w1;
+  if (slot == 0) {
+    st.s[(size_t)kStageDw * st.stride] = (uint32_t)(w0 >> 32);
+    st.s[(size_t)(kStageDw + 1) * st.stride] = (uint32_t)w0;
+  }
+  st.wend += 4;
+}
+
+// Starts the window at bit `pos` for codes ending before bit `end` (the
+// first unit holds the code at pos).
+TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end, int32_t prev) {
+  if (pos >= S.t_bits) raise_err(S, kErrOverrun);
+  st.base = (pos >> 5) & ~(uint64_t)3;
+  st.pos = (uint32_t)(pos - st.base * 32);
+  st.wend = 0;
+  st.lim = (end + 31) >> 5;
+  st.prev = prev;
+}
+
+// Tops the window up: whole units while they fit behind the current dword
+// and hold codes of the item.  A 64-bit peek near the item's end may then see
+// stale dwords after its code, which no valid code depends on.  Positions are
+// kept relative to `base`, which moves up in steps of kStageDw dwords (slot
+// numbers stay put) before they could overflow.
+TGA_HD void stage_fill(const Streams& S, TsStage& st) {
+  if (st.pos >= (1u << 30)) {
+    const uint32_t m = (st.pos >> 5) & ~(uint32_t)(kStageDw - 1);
+    st.base += m;
+    st.pos -= m * 32;
+    st.wend -= m;
+  }
+  const uint32_t d = st.pos >> 5;
+  while (st.wend + 4 <= d + kStageDw && st.base + st.wend < st.lim) stage_unit(S, st);
+}
+
+// 64 stream bits at the current position (the window must hold dwords
+// pos/32 .. pos/32 + 2).
+TGA_HD uint64_t stage_peek(const TsStage& st) {
+  const uint32_t d = (st.pos >> 5) & (kStageDw - 1);
+  const uint32_t sh = st.pos & 31u;
+  const uint32_t* p = st.s + (size_t)d * st.stride;
+  const uint64_t hi = ((uint64_t)p[0] << 32) | p[st.stride];
+  const uint64_t lo = (uint64_t)p[2 * st.stride] << sh;
+  return (hi << sh) | (lo >> 32);
+}
+
+// Next timestamp: zeta_2 of the int2nat delta from the previous one
+// (EvolvingMultiGraph.java:131-144; bfs.cu:640-665).
+TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st) {
+  if ((st.pos >> 5) + 3 > st.wend) stage_fill(S, st);  // codes longer than the burst's share
+  uint32_t len;
+  const uint64_t v = dec_zeta<2>(S, stage_peek(st), len);
+  st.pos += len;
+  st.prev = (int32_t)((uint32_t)st.prev + (uint32_t)nat2int(v));
+  return st.prev;
+}
+
+// Per-lane ring of decoded runs (value, contacts) between the two phases of
+// a walk; entry i of a lane is at e[i * stride] (in the kernels: LDS, the
+// lanes of a block interleaved so that lanes at any ring positions touch
+// distinct banks).
+constexpr int32_t kRing = 4;    // runs held (power of two)
+constexpr int32_t kBurst = 4;   // contacts consumed per refill (<= kRing)
+struct RunRing {
+  int2* e;
+  int32_t stride;
+};
 
 // Walks contacts [k, kend) of one work item and hands (x, r, t, k) to
 // act.contact(); a false return stops the walk.
+//
+// The walk alternates two phases so that every lane of a wave runs the same
+// code at the same time: a refill that decodes neighbor runs into the ring
+// until it covers the next burst of contacts (the sub-stream merge and its
+// branches run once per run, not once per contact), and a consume phase that
+// decodes one timestamp per contact and takes the neighbor from the current
+// run.  On graphs whose pairs repeat (runs of tens of contacts) the refill
+// takes one or two passes per burst.
 template <int MAXD, class Act>
-__device__ __forceinline__ void walk(const Streams& S, Dec<MAXD>& D, TsCursor& tc, int32_t x, int32_t k, int32_t kend,
-                                     Act& act) {
-  for (; k < kend; ++k) {
-    tc.b.step(S.t, S.t_last);
-    dec_step(S, D);
-    const int32_t r = dec_next<0, MAXD>(S, D);
-    const int32_t t = ts_next(S, tc);
-    if ((uint32_t)r >= (uint32_t)S.n) {
-      raise_err(S, r == kInf ? kErrExhausted : kErrNeighbor);
-      break;
+TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t k, int32_t kend, Act& act,
+                 RunRing ring) {
+  uint32_t head = 0, tail = 0;  // ring positions; tail - head runs held
+  int32_t have = 0;             // contacts of the held runs
+  int32_t r = 0, c = 0;         // current run: neighbor, contacts left in it
+  bool more = true;             // the record has runs left
+  while (k < kend) {
+    while (more && c + have < kBurst && tail - head < (uint32_t)kRing) {
+      int32_t cnt;
+      const int32_t v = dec_next_run<MAXD>(S, D, cnt);
+      if ((uint32_t)v >= (uint32_t)S.n) {
+        if (v != kInf) raise_err(S, kErrNeighbor);
+        more = false;
+        break;
+      }
+      ring.e[(tail & (kRing - 1)) * ring.stride] = make_int2(v, cnt);
+      ++tail;
+      have += cnt;
     }
-    if (!act.contact(x, r, t, k)) break;
+    stage_fill(S, ts);
+    const int32_t stop = min32(kend, k + kBurst);
+    for (; k < stop; ++k) {
+      if (c == 0) {
+        if (head == tail) {  // fewer successors than the outdegree
+          raise_err(S, kErrExhausted);
+          return;
+        }
+        const int2 e = ring.e[(head & (kRing - 1)) * ring.stride];
+        ++head;
+        r = e.x;
+        c = e.y;
+        have -= c;
+      }
+      --c;
+      const int32_t t = stage_ts_next(S, ts);
+      if (!act.contact(x, r, t, k)) return;
+    }
   }
 }
 
 // Actions that keep per-item state define item(code); it runs first.
 template <class Act>
-__device__ __forceinline__ auto act_item(Act& a, int32_t code, int) -> decltype(a.item(code), void()) {
+TGA_HD auto act_item(Act& a, int32_t code, int) -> decltype(a.item(code), void()) {
   a.item(code);
 }
 template <class Act>
-__device__ __forceinline__ void act_item(Act&, int32_t, long) {}
+TGA_HD void act_item(Act&, int32_t, long) {}
 
 // Runs one work item.  code >= 0: the whole record of vertex `code`;
 // code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked,
 // split) may veto the item; act.end(x, chunked) runs after the walk.
 template <int MAXD, class Act>
-__device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act) {
+TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, RunRing ring, TsStage ts) {
   Dec<MAXD> D;
-  TsCursor tc;
   int32_t x, k = 0, kend;
   const bool chunked = code < 0;
   act_item(act, code, 0);
@@ -799,18 +1000,65 @@ __device__ __forceinline__ void run_item(const Streams& S, const Ckpt* ck, int32
       act.end(x, false);
       return;
     }
-    t_init(S, tc.b, tpos);
-    tc.prev = S.min_ts;
+    stage_init(S, ts, tpos, rec_tend(S, x), S.min_ts);
   } else {
     const Ckpt* c = ck + (-code - 1);
     x = c->x;
     if (!act.begin(x, true, Split{c->cont_slot, c->cont_r, c->tail_slot, c->tail_r})) return;
     k = c->k0;
     kend = c->k1;
-    ckpt_load<MAXD>(S, c, D, tc);
+    restore_levels<0, 0, MAXD>(S, D, c->lv, c->depth);
+    stage_init(S, ts, c->tpos, rec_tend(S, x), c->tprev);
   }
-  walk<MAXD>(S, D, tc, x, k, kend, act);
+  walk<MAXD>(S, D, ts, x, k, kend, act, ring);
   act.end(x, chunked);
+}
+
+// Decodes the long record of x once and saves a checkpoint every
+// chunk_stride(MAXD, S.flat_stride) contacts into out[0 .. nchunk).  A run of one neighbor
+// crossing a boundary gets a split-run slot: `slot0` + the index of the first
+// chunk it continues into (see Ckpt).  Returns the number of chunks.
+template <int MAXD>
+TGA_HD int32_t ckpt_build_record(const Streams& S, int32_t x, int64_t slot0, Ckpt* out) {
+  const int32_t kStride = chunk_stride(MAXD, S.flat_stride);
+  Dec<MAXD> D;
+  TsCursor tc;
+  int32_t depth = 0;
+  uint64_t gpos, tpos;
+  rec_pos(S, x, gpos, tpos);
+  const int32_t d = dec_init<0, MAXD>(S, D, x, gpos, depth);
+  t_init(S, tc.b, tpos);
+  tc.prev = S.min_ts;
+  ckpt_save<MAXD>(out, D, depth, tc, x, 0);
+  out[0].cont_slot = -1;
+  out[0].cont_r = -1;
+  int32_t nch = 1, prev = -1, run_slot = -1;
+  for (int32_t k = 0; k < d; ++k) {
+    if (k > 0 && k % kStride == 0) {
+      const int32_t nr = dec_peek<MAXD>(D);
+      const bool cont = nr == prev;
+      if (cont && run_slot < 0) run_slot = (int32_t)slot0 + nch;
+      out[nch - 1].k1 = k;
+      out[nch - 1].tail_slot = cont ? run_slot : -1;
+      out[nch - 1].tail_r = prev;
+      ckpt_save<MAXD>(out + nch, D, depth, tc, x, k);
+      out[nch].cont_slot = cont ? run_slot : -1;
+      out[nch].cont_r = nr;
+      ++nch;
+    }
+    const int32_t r = dec_next<0, MAXD>(S, D);
+    ts_next(S, tc);
+    if (r != prev) run_slot = -1;
+    prev = r;
+  }
+  out[nch - 1].k1 = d;
+  out[nch - 1].tail_slot = -1;
+  out[nch - 1].tail_r = -1;
+  for (int32_t j = 0; j < nch; ++j) {
+    out[j].nchunk = nch;
+    out[j].stride = kStride;
+  }
+  return nch;
 }
 
 }  // namespace tga

@@ -37,6 +37,7 @@
 
 #include "../../../include/tga.h"
 #include "decode.hpp"
+#include "loader.hpp"
 
 namespace tga {
 // sort.hip: stable radix sort of (key, value) pairs by key bits [0, end_bit).
@@ -96,20 +97,6 @@ struct Window {
   __device__ __forceinline__ bool active(int32_t v) const { return v >= ab && v < ae; }
 };
 
-__device__ __forceinline__ uint64_t fmix64(uint64_t k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdULL;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ULL;
-  k ^= k >> 33;
-  return k;
-}
-
-__device__ __forceinline__ uint64_t contact_hash(int32_t x, int32_t r, int32_t t) {
-  return fmix64((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(uint32_t)r * 0xC2B2AE3D27D4EB4FULL ^
-                (uint64_t)(uint32_t)t * 0x165667B19E3779F9ULL);
-}
-
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -125,6 +112,17 @@ struct Items {
   const int32_t* code;
   uint32_t count;
 };
+
+// The calling lane's ring of decoded runs and its timestamp window
+// (decode.hpp walk) in LDS, for blocks of at most kBlock threads, interleaved
+// by thread.
+#define LANE_RING()                                                       \
+  __shared__ int2 ring_lds_[kRing * kBlock];                              \
+  __shared__ uint32_t stage_lds_[(kStageDw + 2) * kBlock];                \
+  const RunRing ring_{ring_lds_ + threadIdx.x, (int32_t)blockDim.x};      \
+  const TsStage stage_ {                                                  \
+    stage_lds_ + threadIdx.x, (int32_t)blockDim.x, 0, 0, 0, 0, 0          \
+  }
 
 #define FOR_ITEMS(I, i) \
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (I).count; i += gridDim.x * blockDim.x)
@@ -154,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int
   }
 }
 
-// Decodes each long record once and saves a checkpoint every chunk_stride(MAXD)
+// Decodes each long record once and saves a checkpoint every chunk_stride
 // contacts.  A run of one neighbor crossing a boundary gets a split-run slot:
 // the index of the first chunk it continues into (see Ckpt).
 template <int MAXD>
@@ -162,47 +160,7 @@ __global__ void __launch_bounds__(64) k_ckpt_build(Streams S, const int32_t* big
                                                    Ckpt* ck, int32_t* nchunk_out) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nbig) return;
-  const int32_t x = bigv[i];
-  Ckpt* out = ck + slot[i];
-  constexpr int32_t kStride = chunk_stride(MAXD);
-  Dec<MAXD> D;
-  TsCursor tc;
-  int32_t depth = 0;
-  uint64_t gpos, tpos;
-  rec_pos(S, x, gpos, tpos);
-  const int32_t d = dec_init<0, MAXD>(S, D, x, gpos, depth);
-  t_init(S, tc.b, tpos);
-  tc.prev = S.min_ts;
-  ckpt_save<MAXD>(out, D, depth, tc, x, 0);
-  out[0].cont_slot = -1;
-  out[0].cont_r = -1;
-  int32_t nch = 1, prev = -1, run_slot = -1;
-  for (int32_t k = 0; k < d; ++k) {
-    if (k > 0 && k % kStride == 0) {
-      const int32_t nr = dec_peek<MAXD>(D);
-      const bool cont = nr == prev;
-      if (cont && run_slot < 0) run_slot = (int32_t)slot[i] + nch;
-      out[nch - 1].k1 = k;
-      out[nch - 1].tail_slot = cont ? run_slot : -1;
-      out[nch - 1].tail_r = prev;
-      ckpt_save<MAXD>(out + nch, D, depth, tc, x, k);
-      out[nch].cont_slot = cont ? run_slot : -1;
-      out[nch].cont_r = nr;
-      ++nch;
-    }
-    const int32_t r = dec_next<0, MAXD>(S, D);
-    ts_next(S, tc);
-    if (r != prev) run_slot = -1;
-    prev = r;
-  }
-  out[nch - 1].k1 = d;
-  out[nch - 1].tail_slot = -1;
-  out[nch - 1].tail_r = -1;
-  for (int32_t j = 0; j < nch; ++j) {
-    out[j].nchunk = nch;
-    out[j].stride = kStride;
-  }
-  nchunk_out[i] = nch;
+  nchunk_out[i] = ckpt_build_record<MAXD>(S, bigv[i], slot[i], ck + slot[i]);
 }
 
 // Selection of the vertices a round expands (evaluated on device arrays).
@@ -374,10 +332,15 @@ __global__ void __launch_bounds__(kBlock) k_expand(const int32_t* front, int32_t
 // -------------------------------------------------------------- actions ---
 
 struct RaAct {
-  uint64_t cs = 0, cnt = 0;
-  __device__ bool begin(int32_t, bool, Split) { return true; }
-  __device__ bool contact(int32_t x, int32_t r, int32_t t, int32_t) {
-    cs += contact_hash(x, r, t);
+  uint64_t cs = 0;
+  uint32_t cnt = 0;  // contacts of this lane (far below 2^32 per pass)
+  uint32_t kx = 0;
+  __device__ bool begin(int32_t x, bool, Split) {
+    kx = contact_key_x(x);
+    return true;
+  }
+  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
+    cs += contact_term(kx, r, t);
     ++cnt;
     return true;
   }
@@ -387,7 +350,8 @@ struct RaAct {
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items I, unsigned long long* out) {
   RaAct a;
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
   const unsigned long long cs = wave_sum((unsigned long long)a.cs);
   const unsigned long long cnt = wave_sum((unsigned long long)a.cnt);
   // Spread over kSumSlots accumulator pairs: tens of thousands of waves
@@ -432,7 +396,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ctimes(Streams S, const Ckpt* ck, Items I, Window W, int64_t* ct,
                                                    const uint64_t* coff) {
   CtAct a{W, ct, coff};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Degree count and Java running mean of one record (whole records also get
@@ -484,7 +449,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_dc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* count,
                                                int32_t* centr) {
   DcAct a{W, count, centr};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Java's long division a / k (truncating toward zero) for k >= 1 and
@@ -738,7 +704,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_runs(Streams S, const Ckpt* ck, Items I, Window A, uint32_t* cnt,
                                                  const uint64_t* off, int32_t* rx, int32_t* rr, int32_t* rk) {
   RunAct a{A, cnt, off, rx, rr, rk, S.n};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Run ends (the next run of the record, or its outdegree) and sort keys
@@ -1362,11 +1329,12 @@ __global__ void __launch_bounds__(kBlock) k_prb_out(int32_t n, int32_t nq, const
 template <int MAXD>
 __global__ void __launch_bounds__(64) k_dc_long(Streams S, const int32_t* bigv, int32_t nbig, Window W,
                                                 int32_t* centr) {
+  LANE_RING();
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nbig) return;
   DcAct a{W, nullptr, nullptr};
   const int32_t x = bigv[i];
-  run_item<MAXD>(S, nullptr, x, a);
+  run_item<MAXD>(S, nullptr, x, a, ring_, stage_);
   centr[x] = W.active(x) ? dc_formula(a.deg, a.avg, W.ws, W.we) : dc_formula(0, 0, W.ws, W.we);
 }
 
@@ -1389,7 +1357,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_bfs(Streams S, const Ckpt* ck, Items I, Window W, int32_t* level,
                                                 int32_t lvl) {
   BfsAct a{W, level, lvl};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Min-label propagation: a vertex whose label drops in round r is stamped r
@@ -1419,7 +1388,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* label,
                                                int32_t* stamp, int32_t round) {
   CcAct a{W, label, stamp, round};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
@@ -1471,7 +1441,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_gc(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
                                                int32_t* color, int32_t* blocked, int32_t* blocker, int32_t round) {
   GcAct a{W, prio, color, blocked, blocker, round};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint32_t nbig, Window W, int32_t* color,
@@ -1541,7 +1512,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_gc_list(Streams S, const Ckpt* ck, Items I, Window W, const int32_t* prio,
                                                     uint32_t* cnt, const uint64_t* off, int32_t* hp) {
   GcListAct a{W, prio, cnt, off, hp, S.n};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Exclusive prefix of v over the block (thread order); total receives the sum.
@@ -1798,7 +1770,8 @@ __global__ void __launch_bounds__(kBlock) k_bfsb(Streams S, const Ckpt* ck, Item
   }
   __syncthreads();
   BfsbAct a{W, front, vis, next, bound, mask, tab.nb};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Ends step `lvl` of a batched BFS: the bits newly set in next[v] become the
@@ -1910,7 +1883,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ea(Streams S, const Ckpt* ck, Items I, Window W, int32_t* arr,
                                                int32_t* stamp, int32_t round, uint64_t* runkey) {
   EaAct a{W, arr, stamp, round, runkey};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Resolved split runs: the slot's pair (x, r) and its first window contact t.
@@ -1977,7 +1951,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ld(Streams S, const Ckpt* ck, Items I, Window W, int32_t* ld,
                                                uint32_t* changed) {
   LdAct a{W, ld, changed};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Window contact count (PageRank divisor).
@@ -2005,7 +1980,8 @@ struct WdegAct {
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_wdeg(Streams S, const Ckpt* ck, Items I, Window W, int32_t* du) {
   WdegAct a{W, du};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 __device__ __forceinline__ float pr_coef(float damping, const float* rank, const int32_t* du, int32_t x) {
@@ -2048,7 +2024,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_pr_push(Streams S, const Ckpt* ck, Items I, Window W, const float* rank,
                                                     const int32_t* du, float damping, double* acc, uint64_t* runkey) {
   PrAct a{W, rank, du, damping, acc, runkey};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pr_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
@@ -2093,7 +2070,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_hits_pull(Streams S, const Ckpt* ck, Items I, Window W, const float* auth,
                                                       double* acc, uint64_t* runkey) {
   HubAct a{W, auth, acc, runkey};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 struct AuthAct {
@@ -2125,7 +2103,8 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_hits_push(Streams S, const Ckpt* ck, Items I, Window W, const float* hub,
                                                       double* acc, uint64_t* runkey) {
   AuthAct a{W, hub, acc, runkey};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // mode 0: hub[x] += auth[r]; mode 1: auth[r] += hub[x] (per resolved split run).
@@ -2211,7 +2190,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Streams S, const Ckpt* ck, Ite
                                                   const uint64_t* off, uint32_t* key, uint64_t* val,
                                                   uint64_t* runkey) {
   PairAct a{W, cnt, off, key, val, runkey, S.n};
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a);
+  LANE_RING();
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
 }
 
 // Split-run pairs into key/val[0, nck) (one slot per checkpoint).  An empty
@@ -2320,90 +2300,6 @@ __global__ void __launch_bounds__(kBlock) k_scale_store(const double* x, const d
   const double s = scale[0];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = (float)(s > 0 ? x[i] / s : x[i]);
-}
-
-// ----------------------------------------------------------------- loader ---
-
-bool read_file(const std::string& path, std::vector<uint8_t>& out) {
-  FILE* f = std::fopen(path.c_str(), "rb");
-  if (!f) return false;
-  std::fseek(f, 0, SEEK_END);
-  const long len = std::ftell(f);
-  std::rewind(f);
-  if (len < 0) {
-    std::fclose(f);
-    return false;
-  }
-  out.resize((size_t)len);
-  const bool ok = std::fread(out.data(), 1, out.size(), f) == out.size();
-  std::fclose(f);
-  return ok;
-}
-
-// Bytes -> 64-bit words whose most significant byte is the first stream byte,
-// plus three zero words of padding (BitBuf reads up to two words ahead).
-// Stream words plus kPadWords zero words (bit buffers read up to five words
-// from any position inside the stream).
-constexpr size_t kPadWords = 5;
-// Large streams are converted by up to 16 threads, one slice of words each
-// (a little-endian load of 8 bytes, byte-swapped, puts the first byte on top).
-std::vector<uint64_t> be_words(const std::vector<uint8_t>& b) {
-  const size_t nw = (b.size() + 7) / 8;
-  std::vector<uint64_t> w(nw + kPadWords, 0);
-  auto slice = [&](size_t lo, size_t hi) {
-    const size_t bytes = std::min(b.size(), hi * 8) - lo * 8;
-    std::memcpy(w.data() + lo, b.data() + lo * 8, bytes);
-    for (size_t i = lo; i < hi; ++i) w[i] = __builtin_bswap64(w[i]);
-  };
-  const size_t nth = nw >= ((size_t)1 << 22) ? std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())) : 1;
-  if (nth == 1) {
-    slice(0, nw);
-  } else {
-    std::vector<std::thread> pool;
-    for (size_t k = 0; k < nth; ++k) pool.emplace_back(slice, nw * k / nth, nw * (k + 1) / nth);
-    for (auto& th : pool) th.join();
-  }
-  return w;
-}
-
-std::vector<uint64_t> le_words(const std::vector<uint8_t>& b) {
-  std::vector<uint64_t> w((b.size() + 7) / 8 + 2, 0);
-  std::memcpy(w.data(), b.data(), b.size());
-  return w;
-}
-
-int msb64_host(uint64_t v) { return 63 - __builtin_clzll(v); }
-
-// Sampled select directory over the upper bits (see decode.hpp).
-void build_select(const std::vector<uint64_t>& up, size_t words, uint64_t& ones, std::vector<uint64_t>& samp,
-                  std::vector<uint64_t>& spill) {
-  std::vector<uint64_t> block;
-  ones = 0;
-  samp.clear();
-  spill.clear();
-  auto flush = [&]() {
-    if (block.empty()) return;
-    if (block.back() - block.front() > kSpanBits) {
-      samp.push_back((1ull << 63) | spill.size());
-      for (size_t i = 0; i < 64; ++i) spill.push_back(i < block.size() ? block[i] : block.back());
-    } else {
-      samp.push_back(block.front());
-    }
-    block.clear();
-  };
-  for (size_t w = 0; w < words; ++w) {
-    uint64_t word = up[w];
-    while (word) {
-      const int b = __builtin_ctzll(word);
-      word &= word - 1;
-      block.push_back(w * 64 + (uint64_t)b);
-      ++ones;
-      if (block.size() == 64) flush();
-    }
-  }
-  flush();
-  samp.push_back(0);
-  spill.push_back(0);
 }
 
 }  // namespace
@@ -2736,6 +2632,11 @@ static int build_schedule(tga_graph* g) {
   }
   g->info.contacts = contacts;
   g->info.max_degree = maxd;
+  S.flat_stride = flat_stride_for(contacts);
+  if (const char* fs = std::getenv("TGA_FLAT_STRIDE")) {  // measurement override
+    const long v = std::strtol(fs, nullptr, 10);
+    if (v >= 16 && v <= 1 << 16 && (v & (v - 1)) == 0) S.flat_stride = (int32_t)v;
+  }
 
   // Long records and their checkpoint slots (ceil(d / stride) chunks each).
   std::vector<int32_t> bigv[kClasses];
@@ -2743,10 +2644,10 @@ static int build_schedule(tga_graph* g) {
   int64_t slots = 0;
   for (int64_t x = 0; x < n; ++x) {
     const int c = depth[(size_t)x];
-    if (deg[(size_t)x] <= chunk_stride(c)) continue;
+    if (deg[(size_t)x] <= chunk_stride(c, S.flat_stride)) continue;
     bigv[c].push_back((int32_t)x);
     slotv[c].push_back(slots);
-    slots += (deg[(size_t)x] + chunk_stride(c) - 1) / chunk_stride(c);
+    slots += (deg[(size_t)x] + chunk_stride(c, S.flat_stride) - 1) / chunk_stride(c, S.flat_stride);
   }
   g->nck = slots;
   const size_t nslots = (size_t)std::max<int64_t>(slots, 1);
@@ -2803,7 +2704,7 @@ static int build_schedule(tga_graph* g) {
   std::vector<std::pair<int64_t, int32_t>> chain[kGroups];  // (cost, code)
   auto chain_cost = [](int64_t contacts, int depth) { return contacts * (1 + 2 * depth); };
   for (int c = 0; c < kClasses; ++c) {
-    const int32_t stride = chunk_stride(c);
+    const int32_t stride = chunk_stride(c, S.flat_stride);
     for (size_t i = 0; i < bigv[c].size(); ++i) {
       const int32_t x = bigv[c][i];
       const int32_t nch = nch_host[nch_at[c] + i];
@@ -2825,20 +2726,31 @@ static int build_schedule(tga_graph* g) {
     std::vector<int32_t> small;
     for (int64_t x = 0; x < n; ++x) {
       const int c = depth[(size_t)x];
-      if (deg[(size_t)x] <= 0 || deg[(size_t)x] > chunk_stride(c)) continue;
+      if (deg[(size_t)x] <= 0 || deg[(size_t)x] > chunk_stride(c, S.flat_stride)) continue;
       if (c == 0)
         small.push_back((int32_t)x);
       else
         chain[group_of(c)].emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
     }
-    // Longest length class (msb of the degree) first, id order within a
-    // class: a stable counting sort over the 64 classes.
-    size_t at[65] = {};
-    for (const int32_t x : small) ++at[64 - msb64_host((uint64_t)deg[(size_t)x])];
-    for (int k = 1; k <= 64; ++k) at[k] += at[k - 1];
+    // Longest length class first, id order within a class: a stable
+    // counting sort.  A class is the msb of the degree shifted right by
+    // TGA_FLAT_CLASS_SHIFT (default 0): coarser classes put more consecutive
+    // ids next to each other (shared cache lines of the streams) at the cost
+    // of less even record lengths within a wave.
+    // TGA_FLAT_CLASS_SHIFT = -1 sorts by the exact degree instead (whole
+    // flat records have at most flat_stride contacts).
+    int cshift = 0;
+    if (const char* e = std::getenv("TGA_FLAT_CLASS_SHIFT")) cshift = std::max(-1, std::min(6, std::atoi(e)));
+    const int32_t ncls = cshift < 0 ? S.flat_stride + 1 : 64;
+    auto cls = [&](int32_t x) {
+      return cshift < 0 ? deg[(size_t)x] : msb64_host((uint64_t)deg[(size_t)x]) >> cshift;
+    };
+    std::vector<size_t> at((size_t)ncls + 1, 0);
+    for (const int32_t x : small) ++at[(size_t)(ncls - cls(x))];
+    for (int32_t k = 1; k <= ncls; ++k) at[(size_t)k] += at[(size_t)k - 1];
     const size_t first = items[0].size();
     items[0].resize(first + small.size());
-    for (const int32_t x : small) items[0][first + at[63 - msb64_host((uint64_t)deg[(size_t)x])]++] = x;
+    for (const int32_t x : small) items[0][first + at[(size_t)(ncls - 1 - cls(x))]++] = x;
     for (int c = 1; c < kGroups; ++c) {
       std::stable_sort(chain[c].begin(), chain[c].end(),
                        [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& b) {

@@ -722,22 +722,22 @@ extern "C" int tgaf_gen_netflow(int64_t n, double exponent, double count_min, do
   return TGAF_OK;
 }
 
-// The neighbor-access checksum of a raw contact list: the wrapping sum of
-// fmix64(x * K1 ^ r * K2 ^ t * K3) over its contacts, with x, r and t taken
-// as unsigned 32-bit values (the sum the decoding pass returns).
+// The neighbor-access checksum of a raw contact list: the wrapping 64-bit sum
+// over its contacts of h = t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77, h ^= h >> 16,
+// h *= 0x7FEB352D, h ^= h >> 15 (uint32 arithmetic; the sum the decoding pass
+// returns).
 extern "C" int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m,
                                      uint64_t* out) {
   if (m < 0 || (m > 0 && (!src || !dst || !t)) || !out) return fail(TGAF_EINVAL, "tgaf_contact_checksum: bad arguments");
   constexpr int64_t kChunks = 64;
   std::vector<uint64_t> part(kChunks, 0);
   std::atomic<int64_t> next{0};
-  auto mix = [](uint64_t k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdULL;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ULL;
-    k ^= k >> 33;
-    return k;
+  auto term = [](uint32_t x, uint32_t r, uint32_t t) -> uint64_t {
+    uint32_t h = t ^ (r * 0x9E3779B1u) ^ (x * 0x85EBCA77u);
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    return h;
   };
   auto work = [&]() {
     for (;;) {
@@ -745,8 +745,7 @@ extern "C" int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, con
       if (c >= kChunks) break;
       uint64_t sum = 0;
       for (int64_t i = m * c / kChunks; i < m * (c + 1) / kChunks; ++i)
-        sum += mix((uint64_t)(uint32_t)src[i] * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(uint32_t)dst[i] * 0xC2B2AE3D27D4EB4FULL ^
-                   (uint64_t)(uint32_t)(int32_t)t[i] * 0x165667B19E3779F9ULL);
+        sum += term((uint32_t)src[i], (uint32_t)dst[i], (uint32_t)(int32_t)t[i]);
       part[(size_t)c] = sum;
     }
   };

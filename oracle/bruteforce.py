@@ -14,27 +14,19 @@ INF32 = 0x7FFFFFFF
 FULL_WINDOW = (-(1 << 31), (1 << 31) - 1)
 LD_UNREACHED = -(((1 << 31) - 1) // 2)  # -1073741823
 
-_K1, _K2, _K3 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F, 0x165667B19E3779F9
-
-
-def fmix64(k: np.ndarray) -> np.ndarray:
-    k = k.astype(np.uint64)
-    with np.errstate(over="ignore"):
-        k ^= k >> np.uint64(33)
-        k *= np.uint64(0xFF51AFD7ED558CCD)
-        k ^= k >> np.uint64(33)
-        k *= np.uint64(0xC4CEB9FE1A85EC53)
-        k ^= k >> np.uint64(33)
-    return k
-
 
 def contact_hashes(src, dst, t) -> np.ndarray:
-    x = np.asarray(src).astype(np.uint32).astype(np.uint64)
-    r = np.asarray(dst).astype(np.uint32).astype(np.uint64)
-    tt = np.asarray(t).astype(np.int64).astype(np.uint32).astype(np.uint64)
+    """Per-contact checksum terms: h = t ^ r*0x9E3779B1 ^ x*0x85EBCA77;
+    h ^= h >> 16; h *= 0x7FEB352D; h ^= h >> 15 (uint32 arithmetic)."""
+    x = np.asarray(src).astype(np.uint32)
+    r = np.asarray(dst).astype(np.uint32)
+    tt = np.asarray(t).astype(np.int64).astype(np.uint32)
     with np.errstate(over="ignore"):
-        key = (x * np.uint64(_K1)) ^ (r * np.uint64(_K2)) ^ (tt * np.uint64(_K3))
-    return fmix64(key)
+        h = tt ^ (r * np.uint32(0x9E3779B1)) ^ (x * np.uint32(0x85EBCA77))
+        h ^= h >> np.uint32(16)
+        h *= np.uint32(0x7FEB352D)
+        h ^= h >> np.uint32(15)
+    return h.astype(np.uint64)
 
 
 def checksum(src, dst, t) -> int:

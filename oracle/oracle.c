@@ -334,15 +334,16 @@ static int act(const orc* g, const orc_q* q, int64_t v) {
   return v >= q->active_begin && v < q->active_end && v < g->n;
 }
 
-/* Contact checksum: wrapping sum over contacts of fmix64 of the mixed triple
- * (the reference discards decoded values; this is the build's own RA output). */
-static uint64_t fmix64(uint64_t k) {
-  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
-  return k;
-}
+/* Contact checksum: wrapping 64-bit sum over contacts of a 32-bit mix of the
+ * triple (the reference discards decoded values; this is the build's own RA
+ * output): h = t ^ r*0x9E3779B1 ^ x*0x85EBCA77; h ^= h >> 16;
+ * h *= 0x7FEB352D; h ^= h >> 15 (uint32 arithmetic). */
 uint64_t orc_contact_hash(int32_t x, int32_t r, int32_t t) {
-  return fmix64((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL ^ (uint64_t)(uint32_t)r * 0xC2B2AE3D27D4EB4FULL ^
-                (uint64_t)(uint32_t)t * 0x165667B19E3779F9ULL);
+  uint32_t h = (uint32_t)t ^ ((uint32_t)r * 0x9E3779B1u) ^ ((uint32_t)x * 0x85EBCA77u);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  return h;
 }
 
 /* Neighbor access (neighbor_access.cu:1438-1595; BaselineRunner.java:273-297):

@@ -62,6 +62,27 @@ def test_neighbor_access(ds):
     assert keep.any()
 
 
+def test_checksum_term_known_answers():
+    """The checksum term in plain Python integers (uint32 wrap-around) for a
+    few triples, against the numpy, C-oracle and format-kit versions."""
+    import oracle as O
+
+    def term(x, r, t):
+        m = 0xFFFFFFFF
+        h = (t & m) ^ ((r * 0x9E3779B1) & m) ^ ((x * 0x85EBCA77) & m)
+        h ^= h >> 16
+        h = (h * 0x7FEB352D) & m
+        return h ^ (h >> 15)
+
+    trip = [(0, 0, 0), (1, 2, 3), (33554431, 7, 1073741823), (5, 5, -17), (2 ** 31 - 2, 0, 2 ** 31 - 1)]
+    want = [term(x, r, t) for x, r, t in trip]
+    assert want[0] == 0 and len(set(want[1:])) == 4
+    x, r, t = (np.array(v) for v in zip(*trip))
+    assert B.contact_hashes(x, r, t).tolist() == want
+    assert [O.contact_hash(int(a), int(b), int(c)) for a, b, c in trip] == want
+    assert fmt.contact_checksum(x, r, t) == sum(want) % (1 << 64)
+
+
 def test_neighbor_access_threads(ds):
     """The threaded variant (used only to time the CPU at all cores) returns
     the single-thread checksum and count, with any thread count."""
