@@ -187,6 +187,7 @@ def large_legs(args) -> int:
     root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
     desc = datasets.ensure(root, args.workload)
     g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    g.autotune(passes=2)
     out = large_workload_times(g, desc.get("t_min", 0), desc.get("t_max", 1 << 24))
     g.close()
     print(json.dumps(out), flush=True)
@@ -522,6 +523,7 @@ def query_legs(args) -> int:
     root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
     desc = datasets.ensure(root, args.workload)
     g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    g.autotune(passes=2)
     t_lo, t_hi = desc.get("t_min", 0), desc.get("t_max", 1 << 24)
     bmix = batched_mix(g, t_lo, t_hi)
     log(f"[bench] batched 256-query mix on {args.workload}: {bmix}")
@@ -532,6 +534,8 @@ def query_legs(args) -> int:
         # 100/s, 1k/s and 10k/s.
         workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
                    for k in ("DC", "PR", "CC")}
+        for h in workers.values():
+            h.autotune(passes=2)
         smix = served_mix(g, workers, t_lo, t_hi, rates=((100, 200), (1000, 1000), (10000, 4000)))
         for h in workers.values():
             h.close()
@@ -565,6 +569,8 @@ def query_legs(args) -> int:
         log(f"[bench] served BFS: {served}")
         workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
                    for k in ("DC", "PR", "CC")}
+        for h in workers.values():
+            h.autotune(passes=2)
         smix = served_mix(g, workers, t_lo, t_hi)
         for h in workers.values():
             h.close()
@@ -778,6 +784,7 @@ def main() -> int:
                    "graph_bytes": info["graph_bytes"], "timestamps_bytes": info["timestamps_bytes"],
                    "parallelism": "single GPU"},
         "roofline": rec["roofline"],
+        "autotune": rec["autotune"],
         "checksum": rec["checksum"],
         "cpu_baseline": cpu,
         "workloads_ms": workloads,
@@ -803,6 +810,8 @@ def ra_pass(g, desc: dict, steps: int, warmup: int, traffic_bytes) -> dict:
     contact list's (size-independent parity at full size)."""
     info = g.info
     n, contacts, name = info["n"], info["contacts"], desc["name"]
+    tune = g.autotune(passes=3)  # the walk's burst length, kept for the handle (outside the timed region)
+    log(f"[bench] {name}: autotune {tune}")
     for _ in range(warmup):
         g.neighbor_access()
     g.profile(True)
@@ -832,6 +841,7 @@ def ra_pass(g, desc: dict, steps: int, warmup: int, traffic_bytes) -> dict:
                 "kernel_ms": round(avg_kernel_s * 1e3, 4), "launches_per_pass": launches // steps,
                 "algorithmic_bytes": int(alg_bytes)}
     return {"value": contacts * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "roofline": roofline,
+            "autotune": tune,
             "checksum": {"value": f"{cs:#018x}", "equals_raw_list": raw_cs is not None and cs == raw_cs}}
 
 

@@ -106,6 +106,7 @@ struct Streams {
   int32_t n;       // vertices; ids are [0, n)
   int32_t min_ts;  // .timestamps header value
   int32_t flat_stride;  // checkpoint stride of records without references (chunk_stride)
+  int32_t burst;        // contacts per consume phase of a walk (kBurstDefault unless tuned)
   int32_t* err;    // sticky error word
   // Decode checkpoints of long records: vinfo[x] >= 0 is the first
   // checkpoint of x, checkpoint j covering contacts [j * stride, (j + 1) *
@@ -914,8 +915,11 @@ TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st) {
 // a walk; entry i of a lane is at e[i * stride] (in the kernels: LDS, the
 // lanes of a block interleaved so that lanes at any ring positions touch
 // distinct banks).
-constexpr int32_t kRing = 4;    // runs held (power of two)
-constexpr int32_t kBurst = 4;   // contacts consumed per refill (<= kRing)
+constexpr int32_t kRing = 4;  // runs held (power of two)
+// Contacts consumed per refill at most (Streams::burst, set per graph by the
+// autotuner): long bursts amortise the refill over long runs of one neighbor;
+// a burst ends early when the ring's runs are used up.
+constexpr int32_t kBurstDefault = 4, kBurstMax = 64;
 struct RunRing {
   int2* e;
   int32_t stride;
@@ -939,7 +943,7 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
   int32_t r = 0, c = 0;         // current run: neighbor, contacts left in it
   bool more = true;             // the record has runs left
   while (k < kend) {
-    while (more && c + have < kBurst && tail - head < (uint32_t)kRing) {
+    while (more && c + have < S.burst && tail - head < (uint32_t)kRing) {
       int32_t cnt;
       const int32_t v = dec_next_run<MAXD>(S, D, cnt);
       if ((uint32_t)v >= (uint32_t)S.n) {
@@ -952,11 +956,12 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
       have += cnt;
     }
     stage_fill(S, ts);
-    const int32_t stop = min32(kend, k + kBurst);
+    const int32_t stop = min32(kend, k + S.burst);
     for (; k < stop; ++k) {
       if (c == 0) {
-        if (head == tail) {  // fewer successors than the outdegree
-          raise_err(S, kErrExhausted);
+        if (head == tail) {
+          if (more) break;  // the ring's runs covered fewer contacts than a burst
+          raise_err(S, kErrExhausted);  // fewer successors than the outdegree
           return;
         }
         const int2 e = ring.e[(head & (kRing - 1)) * ring.stride];

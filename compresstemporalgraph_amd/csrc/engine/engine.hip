@@ -2877,6 +2877,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   S.ti.l = lt;
   S.n = (int32_t)n;
   S.min_ts = (int32_t)min_ts;
+  S.burst = kBurstDefault;
   if ((rc = g->alloc(g->d_err, 1))) return bail(rc);
   S.err = g->d_err;
   if (hipMemset(g->d_err, 0, sizeof(int32_t)) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipMemset failed"));
@@ -3020,6 +3021,51 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   }
   if (checksum) *checksum = cs;
   if (contacts) *contacts = (int64_t)cnt;
+  return TGA_OK;
+}
+
+// Autotuner: the walk's burst length (contacts decoded per refill of the
+// neighbor-run ring) decides how far the refill's branches are amortised
+// (long runs of one neighbor favour long bursts, single-contact neighbors
+// short ones).  Each candidate is timed over full neighbor-access passes and
+// the fastest is kept for every later call on the handle.
+extern "C" int tga_autotune(tga_graph* g, int32_t passes, tga_tune_report* report) {
+  if (!g || passes < 1 || passes > 1000) return fail(TGA_EINVAL, "tga_autotune: bad arguments");
+  HIP_TRY(hipSetDevice(g->device));
+  static const int32_t cand[] = {4, 8, 16, 32};
+  constexpr int nc = (int)(sizeof cand / sizeof cand[0]);
+  static_assert(nc <= TGA_TUNE_MAX, "report too small");
+  tga_tune_report rep{};
+  const bool was = g->prof;
+  g->harvest();
+  double best = 0;
+  int32_t pick = g->S.burst;
+  for (int i = 0; i < nc; ++i) {
+    g->S.burst = cand[i];
+    int rc = tga_neighbor_access(g, nullptr, 0, nullptr, nullptr);  // warm
+    if (rc) return rc;
+    g->prof = true;
+    g->prof_ms = 0;
+    g->prof_launches = 0;
+    for (int32_t p = 0; p < passes && !rc; ++p) rc = tga_neighbor_access(g, nullptr, 0, nullptr, nullptr);
+    g->harvest();
+    g->prof = was;
+    if (rc) return rc;
+    const double ms = g->prof_ms / passes;
+    rep.setting[i] = cand[i];
+    rep.pass_ms[i] = ms;
+    if (i == 0 || ms < best) {
+      best = ms;
+      pick = cand[i];
+    }
+  }
+  g->prof_ms = 0;
+  g->prof_launches = 0;
+  g->S.burst = pick;
+  rep.candidates = nc;
+  rep.burst = pick;
+  rep.flat_stride = g->S.flat_stride;
+  if (report) *report = rep;
   return TGA_OK;
 }
 

@@ -52,6 +52,11 @@ class GraphInfo(C.Structure):
                 ("chunks", C.c_int64)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("burst", C.c_int32), ("flat_stride", C.c_int32),
+                ("setting", C.c_int32 * 8), ("pass_ms", C.c_double * 8)]
+
+
 def lib():
     """Loads libtga.so (built in-tree by ``build.build_engine``); raises if absent."""
     global _LIB
@@ -89,6 +94,7 @@ def lib():
         L.tga_profile.argtypes = [vp, C.c_int]
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
         L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
+        L.tga_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
         _LIB = L
     return _LIB
 
@@ -181,6 +187,15 @@ class TemporalGraph:
         info = GraphInfo()
         _check(lib().tga_info(self.h, C.byref(info)))
         return int(info.device_bytes)
+
+    def autotune(self, passes: int = 3) -> dict:
+        """Times the decode walk's burst-length candidates over full
+        neighbor-access passes and keeps the fastest (tga_autotune); returns
+        the report."""
+        rep = TuneReport()
+        _check(lib().tga_autotune(self.h, passes, C.byref(rep)))
+        return {"burst": rep.burst, "flat_stride": rep.flat_stride,
+                "pass_ms": {int(rep.setting[i]): round(rep.pass_ms[i], 4) for i in range(rep.candidates)}}
 
     def debug_fail_alloc(self, nth: int) -> None:
         """Error-path test hook: the nth device allocation from now fails with

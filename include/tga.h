@@ -238,6 +238,25 @@ int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t 
 int tga_profile(tga_graph* g, int enable);
 int tga_profile_read(tga_graph* g, double* kernel_ms, int64_t* launches, int reset);
 
+/* Autotuner: times `passes` full neighbor-access passes (after one warm
+ * pass) for each candidate burst length of the decode walk -- the contacts
+ * decoded per refill of a lane's ring of neighbor runs -- and keeps the
+ * fastest for every later call on the handle (results never depend on it).
+ * The report lists each candidate and its average pass time, the burst kept
+ * and the flat-record checkpoint stride chosen at open (records longer than
+ * it are decoded as chunks by several lanes).  No counterpart in the
+ * reference, whose decode granularity is fixed (one thread per vertex,
+ * bfs.cu:1485-1633). */
+#define TGA_TUNE_MAX 8
+typedef struct tga_tune_report {
+  int32_t candidates;
+  int32_t burst;                  /* burst length kept */
+  int32_t flat_stride;            /* flat checkpoint stride in use */
+  int32_t setting[TGA_TUNE_MAX];  /* candidate burst lengths */
+  double pass_ms[TGA_TUNE_MAX];   /* average pass time of each */
+} tga_tune_report;
+int tga_autotune(tga_graph* g, int32_t passes, tga_tune_report* report);
+
 /* Error-path test hook: the nth device allocation the handle makes from now
  * on fails with TGA_ENOMEM, as if hipMalloc had (nth = 0 turns it off).  A
  * call that fails this way leaves no group of first-use buffers half

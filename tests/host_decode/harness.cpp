@@ -11,7 +11,7 @@
 // past a stream aborts or changes the output, instead of being masked by
 // whatever a register held.
 //
-// Usage: harness <dataset dir> <basename> <out file> [flat chunk stride]
+// Usage: harness <dataset dir> <basename> <out file> [flat chunk stride [burst]]
 // Writes, per vertex, the decoded contact count (int64) and the sum of the
 // neighbor-access checksum terms of its contacts (uint64), then the error
 // word (int64).  Exit status 0 when the walk finished; the caller compares
@@ -72,7 +72,7 @@ struct Act {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 4 && argc != 5) return die("usage: harness <dir> <basename> <out> [flat stride]");
+  if (argc < 4 || argc > 6) return die("usage: harness <dir> <basename> <out> [flat stride [burst]]");
   const std::string dir = argv[1], name = argv[2], base = dir + "/" + name;
   std::vector<uint8_t> graph, ts, glo, gup, tlo, tup;
   const std::pair<std::string, std::vector<uint8_t>*> files[] = {
@@ -145,7 +145,8 @@ int main(int argc, char** argv) {
   if (err) return die("decode error in the headers", err);
   int64_t contacts = 0;
   for (int64_t x = 0; x < n; ++x) contacts += deg[(size_t)x];
-  S.flat_stride = argc == 5 ? std::atoi(argv[4]) : flat_stride_for(contacts);
+  S.flat_stride = argc >= 5 ? std::atoi(argv[4]) : flat_stride_for(contacts);
+  S.burst = argc >= 6 ? std::atoi(argv[5]) : kBurstDefault;
 
   // Checkpoints of long records, class by class (a record's references have
   // smaller depth, so their checkpoints exist when it is built).

@@ -389,3 +389,20 @@ def test_served_mix(pair, worker):
             np.testing.assert_array_equal(keep[i], o.cc(window=r.window), err_msg=msg)
         else:
             np.testing.assert_array_equal(keep[i], o.bfs(r.src, window=r.window), err_msg=msg)
+
+
+def test_autotune_keeps_results(pair):
+    """The autotuner only changes how the walk is scheduled: after it picks a
+    burst length, every result still equals the oracle's."""
+    ds, g = pair
+    rep = g.autotune(passes=1)
+    assert rep["burst"] in rep["pass_ms"] and len(rep["pass_ms"]) >= 3
+    assert rep["pass_ms"][rep["burst"]] == min(rep["pass_ms"].values())
+    cs, cnt = g.neighbor_access()
+    assert (cs, cnt) == (B.checksum(ds.src, ds.dst, ds.t), len(ds.src))
+    o = ds.oracle()
+    s = int(ds.src[len(ds.src) // 2])
+    np.testing.assert_array_equal(g.bfs(s), o.bfs(s))
+    np.testing.assert_array_equal(g.connected_components(), o.cc())
+    ref, _ = o.pagerank(20, dtype=np.float32)
+    assert rel_linf(g.pagerank(iters=20), ref) <= REL_TOL

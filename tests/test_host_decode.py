@@ -45,10 +45,10 @@ def harness(request, tmp_path_factory):
     return out
 
 
-def _run(harness, ds, tmp_path, stride=None):
+def _run(harness, ds, tmp_path, stride=None, burst=None):
     out = str(tmp_path / f"{ds.name}.bin")
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    cmd = [harness, ds.dir, ds.name, out] + ([str(stride)] if stride else [])
+    cmd = [harness, ds.dir, ds.name, out] + ([str(stride)] if stride else []) + ([str(burst)] if burst else [])
     p = subprocess.run(cmd, capture_output=True, text=True, env=env)
     assert p.returncode == 0, p.stderr[-4000:]
     raw = np.fromfile(out, np.uint64)
@@ -76,13 +76,14 @@ def test_host_decode_matches_raw(harness, which, request, tmp_path):
     np.testing.assert_array_equal(sums, want_sums)
 
 
-@pytest.mark.parametrize("stride", [16, 1024])
+@pytest.mark.parametrize("stride,burst", [(16, 4), (16, 32), (1024, 16), (128, 1)])
 @pytest.mark.parametrize("which", ["ds_adv", "ds_nf"])
-def test_host_decode_flat_strides(harness, which, stride, request, tmp_path):
+def test_host_decode_flat_strides(harness, which, stride, burst, request, tmp_path):
     """The same with the flat-record checkpoint stride forced to 16 (most
-    records chunked, many split runs) and to 1024 (the large-graph stride)."""
+    records chunked, many split runs) and to 1024 (the large-graph stride),
+    and the walk's burst length from 1 to 32 (the autotuner's range)."""
     ds = request.getfixturevalue(which)
-    cnt, sums, err = _run(harness, ds, tmp_path, stride)
+    cnt, sums, err = _run(harness, ds, tmp_path, stride, burst)
     assert err == 0
     want_cnt, want_sums = _raw_sums(ds)
     np.testing.assert_array_equal(cnt, want_cnt)
