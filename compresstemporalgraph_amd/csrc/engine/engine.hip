@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -2546,13 +2547,18 @@ struct tga_graph {
   int expand(const int32_t* front, int32_t base, uint32_t count, Sel sel, bool keep_empty, bool want_big,
              uint32_t hc[kGroups + 1]) {
     HIP_TRY(hipMemsetAsync(cnt, 0, (kGroups + 1) * sizeof(uint32_t), stream));
+    return expand_into(front, base, count, sel, keep_empty, want_big ? xq_big : nullptr, xq, cnt, hc);
+  }
+  // The same into caller-chosen queues and counters (zeroed by the caller).
+  int expand_into(const int32_t* front, int32_t base, uint32_t count, Sel sel, bool keep_empty, int32_t* bigq,
+                  Queues q, uint32_t* counters, uint32_t hc[kGroups + 1]) {
     if (count > 0) {
       const int blocks = (int)std::min<int64_t>(kExpandBlocks, (count + kBlock - 1) / kBlock);
-      k_expand<<<blocks, kBlock, 0, stream>>>(front, base, count, sel, vinfo, ck, xq, cnt,
-                                              want_big ? xq_big : nullptr, keep_empty ? 1 : 0);
+      k_expand<<<blocks, kBlock, 0, stream>>>(front, base, count, sel, vinfo, ck, q, counters, bigq,
+                                              keep_empty ? 1 : 0);
       HIP_TRY(hipGetLastError());
     }
-    return read_counts(cnt, hc, kGroups + 1);
+    return read_counts(counters, hc, kGroups + 1);
   }
   int create_streams() {
     HIP_TRY(hipStreamCreate(&stream));
@@ -2594,6 +2600,9 @@ struct AllocTxn {
 };
 
 extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
+
+struct NoAllocScratch;
+static int scratch_for(int device, NoAllocScratch** out);
 
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
 static int build_schedule(tga_graph* g) {
@@ -2850,6 +2859,11 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipSetDevice failed"));
   g->device = device;
+  {
+    // the allocation-free entry point's device array, found once per device
+    NoAllocScratch* sc = nullptr;
+    if ((rc = scratch_for(device, &sc))) return bail(rc);
+  }
   if ((rc = g->create_streams())) return bail(rc);
   g->n = n;
   g->ab = (int32_t)std::max<int64_t>(0, desc ? desc->active_begin : 0);
@@ -2971,47 +2985,81 @@ static bool bad_window(tga_window w) { return w.start > w.end; }
 #define STATIC_ITEMS(c) (Items{g->items[c], g->n_items[c]})
 #define QUEUED_ITEMS(c) (Items{g->xq.q[c], hc_[c]})
 #define LAUNCH_STATIC(K, ...) LAUNCH_CLASSES(K, STATIC_ITEMS, __VA_ARGS__)
+#define LISTED_ITEMS(c) (Items{qs_.q[c], hc_[c]})
+#define LAUNCH_LISTED(K, QS, HC, ...)                \
+  do {                                               \
+    const Queues qs_ = (QS);                         \
+    const uint32_t* hc_ = (HC);                      \
+    LAUNCH_CLASSES(K, LISTED_ITEMS, __VA_ARGS__);    \
+  } while (0)
 #define LAUNCH_QUEUED(K, HC, ...)         \
   do {                                    \
     const uint32_t* hc_ = (HC);           \
     LAUNCH_CLASSES(K, QUEUED_ITEMS, __VA_ARGS__); \
   } while (0)
 
-extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
-                                   int64_t* contacts) {
-  if (!g || (nverts > 0 && !verts) || nverts < 0) return fail(TGA_EINVAL, "tga_neighbor_access: bad arguments");
-  HIP_TRY(hipSetDevice(g->device));
-  HIP_TRY(hipMemsetAsync(g->u64s, 0, 2 * kSumSlots * sizeof(unsigned long long), g->stream));
+// Buffers of one neighbor-access call: checksum slots, queue counters, the
+// copied vertex list and the group item queues, with their capacities.
+struct RaBufs {
+  unsigned long long* sums;  // 2 * kSumSlots
+  uint32_t* counters;        // kGroups + 1
+  int32_t* front;            // cap_verts
+  Queues q;                  // kGroups queues of cap_items
+  int64_t cap_verts, cap_items;
+  bool batch;                // split lists that do not fit into several launches (else reject)
+};
+
+static int ra_run(tga_graph* g, const int32_t* verts, int64_t nverts, const RaBufs& B, uint64_t* checksum,
+                  int64_t* contacts, const char* who) {
+  k_fill<unsigned long long><<<1, kBlock, 0, g->stream>>>(B.sums, 2 * kSumSlots, 0ull);
+  HIP_TRY(hipGetLastError());
+  auto zero_counters = [&]() -> int {
+    k_fill<uint32_t><<<1, kBlock, 0, g->stream>>>(B.counters, kGroups + 1, 0u);
+    HIP_TRY(hipGetLastError());
+    return TGA_OK;
+  };
   if (nverts > 0) {
     // Listed vertices: expanded into flat/chain items like a frontier, in
     // batches whose expansion fits the item queues (a list may repeat a long
     // record many times).
     for (int64_t i = 0; i < nverts; ++i)
-      if (verts[i] < 0 || verts[i] >= g->n) return fail(TGA_EINVAL, "tga_neighbor_access: vertex out of range");
-    const int64_t cap_items = g->n + g->nck, cap_verts = g->n;
+      if (verts[i] < 0 || verts[i] >= g->n) return fail(TGA_EINVAL, std::string(who) + ": vertex out of range");
     int64_t i0 = 0;
     while (i0 < nverts) {
       int64_t i1 = i0, items = 0;
-      while (i1 < nverts && i1 - i0 < cap_verts && items + g->h_items[(size_t)verts[i1]] <= cap_items)
+      while (i1 < nverts && i1 - i0 < B.cap_verts && items + g->h_items[(size_t)verts[i1]] <= B.cap_items)
         items += g->h_items[(size_t)verts[i1++]];
-      HIP_TRY(hipMemcpyAsync(g->q0, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
+      if (i1 < nverts && !B.batch)
+        return fail(TGA_ERANGE, std::string(who) + ": the list exceeds " + std::to_string(B.cap_verts) +
+                                    " vertices or " + std::to_string(B.cap_items) + " work items");
+      if (i1 == i0) return fail(TGA_ERANGE, std::string(who) + ": a record has more chunks than the item queues");
+      HIP_TRY(hipMemcpyAsync(B.front, verts + i0, (size_t)(i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice,
                              g->stream));
       uint32_t hc[kGroups + 1];
-      int rc = g->expand(g->q0, 0, (uint32_t)(i1 - i0), Sel{kSelAll, nullptr, 0}, false, false, hc);
+      int rc = zero_counters();
+      if (!rc) rc = g->expand_into(B.front, 0, (uint32_t)(i1 - i0), Sel{kSelAll, nullptr, 0}, false, nullptr, B.q,
+                                   B.counters, hc);
       if (rc) return rc;
-      LAUNCH_QUEUED(k_ra, hc, g->u64s);
+      LAUNCH_LISTED(k_ra, B.q, hc, B.sums);
       i0 = i1;
     }
   } else if (g->ab == 0 && g->ae == g->n) {
-    LAUNCH_STATIC(k_ra, g->u64s);
+    LAUNCH_STATIC(k_ra, B.sums);
   } else {
+    int64_t items = 0;
+    for (int64_t x = g->ab; x < g->ae; ++x) items += g->h_items[(size_t)x];
+    if (items > B.cap_items)
+      return fail(TGA_ERANGE, std::string(who) + ": the active range expands to more than " +
+                                  std::to_string(B.cap_items) + " work items");
     uint32_t hc[kGroups + 1];
-    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelAll, nullptr, 0}, false, false, hc);
+    int rc = zero_counters();
+    if (!rc) rc = g->expand_into(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelAll, nullptr, 0}, false,
+                                 nullptr, B.q, B.counters, hc);
     if (rc) return rc;
-    LAUNCH_QUEUED(k_ra, hc, g->u64s);
+    LAUNCH_LISTED(k_ra, B.q, hc, B.sums);
   }
   unsigned long long hv[2 * kSumSlots];
-  HIP_TRY(hipMemcpyAsync(hv, g->u64s, sizeof hv, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(hv, B.sums, sizeof hv, hipMemcpyDeviceToHost, g->stream));
   int rc = g->check_err();
   if (rc) return rc;
   unsigned long long cs = 0, cnt = 0;
@@ -3022,6 +3070,65 @@ extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t n
   if (checksum) *checksum = cs;
   if (contacts) *contacts = (int64_t)cnt;
   return TGA_OK;
+}
+
+extern "C" int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
+                                   int64_t* contacts) {
+  if (!g || (nverts > 0 && !verts) || nverts < 0) return fail(TGA_EINVAL, "tga_neighbor_access: bad arguments");
+  HIP_TRY(hipSetDevice(g->device));
+  const RaBufs B{g->u64s, g->cnt, g->q0, g->xq, g->n, g->n + g->nck, true};
+  return ra_run(g, verts, nverts, B, checksum, contacts, "tga_neighbor_access");
+}
+
+// Allocation-free neighbor access: the same pass with every buffer beyond the
+// caller's inputs and outputs in one file-scope device array, so it calls no
+// hipMalloc (for callers that cannot allocate: graph capture, tight serving
+// loops).  Each GPU holds its own copy of the array, which host code reaches
+// through the per-device address from hipGetSymbolAddress (the array's name
+// is not a device address in host code).  Calls that overlap on one GPU share
+// the array and race.
+constexpr int64_t kNoAllocVerts = 1 << 16;  // listed vertices per call
+constexpr int64_t kNoAllocItems = 1 << 18;  // work items (records and chunks) per group queue
+struct NoAllocScratch {
+  unsigned long long sums[2 * kSumSlots];
+  uint32_t counters[kGroups + 1];
+  int32_t front[kNoAllocVerts];
+  int32_t queue[kGroups][kNoAllocItems];
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<NoAllocScratch*> g_scratch_ptr;  // per device, found once
+}  // namespace
+
+static int scratch_for(int device, NoAllocScratch** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the current device's copy
+    g_scratch_ptr[(size_t)device] = (NoAllocScratch*)p;
+  }
+  *out = g_scratch_ptr[(size_t)device];
+  return TGA_OK;
+}
+
+extern "C" int tga_neighbor_access_noalloc(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
+                                           int64_t* contacts) {
+  if (!g || (nverts > 0 && !verts) || nverts < 0)
+    return fail(TGA_EINVAL, "tga_neighbor_access_noalloc: bad arguments");
+  if (nverts > kNoAllocVerts)
+    return fail(TGA_ERANGE, "tga_neighbor_access_noalloc: at most " + std::to_string(kNoAllocVerts) +
+                                " listed vertices per call");
+  HIP_TRY(hipSetDevice(g->device));
+  NoAllocScratch* sc = nullptr;
+  int rc = scratch_for(g->device, &sc);
+  if (rc) return rc;
+  Queues q;
+  for (int c = 0; c < kGroups; ++c) q.q[c] = sc->queue[c];
+  const RaBufs B{sc->sums, sc->counters, sc->front, q, kNoAllocVerts, kNoAllocItems, false};
+  return ra_run(g, verts, nverts, B, checksum, contacts, "tga_neighbor_access_noalloc");
 }
 
 // Autotuner: the walk's burst length (contacts decoded per refill of the

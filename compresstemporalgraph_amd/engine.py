@@ -95,6 +95,7 @@ def lib():
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
         L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
         L.tga_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
+        L.tga_neighbor_access_noalloc.argtypes = [vp, p32, C.c_int64, C.POINTER(C.c_uint64), p64]
         _LIB = L
     return _LIB
 
@@ -212,6 +213,18 @@ class TemporalGraph:
             v = np.ascontiguousarray(verts, np.int32)
             _check(lib().tga_neighbor_access(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), len(v), C.byref(cs),
                                              C.byref(cnt)))
+        return cs.value, cnt.value
+
+    def neighbor_access_noalloc(self, verts=None):
+        """neighbor_access through the entry point that allocates nothing (a
+        fixed per-GPU device array; at most 65536 listed vertices)."""
+        cs, cnt = C.c_uint64(), C.c_int64()
+        if verts is None:
+            _check(lib().tga_neighbor_access_noalloc(self.h, None, 0, C.byref(cs), C.byref(cnt)))
+        else:
+            v = np.ascontiguousarray(verts, np.int32)
+            _check(lib().tga_neighbor_access_noalloc(self.h, v.ctypes.data_as(C.POINTER(C.c_int32)), len(v),
+                                                     C.byref(cs), C.byref(cnt)))
         return cs.value, cnt.value
 
     def degree_centrality(self, window=None, centrality=False, out=None, out_centrality=None):

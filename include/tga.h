@@ -109,6 +109,18 @@ int tga_info(const tga_graph* g, tga_graph_info* info);
  * discards. */
 int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum, int64_t* contacts);
 
+/* Neighbor access that allocates nothing: the same results as
+ * tga_neighbor_access, with every buffer beyond the caller's inputs and
+ * outputs in one fixed device array (3.4 MB per GPU, held until the
+ * process ends) instead of device memory the handle allocates.  No
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  Limits: at most 65536
+ * listed vertices, expanding to at most 262144 work items per launch group
+ * (an active sub-range counts as its vertices); beyond them it returns
+ * TGA_ERANGE.  Calls that overlap on one GPU share the array and race. */
+int tga_neighbor_access_noalloc(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum,
+                                int64_t* contacts);
+
 /* Degree centrality: count[x] = contacts of active x with an active neighbor
  * and t in the window (multi-edges counted).  centrality (nullable) receives
  * the Java formula (int)(0.85 c + 0.15 c (mean - ws) / (we - ws) + 1) with a
