@@ -657,6 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_dcb_long(const int32_t* bigv, int32_
 constexpr int kPrbSeg = 512;
 constexpr int kPrbLongDeg = 1024;  // window degrees of longer records: one wave per (record, query)
 constexpr int kPrbRedBlocks = 512;
+constexpr int kCcbGroup = 8;  // batched CC rounds per replayed graph
 
 struct RunAct {
   Window A;             // active range
@@ -1057,7 +1058,9 @@ __global__ void __launch_bounds__(kBlock) k_ccb_init(Window A, int32_t n, int32_
 __global__ void __launch_bounds__(kBlock) k_ccb_pull(int64_t nseg, const int32_t* seg_row, const uint64_t* seg_beg,
                                                      const int32_t* seg_len, const uint64_t* seg_off, const int4* runs,
                                                      const int32_t* midx, const uint64_t* mask, const int64_t* wins,
-                                                     int32_t nq, int32_t* label, int32_t* partial, int32_t* changed) {
+                                                     int32_t nq, int32_t* label, int32_t* partial, int32_t* changed,
+                                                     const int32_t* run_if) {
+  if (run_if && *run_if == 0) return;  // an earlier round of the group lowered nothing: converged
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
   const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
@@ -1109,7 +1112,8 @@ __global__ void __launch_bounds__(kBlock) k_ccb_pull(int64_t nseg, const int32_t
 
 __global__ void __launch_bounds__(kBlock) k_ccb_fix(const int32_t* mrows, const uint32_t* nm, const uint64_t* seg_off,
                                                     const int32_t* partial, int32_t nq, int32_t* label,
-                                                    int32_t* changed) {
+                                                    int32_t* changed, const int32_t* run_if) {
+  if (run_if && *run_if == 0) return;
   const int lane = threadIdx.x & 63;
   const uint32_t cnt = *nm;
   const uint32_t nw = gridDim.x * (kBlock / 64);
@@ -2407,6 +2411,13 @@ struct tga_graph {
   int32_t* bq_levels = nullptr;  // nq rows of n when the caller's output is on the host
   int32_t bq_levels_rows = 0;
   int64_t device_bytes = 0;
+  // Captured iteration loops of the batched calls (replay).
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
+  };
+  Graph gr_prb, gr_hb, gr_ccb;
+  bool use_graphs = true;  // TGA_GRAPHS=0: launch the loops one kernel at a time
   std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
   bool prof = false;
@@ -2560,6 +2571,36 @@ struct tga_graph {
     }
     return read_counts(counters, hc, kGroups + 1);
   }
+  // Runs `enqueue` (kernel launches on `stream`) from a captured graph: the
+  // first call for a key captures and instantiates it, later calls with the
+  // same key replay it with one launch.  The captured work launches kernels
+  // only (fills are kernels too, not memsets).
+  template <class F>
+  int replay(Graph& gr, uint64_t key, F&& enqueue) {
+    if (!use_graphs) return enqueue();
+    if (!gr.exec || gr.key != key) {
+      if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+      gr.exec = nullptr;
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue();
+      const hipError_t e = hipStreamEndCapture(stream, &graph);
+      if (rc) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIP_TRY(e);
+      const hipError_t ei = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        gr.exec = nullptr;
+        return fail(TGA_EDEVICE, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
+      }
+      gr.key = key;
+    }
+    HIP_TRY(hipGraphLaunch(gr.exec, stream));
+    return TGA_OK;
+  }
   int create_streams() {
     HIP_TRY(hipStreamCreate(&stream));
     HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -2571,6 +2612,8 @@ struct tga_graph {
   }
   ~tga_graph() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (Graph* gr : {&gr_prb, &gr_hb, &gr_ccb})
+      if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     for (int c = 0; c < kGroups; ++c) {
@@ -2859,6 +2902,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipSetDevice failed"));
   g->device = device;
+  if (const char* e = std::getenv("TGA_GRAPHS")) g->use_graphs = std::atoi(e) != 0;
   {
     // the allocation-free entry point's device array, found once per device
     NoAllocScratch* sc = nullptr;
@@ -3516,6 +3560,9 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
     k_prb_wmulti<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
                                                                          g->coff, g->ct, g->prb_win, nq, g->prb_wm);
   k_prb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, inv, g->prb_rank);
+  HIP_TRY(hipGetLastError());
+  // The iterations replay one captured graph (cached per batch shape).
+  auto iterations = [&]() -> int {
   for (int32_t it = 0; it < iters; ++it) {
     if (it == 0)
       k_prb_coef<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->prb_rank, g->prb_du, damping, g->prb_coef,
@@ -3533,6 +3580,13 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
     k_prb_colsum<<<kPrbRedBlocks, kBlock, 0, st>>>((int32_t)n, g->prb_acc, g->prb_part);
     k_prb_colsum2<<<1, 64, 0, st>>>(g->prb_part, kPrbRedBlocks, g->prb_sums);
   }
+  HIP_TRY(hipGetLastError());
+  return TGA_OK;
+  };
+  uint32_t dbits;
+  std::memcpy(&dbits, &damping, sizeof dbits);
+  if ((rc = g->replay(g->gr_prb, (uint64_t)nq | (uint64_t)(uint32_t)iters << 8 | (uint64_t)dbits << 32, iterations)))
+    return rc;
   k_prb_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->prb_acc, iters > 0 ? g->prb_sums : nullptr,
                                                 g->prb_rank, out);
   g->kend();
@@ -3564,7 +3618,7 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
     if ((rc = g->alloc(g->ccb_label, (size_t)n * 64)) ||
         (rc = g->alloc(g->ccb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
         (rc = g->alloc(g->ccb_mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))) ||
-        (rc = g->alloc(g->ccb_changed, 1)))
+        (rc = g->alloc(g->ccb_changed, kCcbGroup + 1)))
       return rc;
     if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
     txn.commit();
@@ -3588,18 +3642,30 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
                                                                        g->coff, g->ct, g->prb_win, nq, g->ccb_mask);
   k_ccb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->ccb_label);
   HIP_TRY(hipGetLastError());
-  for (int64_t round = 0; round < n && g->prb_nseg; ++round) {
-    k_fill<int32_t><<<1, kBlock, 0, st>>>(g->ccb_changed, 1, 0);
-    k_ccb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
-        g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
-        g->ccb_mask, g->prb_win, nq, g->ccb_label, g->ccb_partial, g->ccb_changed);
-    if (g->prb_nmrows)
-      k_ccb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(g->prb_mrows, g->prb_nm, g->prb_seg_off,
-                                                                       g->ccb_partial, nq, g->ccb_label,
-                                                                       g->ccb_changed);
-    HIP_TRY(hipGetLastError());
+  // Rounds run in groups of kCcbGroup, replayed from one captured graph:
+  // flag r of a group records whether round r lowered a label, and the
+  // kernels of round r + 1 return at once when it did not (the labels are
+  // then the fixpoint), so the host reads one flag per group instead of
+  // synchronising every round.
+  auto rounds = [&]() -> int {
+    for (int r = 0; r < kCcbGroup; ++r) {
+      k_fill<int32_t><<<1, kBlock, 0, st>>>(g->ccb_changed + r + 1, 1, 0);
+      const int32_t* run_if = r ? g->ccb_changed + r : nullptr;
+      k_ccb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+          g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
+          g->ccb_mask, g->prb_win, nq, g->ccb_label, g->ccb_partial, g->ccb_changed + r + 1, run_if);
+      if (g->prb_nmrows)
+        k_ccb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(
+            g->prb_mrows, g->prb_nm, g->prb_seg_off, g->ccb_partial, nq, g->ccb_label, g->ccb_changed + r + 1,
+            run_if);
+      HIP_TRY(hipGetLastError());
+    }
+    return TGA_OK;
+  };
+  for (int64_t round = 0; round < n && g->prb_nseg; round += kCcbGroup) {
+    if ((rc = g->replay(g->gr_ccb, (uint64_t)nq, rounds))) return rc;
     int32_t changed = 0;
-    HIP_TRY(hipMemcpyAsync(&changed, g->ccb_changed, sizeof changed, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&changed, g->ccb_changed + kCcbGroup, sizeof changed, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (!changed) break;
   }
@@ -3659,6 +3725,8 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
     k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
                                                                        g->coff, g->ct, g->prb_win, nq, g->hb_mask);
   k_cols_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, 1.0f, g->hb_hub, g->hb_auth);
+  HIP_TRY(hipGetLastError());
+  auto iterations = [&]() -> int {
   for (int32_t it = 0; it < iters; ++it) {
     for (int step = 0; step < 2; ++step) {  // 0: hub from authority over out-runs; 1: authority from hub, in-runs
       k_fill<double><<<g->grid(n * 64), kBlock, 0, st>>>(g->prb_acc, n * 64, 0.0);
@@ -3686,6 +3754,9 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
       HIP_TRY(hipGetLastError());
     }
   }
+  return TGA_OK;
+  };
+  if ((rc = g->replay(g->gr_hb, (uint64_t)nq | (uint64_t)(uint32_t)iters << 8, iterations))) return rc;
   k_cols_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->hb_hub, oh);
   k_cols_out<<<g->grid(n * 64), kBlock, 0, st>>>((int32_t)n, nq, g->hb_auth, oa);
   g->kend();
