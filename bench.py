@@ -44,6 +44,9 @@ Prints one JSON line with the benchmark record's fields plus:
                 against 64 plain calls, windowed and full windows.
   batched_hits  64 HITS queries (20 iterations) in one batched call against
                 64 plain calls, windowed and full windows.
+  batched_gc    64 graph-coloring queries (512 rounds) in one batched call
+                against 64 plain calls, windowed and full windows, on the
+                at-scale graph and on rmat16.
   served_mix    Poisson arrivals of 40% BFS, 20% CC, 30% PR, 10% DC at 500/s
                 and 2000/s: continuous batching against sequential plain
                 execution.
@@ -338,6 +341,39 @@ def batched_cc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
     return out
 
 
+def batched_gc(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
+    """64 graph-coloring queries (default priorities, 512 rounds) in one
+    tga_graph_coloring_batch call against 64 plain calls, outputs in HBM,
+    windowed and full windows; every row compared with its plain result."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(17)
+    n = g.info["n"]
+    lens = np.exp(rng.uniform(np.log(2.0 ** 10), np.log(2.0 ** 24), nq)).astype(np.int64)
+    starts = rng.integers(t_lo, max(t_hi, t_lo + 1), nq)
+    out = {}
+    for name, wins in (("windowed", [(int(a), int(a + b)) for a, b in zip(starts, lens)]), ("full", [None] * nq)):
+        bat = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+        pla = torch.empty((nq, n), dtype=torch.int32, device="cuda:0")
+        g.graph_coloring_batch(wins, out=bat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.graph_coloring_batch(wins, out=bat)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        g.graph_coloring(wins[0], out=pla[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q in range(nq):
+            g.graph_coloring(wins[q], out=pla[q])
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        out[name] = {"queries": nq, "batched_ms": round(tb * 1e3, 3), "plain_ms": round(tp * 1e3, 3),
+                     "batched_qps": round(nq / tb, 1), "plain_qps": round(nq / tp, 1),
+                     "speedup": round(tp / tb, 2), "identical": bool(torch.equal(bat, pla))}
+    return out
+
+
 def batched_hits(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
     """64 HITS queries (20 iterations, FP32) in one tga_hits_batch call
     against 64 plain calls, outputs in HBM, windowed and full windows; rows
@@ -540,8 +576,12 @@ def query_legs(args) -> int:
         for h in workers.values():
             h.close()
         log(f"[bench] served mix on {args.workload}: {smix}")
+        bgc = batched_gc(g, t_lo, t_hi)
+        log(f"[bench] batched GC on {args.workload}: {bgc}")
+        if not all(v["identical"] for v in bgc.values()):
+            raise SystemExit("batched GC results differ from the plain results")
         g.close()
-        print(json.dumps({"batched_mix": bmix, "served_mix": smix}), flush=True)
+        print(json.dumps({"batched_mix": bmix, "served_mix": smix, "batched_gc": bgc}), flush=True)
         return 0
     batched = batched_bfs(g, t_lo, t_hi)
     log(f"[bench] batched BFS: {batched}")
@@ -563,6 +603,10 @@ def query_legs(args) -> int:
     log(f"[bench] batched HITS: {bhits}")
     if max(v["max_rel_diff"] for v in bhits.values()) > 1e-5:
         raise SystemExit("batched HITS differs from the plain results beyond 1e-5")
+    bgc = batched_gc(g, t_lo, t_hi)
+    log(f"[bench] batched GC: {bgc}")
+    if not all(v["identical"] for v in bgc.values()):
+        raise SystemExit("batched GC results differ from the plain results")
     served = smix = None
     if not args.no_served:
         served = served_bfs(g, t_lo, t_hi)
@@ -577,7 +621,8 @@ def query_legs(args) -> int:
         log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
     print(json.dumps({"batched_mix": bmix, "batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc,
-                      "batched_pr": bpr, "batched_hits": bhits, "served_bfs": served, "served_mix": smix}),
+                      "batched_pr": bpr, "batched_hits": bhits, "batched_gc": bgc, "served_bfs": served,
+                      "served_mix": smix}),
           flush=True)
     return 0
 
@@ -678,7 +723,7 @@ def main() -> int:
     # headline it is measured first, as extra keys of the same line.
     side = "chunglu21" if args.workload == "netflow25" and not args.no_side else None
     legs_on = args.workload if args.workload != "netflow25" else side
-    batched = bdc = bcc = bpr = bhits = served = smix = bmix = smix_c1 = None
+    batched = bdc = bcc = bpr = bhits = bgc = served = smix = bmix = smix_c1 = None
     if not args.no_workloads and not args.no_query_legs and legs_on:
         datasets.ensure(root, legs_on)
         # The query legs keep results in torch device tensors.  torch brings
@@ -714,6 +759,7 @@ def main() -> int:
         bmix["rmat16"], smix_c1 = c1["batched_mix"], c1["served_mix"]
         batched, bdc, bcc, bpr = legs["batched_bfs"], legs["batched_dc"], legs["batched_cc"], legs["batched_pr"]
         bhits = legs["batched_hits"]
+        bgc = {legs_on: legs["batched_gc"], "rmat16": c1["batched_gc"]}
         served, smix = legs["served_bfs"], legs["served_mix"]
 
     side_rec = None
@@ -795,6 +841,7 @@ def main() -> int:
         "batched_cc": bcc,
         "batched_pr": bpr,
         "batched_hits": bhits,
+        "batched_gc": bgc,
         "served_mix": smix,
         "served_mix_rmat16": smix_c1,
         "served_bfs": served,

@@ -1260,6 +1260,119 @@ __global__ void __launch_bounds__(kBlock) k_hitsb_pull(int64_t nseg, const int32
   }
 }
 
+constexpr int kGcbPieceBits = 3;
+constexpr int32_t kGcbPiece = 64;  // runs per piece
+static_assert((kGcbPiece << kGcbPieceBits) >= kPrbSeg, "a segment must fit its pieces");
+// Batched graph coloring.  The round rule (an uncolored active vertex takes
+// color c in round c unless an outranking active window out-neighbor is
+// still uncolored) gives every vertex one more than the largest color of its
+// outranking window out-neighbors, and 0 when it has none: the longest path
+// in the outranking order, a DAG.  So the colors of 64 windows come from one
+// sweep over priority levels, highest first: a level's vertices only need the
+// final colors of higher levels, and passes within a level settle the
+// (rare) outranking neighbors inside it.  One wave per out-segment of the
+// level's vertices, lane q for window q: the largest color + 1 over the
+// segment's runs linked in q whose destination outranks the row, merged into
+// the row's column with atomicMax.  `changed` records a raised color; a pass
+// whose `run_if` flag is 0 returns at once (the level had settled).
+__global__ void __launch_bounds__(kBlock) k_gcb_level(const int32_t* segs, int32_t nsegs, const int32_t* seg_row,
+                                                      const uint64_t* seg_beg, const int32_t* seg_len,
+                                                      const int4* rr, const int32_t* midx, const uint64_t* mask,
+                                                      const int64_t* wins, int32_t nq, const int32_t* prio,
+                                                      int32_t* col, int32_t* changed, const int32_t* run_if) {
+  if (run_if && *run_if == 0) return;
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < nq;
+  const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
+  const int32_t nw = gridDim.x * (kBlock / 64);
+  for (int32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < nsegs; i += nw) {
+    // a piece of kGcbPiece runs of out-segment s (short pieces keep the
+    // dependent gathers of one wave few, and levels run one after another)
+    const int32_t s = segs[i] >> kGcbPieceBits, piece = segs[i] & ((1 << kGcbPieceBits) - 1);
+    const int32_t row = seg_row[s];
+    const int32_t pv = prio[row];
+    const uint64_t sb = seg_beg[s] + (uint64_t)piece * kGcbPiece;
+    const int32_t len = min(kGcbPiece, seg_len[s] - piece * kGcbPiece);
+    int32_t cur = 0;
+    for (int32_t b0 = 0; b0 < len; b0 += 64) {
+      const int32_t j = b0 + lane;
+      const int4 my = j < len ? rr[sb + j] : int4{-1, -1, 0, 0};
+      // the run's destination outranks the row (priority, then the smaller
+      // id); other runs are dropped from the wave's list
+      const int32_t y = (my.x >= 0 && (prio[my.x] > pv || (prio[my.x] == pv && my.x < row))) ? my.x : -1;
+      const uint64_t upm = __ballot(y >= 0);
+      const int32_t mym = (y >= 0 && my.y >= 0) ? midx[my.y] : -1;
+      const int32_t nup = __popcll(upm);
+      // the outranking runs, compacted in lane order: the k-th is lane kth[k]
+      const int32_t rank = __popcll(upm & ((1ull << lane) - 1));
+      __shared__ int32_t kth_lds[kBlock];
+      int32_t* kth = kth_lds + (threadIdx.x & ~63);
+      if (y >= 0) kth[rank] = lane;
+      __builtin_amdgcn_wave_barrier();
+      for (int32_t u0 = 0; u0 < nup; u0 += kPrbAhead) {
+        int32_t c[kPrbAhead];
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          const int l = __builtin_amdgcn_readfirstlane(u0 + u < nup ? kth[u0 + u] : 0);  // same in every lane
+          const int32_t yy = rl32(y, l);
+          c[u] = (u0 + u < nup && on) ? col[(int64_t)yy * 64 + lane] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kPrbAhead; ++u) {
+          if (u0 + u >= nup) break;
+          const int l = __builtin_amdgcn_readfirstlane(kth[u0 + u]);
+          const int32_t mi = rl32(mym, l);
+          bool hit;
+          if (mi >= 0) {
+            hit = mask[mi] >> lane & 1;
+          } else {
+            const int32_t tt = rl32(my.z, l);
+            hit = tt >= ws && tt <= we;
+          }
+          if (hit) cur = max(cur, c[u] + 1);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    bool raised = false;
+    if (on && cur > 0) raised = atomicMax(&col[(int64_t)row * 64 + lane], cur) < cur;
+    if (__ballot(raised) && lane == 0) *changed = 1;
+  }
+}
+
+// Out-segments whose row has an outranking run inside its own level (the
+// only ones a second pass of the level can change): flag[s] = 1.
+__global__ void __launch_bounds__(kBlock) k_gcb_inner(int64_t nseg, const int32_t* seg_row, const uint64_t* seg_beg,
+                                                      const int32_t* seg_len, const int4* rr, const int32_t* prio,
+                                                      const int32_t* vlevel, int32_t* flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t s = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); s < nseg; s += nw) {
+    const int32_t row = seg_row[s];
+    const int32_t pv = prio[row], lv = vlevel[row];
+    const uint64_t sb = seg_beg[s];
+    const int32_t len = seg_len[s];
+    bool in = false;
+    for (int32_t j = lane; j < len; j += 64) {
+      const int32_t y = rr[sb + j].x;
+      if (y >= 0 && vlevel[y] == lv && (prio[y] > pv || (prio[y] == pv && y < row))) in = true;
+    }
+    const bool any = __ballot(in) != 0;  // every lane of the wave votes
+    if (lane == 0) flag[s] = any;
+  }
+}
+
+// Colors past the round limit stay uncolored (kInf), as do inactive vertices.
+__global__ void __launch_bounds__(kBlock) k_gcb_out(Window A, int32_t n, int32_t nq, int32_t rounds,
+                                                    const int32_t* col, int32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * nq;
+       i += (int64_t)gridDim.x * kBlock) {
+    const int32_t q = (int32_t)(i / n), v = (int32_t)(i % n);
+    const int32_t c = col[(int64_t)v * 64 + q];
+    out[i] = (A.active(v) && c < rounds) ? c : kInf;
+  }
+}
+
 // Column sums of squares (k_prb_colsum's fixed order) and their square roots.
 __global__ void __launch_bounds__(kBlock) k_colsq(int32_t n, const double* acc, double* part) {
   __shared__ double sh[kBlock / 64][64];
@@ -2416,7 +2529,13 @@ struct tga_graph {
     hipGraphExec_t exec = nullptr;
     uint64_t key = 0;
   };
-  Graph gr_prb, gr_hb, gr_ccb;
+  Graph gr_prb, gr_hb, gr_ccb, gr_gcb;
+  // batched coloring: out-segments in priority-level order, per-level flags
+  int32_t* gcb_segs = nullptr;
+  int32_t* gcb_flags = nullptr;
+  std::vector<int32_t> h_oseg_row;         // row of every out-segment (host copy)
+  std::vector<int64_t> gcb_level_off;      // level boundaries in gcb_segs
+  bool gcb_default_levels = false;         // gcb_segs holds the default priorities' levels
   bool use_graphs = true;  // TGA_GRAPHS=0: launch the loops one kernel at a time
   std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
@@ -2612,7 +2731,7 @@ struct tga_graph {
   }
   ~tga_graph() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (Graph* gr : {&gr_prb, &gr_hb, &gr_ccb})
+    for (Graph* gr : {&gr_prb, &gr_hb, &gr_ccb, &gr_gcb})
       if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
@@ -4244,6 +4363,241 @@ extern "C" int tga_graph_coloring(tga_graph* g, tga_window w, int32_t rounds, co
   }
   int rc = g->out_copy(color, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;
+  return g->check_err();
+}
+
+// Batched graph coloring (see k_gcb_level): up to 64 windows, one sweep over
+// priority levels.  Levels are the distinct priority values, highest first
+// (at most kGcbLevels; with more distinct values, runs of consecutive values
+// share a level and the passes inside it settle their order too).
+constexpr int kGcbLevels = 512;
+constexpr int kGcbPasses = 4;  // passes per level in the replayed sweep (later ones over inner segments)
+
+// Levels of a priority array: gcb_segs holds, level by level, all
+// out-segments of the level's rows, then its inner segments (k_gcb_inner);
+// gcb_level_off[2l] .. [2l + 1] .. [2l + 2] bound them.
+static int gcb_levels(tga_graph* g, const std::vector<int32_t>& prio, const int32_t* dprio) {
+  const int64_t nos = g->prb_noseg;
+  const int64_t n = g->n;
+  if (g->h_oseg_row.size() != (size_t)nos) {
+    g->h_oseg_row.resize((size_t)nos);
+    if (nos)
+      HIP_TRY(hipMemcpy(g->h_oseg_row.data(), g->prb_oseg_row, (size_t)nos * 4, hipMemcpyDeviceToHost));
+  }
+  // level of every priority value: distinct values, descending, grouped so
+  // that there are at most kGcbLevels levels
+  std::vector<int32_t> vals(prio);
+  std::sort(vals.begin(), vals.end(), std::greater<int32_t>());
+  vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+  const size_t per = std::max<size_t>(1, (vals.size() + kGcbLevels - 1) / kGcbLevels);
+  auto level_of = [&](int32_t p) {
+    return (int32_t)((size_t)(std::lower_bound(vals.begin(), vals.end(), p, std::greater<int32_t>()) - vals.begin()) /
+                     per);
+  };
+  const int64_t nl = vals.empty() ? 0 : level_of(vals.back()) + 1;
+  std::vector<int32_t> vlevel((size_t)n);
+  for (int64_t v = 0; v < n; ++v) vlevel[(size_t)v] = level_of(prio[(size_t)v]);
+  // segment lengths give each segment's pieces (list entries s << 3 | piece)
+  std::vector<int32_t> slen((size_t)nos);
+  if (nos) HIP_TRY(hipMemcpy(slen.data(), g->prb_oseg_len, (size_t)nos * 4, hipMemcpyDeviceToHost));
+  auto pieces = [&](int64_t s) { return (slen[(size_t)s] + kGcbPiece - 1) / kGcbPiece; };
+  int64_t total = 0;
+  for (int64_t s = 0; s < nos; ++s) total += pieces(s);
+  if (!g->gcb_segs) {  // every piece at most twice (all, inner); the inner flags use nos of them first
+    AllocTxn txn(g);
+    int rc;
+    if ((rc = g->alloc(g->gcb_segs, (size_t)std::max<int64_t>(2 * std::max(total, nos), 1))) ||
+        (rc = g->alloc(g->gcb_flags, (size_t)kGcbLevels * (kGcbPasses + 1))))
+      return rc;
+    txn.commit();
+  }
+  // inner flags into the second half of gcb_segs, levels into i2
+  HIP_TRY(hipMemcpyAsync(g->i2, vlevel.data(), (size_t)n * 4, hipMemcpyHostToDevice, g->stream));
+  std::vector<int32_t> inner((size_t)nos, 0);
+  if (nos) {
+    k_gcb_inner<<<g->grid(nos * 64), kBlock, 0, g->stream>>>(nos, g->prb_oseg_row, g->prb_oseg_beg, g->prb_oseg_len,
+                                                           g->prb_out_runs, dprio, g->i2, g->gcb_segs + std::max(total, nos));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(inner.data(), g->gcb_segs + std::max(total, nos), (size_t)nos * 4, hipMemcpyDeviceToHost,
+                           g->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  std::vector<int64_t> cnt_all((size_t)nl, 0), cnt_in((size_t)nl, 0);
+  for (int64_t s = 0; s < nos; ++s) {
+    const int32_t l = vlevel[(size_t)g->h_oseg_row[(size_t)s]];
+    cnt_all[(size_t)l] += pieces(s);
+    cnt_in[(size_t)l] += inner[(size_t)s] != 0 ? pieces(s) : 0;
+  }
+  std::vector<int64_t> off(2 * (size_t)nl + 1, 0);
+  for (int64_t l = 0; l < nl; ++l) {
+    off[2 * (size_t)l + 1] = off[2 * (size_t)l] + cnt_all[(size_t)l];
+    off[2 * (size_t)l + 2] = off[2 * (size_t)l + 1] + cnt_in[(size_t)l];
+  }
+  std::vector<int32_t> segs((size_t)off.back());
+  std::vector<int64_t> at_all((size_t)nl), at_in((size_t)nl);
+  for (int64_t l = 0; l < nl; ++l) {
+    at_all[(size_t)l] = off[2 * (size_t)l];
+    at_in[(size_t)l] = off[2 * (size_t)l + 1];
+  }
+  for (int64_t s = 0; s < nos; ++s) {
+    const int32_t l = vlevel[(size_t)g->h_oseg_row[(size_t)s]];
+    for (int32_t p = 0; p < pieces(s); ++p) {
+      const int32_t e = (int32_t)(s << kGcbPieceBits) | p;
+      segs[(size_t)at_all[(size_t)l]++] = e;
+      if (inner[(size_t)s]) segs[(size_t)at_in[(size_t)l]++] = e;
+    }
+  }
+  if (!segs.empty())
+    HIP_TRY(hipMemcpyAsync(g->gcb_segs, segs.data(), segs.size() * 4, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));  // before the host vectors go
+  g->gcb_level_off = std::move(off);
+  return TGA_OK;
+}
+
+extern "C" int tga_graph_coloring_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t rounds,
+                                        const int32_t* priority, int32_t* color, uint32_t flags) {
+  if (!g || !windows || !color || nq < 1 || nq > 64 || rounds < 0)
+    return fail(TGA_EINVAL, "tga_graph_coloring_batch: bad arguments (1 <= nq <= 64)");
+  for (int32_t q = 0; q < nq; ++q)
+    if (bad_window(windows[q])) return fail(TGA_EINVAL, "tga_graph_coloring_batch: window start > end");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t n = g->n;
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) {
+    // No room for the run structure: one call per window.
+    for (int32_t q = 0; q < nq; ++q)
+      if ((rc = tga_graph_coloring(g, windows[q], rounds, priority, color + q * n, flags))) return rc;
+    return TGA_OK;
+  }
+  if (!g->ccb_label) {  // the [vertex][64] int32 columns and window masks of batched CC
+    AllocTxn txn(g);
+    if ((rc = g->alloc(g->ccb_label, (size_t)n * 64)) ||
+        (rc = g->alloc(g->ccb_partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
+        (rc = g->alloc(g->ccb_mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))) ||
+        (rc = g->alloc(g->ccb_changed, kCcbGroup + 1)))
+      return rc;
+    if (!g->prb_win && (rc = g->alloc(g->prb_win, 2 * 64))) return rc;
+    txn.commit();
+  }
+  hipStream_t st = g->stream;
+  // priorities: the caller's, or java.util.Random(123).nextInt(256) per vertex
+  std::vector<int32_t> hp((size_t)n);
+  const int32_t* dprio = nullptr;
+  if (priority) {
+    if (flags & TGA_OUT_DEVICE) {
+      HIP_TRY(hipMemcpyAsync(hp.data(), priority, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    } else {
+      std::memcpy(hp.data(), priority, (size_t)n * 4);
+    }
+    HIP_TRY(hipMemcpyAsync(g->i1, hp.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    dprio = g->i1;
+    g->gcb_default_levels = false;
+    if ((rc = gcb_levels(g, hp, dprio))) return rc;
+  } else {
+    if (!g->prio) {
+      java_random_ints(123, 256, n, hp.data());
+      AllocTxn txn(g);
+      if ((rc = g->upload(g->prio, hp))) return rc;
+      txn.commit();
+    }
+    dprio = g->prio;
+    if (!g->gcb_default_levels) {
+      HIP_TRY(hipMemcpy(hp.data(), g->prio, (size_t)n * 4, hipMemcpyDeviceToHost));
+      if ((rc = gcb_levels(g, hp, dprio))) return rc;
+      g->gcb_default_levels = true;
+    }
+  }
+  int64_t hw[2 * 64];
+  for (int32_t q = 0; q < nq; ++q) {
+    hw[2 * q] = windows[q].start;
+    hw[2 * q + 1] = windows[q].end;
+  }
+  HIP_TRY(hipMemcpyAsync(g->prb_win, hw, sizeof(int64_t) * 2 * (size_t)nq, hipMemcpyHostToDevice, st));
+  int32_t* out = color;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if ((rc = g->rows_buffer(g->ccb_out, g->ccb_out_rows, nq))) return rc;
+    out = g->ccb_out;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  const std::vector<int64_t>& off = g->gcb_level_off;
+  const int64_t nl = ((int64_t)off.size() - 1) / 2;
+  g->kbegin();
+  if (g->prb_nmulti)
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
+                                                                       g->coff, g->ct, g->prb_win, nq, g->ccb_mask);
+  k_fill<int32_t><<<g->grid(n * 64), kBlock, 0, st>>>(g->ccb_label, n * 64, 0);
+  HIP_TRY(hipGetLastError());
+  int32_t* const fl = g->gcb_flags;
+  // Pass k of level l: k == 1 over all its segments, k > 1 over its inner
+  // segments, raising flag k; with `chain`, pass k > 1 runs only if pass
+  // k - 1 raised a color.
+  auto pass = [&](int64_t l, int k, bool chain) -> int {
+    const int64_t a = off[2 * (size_t)l], b = off[2 * (size_t)l + 1], c = off[2 * (size_t)l + 2];
+    const int64_t s0 = k == 1 ? a : b, ns = k == 1 ? b - a : c - b;
+    if (ns == 0) return TGA_OK;
+    int32_t* f = fl + (size_t)l * (kGcbPasses + 1);
+    k_gcb_level<<<g->grid(ns * 64), kBlock, 0, st>>>(g->gcb_segs + s0, (int32_t)ns, g->prb_oseg_row, g->prb_oseg_beg,
+                                                      g->prb_oseg_len, g->prb_out_runs, g->prb_midx, g->ccb_mask,
+                                                      g->prb_win, nq, dprio, g->ccb_label, f + k,
+                                                      chain && k > 1 ? f + k - 1 : nullptr);
+    HIP_TRY(hipGetLastError());
+    return TGA_OK;
+  };
+  auto has_segs = [&](int64_t l) { return off[2 * (size_t)l + 1] > off[2 * (size_t)l]; };
+  auto sweep = [&]() -> int {
+    // every flag starts at 0, so levels without segments read as settled
+    k_fill<int32_t><<<g->grid((int64_t)kGcbLevels * (kGcbPasses + 1)), kBlock, 0, st>>>(
+        fl, (int64_t)kGcbLevels * (kGcbPasses + 1), 0);
+    for (int64_t l = 0; l < nl; ++l)
+      for (int k = 1; k <= kGcbPasses && has_segs(l); ++k)
+        if ((rc = pass(l, k, true))) return rc;
+    return TGA_OK;
+  };
+  if (priority) {
+    if ((rc = sweep())) return rc;
+  } else if ((rc = g->replay(g->gr_gcb, (uint64_t)nq, sweep))) {
+    return rc;
+  }
+  // A level still raising colors in its last pass (an outranking chain of
+  // more than kGcbPasses - 1 vertices inside one level) is finished here,
+  // one pass at a time, and the levels after it are swept again.
+  std::vector<int32_t> hf((size_t)kGcbLevels * (kGcbPasses + 1));
+  HIP_TRY(hipMemcpyAsync(hf.data(), fl, hf.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int64_t first = -1;
+  for (int64_t l = 0; l < nl && first < 0; ++l)
+    if (has_segs(l) && hf[(size_t)l * (kGcbPasses + 1) + kGcbPasses]) first = l;
+  if (first >= 0) {
+    for (int64_t l = first; l < nl; ++l) {
+      if (!has_segs(l)) continue;
+      // A later level saw too small colors of this one: a pass over all its
+      // segments, then inner passes until one raises nothing.  Colors only
+      // rise and are bounded by the longest outranking path, so passes end;
+      // the bound turns a broken invariant into an error.
+      int32_t* f = fl + (size_t)l * (kGcbPasses + 1);
+      if (l > first) {
+        k_fill<int32_t><<<1, 64, 0, st>>>(f + 1, 1, 0);
+        if ((rc = pass(l, 1, false))) return rc;
+      }
+      for (int64_t it = 0;; ++it) {
+        if (it > n) return fail(TGA_EFORMAT, "tga_graph_coloring_batch: a level did not settle");
+        k_fill<int32_t><<<1, 64, 0, st>>>(f + 2, 1, 0);
+        if ((rc = pass(l, 2, false))) return rc;
+        int32_t f1 = 0;
+        HIP_TRY(hipMemcpyAsync(&f1, f + 2, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (!f1) break;
+      }
+    }
+  }
+  k_gcb_out<<<g->grid(n * nq), kBlock, 0, st>>>(A, (int32_t)n, nq, rounds, g->ccb_label, out);
+  g->kend();
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE))
+    HIP_TRY(hipMemcpyAsync(color, out, sizeof(int32_t) * (size_t)nq * (size_t)n, hipMemcpyDeviceToHost, st));
   return g->check_err();
 }
 

@@ -96,6 +96,7 @@ def lib():
         L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
         L.tga_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
         L.tga_neighbor_access_noalloc.argtypes = [vp, p32, C.c_int64, C.POINTER(C.c_uint64), p64]
+        L.tga_graph_coloring_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, vp, vp, C.c_uint32]
         _LIB = L
     return _LIB
 
@@ -308,6 +309,29 @@ class TemporalGraph:
         else:
             arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
         _check(lib().tga_connected_components_batch(self.h, wins, nq, ptr, flags))
+        return arr
+
+    def graph_coloring_batch(self, windows, rounds: int = 512, priority=None, out=None):
+        """Batched graph coloring: up to 64 windows in one call (one priority
+        array for all).  Returns an (nq, n) int32 array whose row q equals
+        graph_coloring(windows[q], rounds, priority); `out` may be a caller's
+        (nq, n) device tensor instead (then `priority`, if given, is a device
+        tensor too)."""
+        nq = len(windows)
+        wins = (Window * nq)(*[_win(w) for w in windows])
+        if out is None:
+            arr = np.empty((nq, self.n), np.int32)
+            ptr, flags = arr.ctypes.data_as(C.c_void_p), 0
+        else:
+            arr, ptr, flags = out, C.c_void_p(out.data_ptr()), TGA_OUT_DEVICE
+        pr = None
+        if priority is not None:
+            if flags & TGA_OUT_DEVICE:
+                pr = C.c_void_p(priority.data_ptr())
+            else:
+                priority = np.ascontiguousarray(priority, np.int32)
+                pr = priority.ctypes.data_as(C.c_void_p)
+        _check(lib().tga_graph_coloring_batch(self.h, wins, nq, rounds, pr, ptr, flags))
         return arr
 
     def graph_coloring(self, window=None, rounds: int = 512, priority=None, out=None):

@@ -242,6 +242,17 @@ int tga_hits(tga_graph* g, tga_window w, int32_t iters, float* hub, float* auth,
 int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t iters, float* hub, float* auth,
                    uint32_t flags);
 
+/* Batched graph coloring: up to 64 windows, one priority array for all;
+ * row q of `color` (nq rows of MAX_VERTEX, host or device per flags, like
+ * `priority`) equals tga_graph_coloring(g, windows[q], rounds, priority, ...).
+ * The round rule makes a vertex's color one more than the largest color of
+ * its outranking window neighbors (0 without any), so the batch computes
+ * all windows in one sweep over priority levels, highest first, instead of
+ * `rounds` rounds per window.  No counterpart in the reference (one window
+ * per GC_kernel run, graph_coloring.cu:1634-1734). */
+int tga_graph_coloring_batch(tga_graph* g, const tga_window* windows, int32_t nq, int32_t rounds,
+                             const int32_t* priority, int32_t* color, uint32_t flags);
+
 /* Kernel timing on the handle's own stream: while enabled, every launch of a
  * decode kernel (the kernels that walk the compressed streams) is bracketed by
  * HIP events, and tga_profile_read returns the summed elapsed milliseconds and
