@@ -436,35 +436,35 @@ def served_bfs(g, t_lo: int, t_hi: int) -> dict:
 MIX = {"DC": 0.1, "PR": 0.3, "CC": 0.2}  # the rest BFS: the batching mix of the at-scale queries
 
 
-def served_mix(g, workers: dict, t_lo: int, t_hi: int, rates=((500, 400), (2000, 800))) -> dict:
+def served_mix(g, t_lo: int, t_hi: int, rates=((500, 400), (2000, 800))) -> dict:
     """Serving a mixed stream of window queries (40% BFS, 20% CC, 30% PR with
     20 iterations, 10% DC) at Poisson rates (default 500 and 2000 per second,
-    `rates` holds (rate, queries) pairs): BFS by
-    continuous batching in lanes; DC, PR and CC in batched calls, each kind on
-    a handle of its own in a worker thread (plain calls when fewer than
-    serving.MIN_BATCH are pending); against sequential plain execution.
-    Results in HBM.  The timed runs drop each result once answered (as a
-    server handing results on would); at the first rate a further batched and
-    plain run keep every result, and each is compared with the plain one
-    (PageRank rows within 1e-6 relative L-inf, the rest exact)."""
+    `rates` holds (rate, queries) pairs) on one handle: BFS, CC and PR by
+    continuous admission into their servers' lanes, DC in batched calls, with
+    every kind's state charged against the 8 GB budget; against sequential
+    plain execution.  Results in HBM.  The timed runs drop each result once
+    answered (as a server handing results on would); at the first rate a
+    further batched and plain run keep every result, and each is compared
+    with the plain one (PageRank rows within 1e-6 relative L-inf, the rest
+    exact)."""
     import torch
     from compresstemporalgraph_amd import serving
-    # Untimed: each worker handle builds its per-handle caches (run structure,
-    # contact-time cache) on a first small batched call, as a long-running
-    # server would have done before its first request.
+    # Untimed: the handle builds its caches (run structure, contact-time
+    # cache) on a first small batched call, as a long-running server would
+    # have done before its first request.
     wins = [(t_lo, t_hi), (t_lo, (t_lo + t_hi) // 2)]
-    workers["PR"].pagerank_batch(wins, iters=1)
-    workers["CC"].connected_components_batch(wins)
-    workers["DC"].degree_centrality_batch(wins)
+    g.pagerank_batch(wins, iters=1)
+    g.connected_components_batch(wins)
+    g.degree_centrality_batch(wins)
     out = {}
     for rate, count in rates:
         req = serving.mixed_requests(rate, count, g.info["n"], t_lo, t_hi, seed=rate + 7, shares=MIX)
-        sb = serving.serve_batched(g, req, device_out=True, workers=workers)
+        sb = serving.serve_batched(g, req, device_out=True)
         sp = serving.serve_plain(g, req, device_out=True)
         row = {"batched": sb, "plain": sp}
         if rate == rates[0][0]:
             kb, kp = {}, {}
-            serving.serve_batched(g, req, keep=kb, device_out=True, workers=workers)
+            serving.serve_batched(g, req, keep=kb, device_out=True)
             serving.serve_plain(g, req, keep=kp, device_out=True)
             def same(a, b):
                 if isinstance(a, tuple):
@@ -568,13 +568,7 @@ def query_legs(args) -> int:
     if args.mix_only:
         # The serving configuration on this graph: the same mix at Poisson
         # 100/s, 1k/s and 10k/s.
-        workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
-                   for k in ("DC", "PR", "CC")}
-        for h in workers.values():
-            h.autotune(passes=2)
-        smix = served_mix(g, workers, t_lo, t_hi, rates=((100, 200), (1000, 1000), (10000, 4000)))
-        for h in workers.values():
-            h.close()
+        smix = served_mix(g, t_lo, t_hi, rates=((100, 200), (1000, 1000), (10000, 4000)))
         log(f"[bench] served mix on {args.workload}: {smix}")
         bgc = batched_gc(g, t_lo, t_hi)
         log(f"[bench] batched GC on {args.workload}: {bgc}")
@@ -611,13 +605,7 @@ def query_legs(args) -> int:
     if not args.no_served:
         served = served_bfs(g, t_lo, t_hi)
         log(f"[bench] served BFS: {served}")
-        workers = {k: TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
-                   for k in ("DC", "PR", "CC")}
-        for h in workers.values():
-            h.autotune(passes=2)
-        smix = served_mix(g, workers, t_lo, t_hi)
-        for h in workers.values():
-            h.close()
+        smix = served_mix(g, t_lo, t_hi)
         log(f"[bench] served BFS/CC/PR/DC mix: {smix}")
     g.close()
     print(json.dumps({"batched_mix": bmix, "batched_bfs": batched, "batched_dc": bdc, "batched_cc": bcc,

@@ -772,11 +772,11 @@ __global__ void __launch_bounds__(kBlock) k_multi_list(uint64_t nv, const int4* 
 // lanes over 64 contacts at a time and one ballot per query still open.
 __global__ void __launch_bounds__(kBlock) k_prb_wmulti(const int32_t* mlist, int32_t nm, const int4* runs,
                                                        const uint64_t* coff, const int64_t* ct, const int64_t* wins,
-                                                       int32_t nq, float* wm) {
+                                                       int32_t nq, float* wm, uint64_t only) {
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
   const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
-  const uint64_t all = nq >= 64 ? ~0ull : ((1ull << nq) - 1);
+  const uint64_t all = (nq >= 64 ? ~0ull : ((1ull << nq) - 1)) & only;
   const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
   for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nm; i += nw) {
     const int4 run = runs[mlist[i]];
@@ -787,7 +787,7 @@ __global__ void __launch_bounds__(kBlock) k_prb_wmulti(const int32_t* mlist, int
       const int32_t k = b0 + lane;
       const int64_t t = k < run.z ? tb[k] : kNoTime;  // r is active: times as stored
       for (int q = 0; q < nq; ++q) {
-        if (found >> q & 1) continue;
+        if ((found | ~all) >> q & 1) continue;
         const int64_t qs = readlane64(ws, q), qe = readlane64(we, q);
         const uint64_t m = __ballot(t >= qs && t <= qe);
         if (!m) continue;
@@ -796,7 +796,7 @@ __global__ void __launch_bounds__(kBlock) k_prb_wmulti(const int32_t* mlist, int
         found |= 1ull << q;
       }
     }
-    wm[(int64_t)i * 64 + lane] = w;
+    if (only >> lane & 1) wm[(int64_t)i * 64 + lane] = w;
   }
 }
 
@@ -827,17 +827,21 @@ __global__ void __launch_bounds__(kBlock) k_seg_fill(int32_t n, const uint64_t* 
 // Window degree of every vertex for every query (all window contacts, any
 // neighbor): one wave per vertex, lane q counting query q; records longer
 // than kPrbLongDeg are left to k_prb_du_long.
+// Batch-state kernels take `only`: the lanes whose columns they write (the
+// lanes a query server admits; ~0 for a whole batch).
 __global__ void __launch_bounds__(kBlock) k_prb_du(Window A, int32_t n, const int32_t* vdeg, const uint64_t* coff,
-                                                   const int64_t* ct, const int64_t* wins, int32_t nq, int32_t* du) {
+                                                   const int64_t* ct, const int64_t* wins, int32_t nq, int32_t* du,
+                                                   uint64_t only) {
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
+  const bool mine = only >> lane & 1;
   const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
   const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
   for (int64_t v = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); v < n; v += nw) {
     const bool act = A.active((int32_t)v);
     const int32_t len = act ? vdeg[v] : 0;
     if (len > kPrbLongDeg) {
-      du[v * 64 + lane] = 0;  // counted by k_prb_du_long
+      if (mine) du[v * 64 + lane] = 0;  // counted by k_prb_du_long
       continue;
     }
     int32_t c = 0;
@@ -846,7 +850,7 @@ __global__ void __launch_bounds__(kBlock) k_prb_du(Window A, int32_t n, const in
       const int64_t t = ct_time(tb[i]);
       c += (t >= ws && t <= we) ? 1 : 0;
     }
-    du[v * 64 + lane] = c;
+    if (mine) du[v * 64 + lane] = c;
   }
 }
 
@@ -857,9 +861,9 @@ constexpr int kPrbDuChunk = 4096;
 
 __global__ void __launch_bounds__(kBlock) k_prb_du_long(const int2* items, int32_t ni, const uint64_t* coff,
                                                         const int32_t* vdeg, const int64_t* ct, const int64_t* wins,
-                                                        int32_t nq, int32_t* du) {
+                                                        int32_t nq, int32_t* du, uint64_t only) {
   const int lane = threadIdx.x & 63;
-  const bool on = lane < nq;
+  const bool on = lane < nq && (only >> lane & 1);
   const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
   const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
   for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < ni; i += nw) {
@@ -886,9 +890,40 @@ __global__ void __launch_bounds__(kBlock) k_prb_du_items(const int32_t* lv, int3
 }
 
 // rank = 1/N for active vertices (all 64 columns), coef and base per iteration.
-__global__ void __launch_bounds__(kBlock) k_prb_init(Window A, int32_t n, float inv, float* rank) {
+__global__ void __launch_bounds__(kBlock) k_prb_init(Window A, int32_t n, float inv, float* rank, uint64_t only) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock)
-    rank[i] = A.active((int32_t)(i >> 6)) ? inv : 0.0f;
+    if (only >> (i & 63) & 1) rank[i] = A.active((int32_t)(i >> 6)) ? inv : 0.0f;
+}
+// One iteration's coefficients for lanes at different iterations (a query
+// server): lanes in `fresh` start from their initial rank, the others from
+// the last iteration's sums.
+__global__ void __launch_bounds__(kBlock) k_prb_coef_step(Window A, int32_t n, const float* rank, const double* sums,
+                                                          const int32_t* du, float damping, float* coef, double base,
+                                                          double* acc, uint64_t fresh) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
+    const bool act = A.active((int32_t)(i >> 6));
+    float r;
+    if (fresh >> (i & 63) & 1) {
+      r = rank[i];
+    } else {
+      const double s = sums[i & 63];
+      r = (float)(s > 0 ? acc[i] / s : acc[i]);
+    }
+    coef[i] = (act && du[i] > 0) ? damping * r / (float)du[i] : 0.0f;
+    acc[i] = act ? base : 0.0;
+  }
+}
+// Column q of [vertex][64] values as a row (query servers' results).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_col_out(int32_t n, int32_t q, const T* cols, T* out) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    out[v] = cols[v * 64 + q];
+}
+__global__ void __launch_bounds__(kBlock) k_prb_col_out(int32_t n, int32_t q, const double* acc, const double* sums,
+                                                        float* out) {
+  const double s = sums[q];
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    out[v] = (float)(s > 0 ? acc[v * 64 + q] / s : acc[v * 64 + q]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_prb_coef(Window A, int32_t n, const float* rank, const int32_t* du,
@@ -1025,11 +1060,11 @@ __global__ void __launch_bounds__(64) k_prb_colsum2(const double* part, int32_t 
 // lowers a label.
 __global__ void __launch_bounds__(kBlock) k_ccb_mask(const int32_t* mlist, int32_t nm, const int4* runs,
                                                      const uint64_t* coff, const int64_t* ct, const int64_t* wins,
-                                                     int32_t nq, uint64_t* mask) {
+                                                     int32_t nq, uint64_t* mask, uint64_t only) {
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
   const int64_t ws = on ? clamp_lo(wins[2 * lane]) : 1, we = on ? clamp_hi(wins[2 * lane + 1]) : 0;
-  const uint64_t all = nq >= 64 ? ~0ull : ((1ull << nq) - 1);
+  const uint64_t all = (nq >= 64 ? ~0ull : ((1ull << nq) - 1)) & only;
   const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
   for (int64_t i = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6); i < nm; i += nw) {
     const int4 run = runs[mlist[i]];
@@ -1039,19 +1074,19 @@ __global__ void __launch_bounds__(kBlock) k_ccb_mask(const int32_t* mlist, int32
       const int32_t k = b0 + lane;
       const int64_t t = k < run.z ? tb[k] : kNoTime;
       for (int q = 0; q < nq; ++q) {
-        if (bits >> q & 1) continue;
+        if ((bits | ~all) >> q & 1) continue;
         const int64_t qs = readlane64(ws, q), qe = readlane64(we, q);
         if (__ballot(t >= qs && t <= qe)) bits |= 1ull << q;
       }
     }
-    if (lane == 0) mask[i] = bits;
+    if (lane == 0) mask[i] = only == ~0ull ? bits : ((mask[i] & ~only) | bits);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_ccb_init(Window A, int32_t n, int32_t* label) {
+__global__ void __launch_bounds__(kBlock) k_ccb_init(Window A, int32_t n, int32_t* label, uint64_t only) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (int64_t)n * 64; i += (int64_t)gridDim.x * kBlock) {
     const int32_t v = (int32_t)(i >> 6);
-    label[i] = A.active(v) ? v : -1;
+    if (only >> (i & 63) & 1) label[i] = A.active(v) ? v : -1;
   }
 }
 
@@ -1059,7 +1094,7 @@ __global__ void __launch_bounds__(kBlock) k_ccb_pull(int64_t nseg, const int32_t
                                                      const int32_t* seg_len, const uint64_t* seg_off, const int4* runs,
                                                      const int32_t* midx, const uint64_t* mask, const int64_t* wins,
                                                      int32_t nq, int32_t* label, int32_t* partial, int32_t* changed,
-                                                     const int32_t* run_if) {
+                                                     const int32_t* run_if, unsigned long long* lane_changed) {
   if (run_if && *run_if == 0) return;  // an earlier round of the group lowered nothing: converged
   const int lane = threadIdx.x & 63;
   const bool on = lane < nq;
@@ -1106,13 +1141,18 @@ __global__ void __launch_bounds__(kBlock) k_ccb_pull(int64_t nseg, const int32_t
         lowered = true;
       }
     }
-    if (__ballot(lowered) && lane == 0) *changed = 1;
+    const uint64_t low = __ballot(lowered);
+    if (low && lane == 0) {
+      *changed = 1;
+      if (lane_changed) atomicOr(lane_changed, (unsigned long long)low);
+    }
   }
 }
 
 __global__ void __launch_bounds__(kBlock) k_ccb_fix(const int32_t* mrows, const uint32_t* nm, const uint64_t* seg_off,
                                                     const int32_t* partial, int32_t nq, int32_t* label,
-                                                    int32_t* changed, const int32_t* run_if) {
+                                                    int32_t* changed, const int32_t* run_if,
+                                                    unsigned long long* lane_changed) {
   if (run_if && *run_if == 0) return;
   const int lane = threadIdx.x & 63;
   const uint32_t cnt = *nm;
@@ -1129,7 +1169,11 @@ __global__ void __launch_bounds__(kBlock) k_ccb_fix(const int32_t* mrows, const 
         lowered = true;
       }
     }
-    if (__ballot(lowered) && lane == 0) *changed = 1;
+    const uint64_t low = __ballot(lowered);
+    if (low && lane == 0) {
+      *changed = 1;
+      if (lane_changed) atomicOr(lane_changed, (unsigned long long)low);
+    }
   }
 }
 
@@ -2420,7 +2464,46 @@ __global__ void __launch_bounds__(kBlock) k_scale_store(const double* x, const d
     out[i] = (float)(s > 0 ? x[i] / s : x[i]);
 }
 
+// Lanes that finished at the end of a PR server step: rank[v][q] = float(acc
+// / sum) as k_prb_out, kept in the (then unused) initial-rank column.
+__global__ void __launch_bounds__(kBlock) k_prb_fin(int32_t n, uint64_t lanes, const double* acc, const double* sums,
+                                                    float* rank) {
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+    for (uint64_t b = lanes; b; b &= b - 1) {
+      const int q = __ffsll((unsigned long long)b) - 1;
+      const double s = sums[q];
+      const int64_t i = v * 64 + q;
+      rank[i] = (float)(s > 0 ? acc[i] / s : acc[i]);
+    }
+}
+
 }  // namespace
+
+struct tga_graph;
+// State shared by the query servers (BFS, CC, PR): 64 lanes, each holding
+// one query from admission to the collection of its result.  Their device
+// state lives in allocations of the handle (so the handle's device byte
+// count includes it); closing the handle first orphans a server (g reset):
+// its calls then fail, and closing it frees nothing more.
+struct LaneServer {
+  tga_graph* g = nullptr;
+  uint64_t running = 0, pending = 0, done = 0;
+  tga_window h_win[64] = {};
+  std::vector<void*> mem;
+  int64_t* h_wtab = nullptr;       // pinned window table being uploaded
+  hipEvent_t ev_wtab = nullptr;    // its last upload
+  explicit LaneServer(tga_graph* g_);
+  virtual ~LaneServer();
+  template <class T>
+  int alloc(T*& p, size_t count);
+  // A free lane for a new query, or -1.
+  int free_lane() const {
+    const uint64_t used = running | pending | done;
+    return ~used ? __builtin_ctzll(~used) : -1;
+  }
+  LaneServer(const LaneServer&) = delete;
+  LaneServer& operator=(const LaneServer&) = delete;
+};
 
 struct tga_graph {
   int device = 0;
@@ -2537,6 +2620,7 @@ struct tga_graph {
   std::vector<int64_t> gcb_level_off;      // level boundaries in gcb_segs
   bool gcb_default_levels = false;         // gcb_segs holds the default priorities' levels
   bool use_graphs = true;  // TGA_GRAPHS=0: launch the loops one kernel at a time
+  std::vector<LaneServer*> servers;  // open query servers (orphaned when the handle closes)
   std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
   bool prof = false;
@@ -2729,7 +2813,16 @@ struct tga_graph {
     }
     return TGA_OK;
   }
+  // Frees one allocation of the handle by address.
+  void release_ptr(void* p) {
+    for (size_t i = allocs.size(); p && i-- > 0;)
+      if (allocs[i] == p) {
+        free_at(i);
+        return;
+      }
+  }
   ~tga_graph() {
+    for (LaneServer* s : servers) s->g = nullptr;
     if (stream) (void)hipStreamSynchronize(stream);
     for (Graph* gr : {&gr_prb, &gr_hb, &gr_ccb, &gr_gcb})
       if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
@@ -2748,6 +2841,27 @@ struct tga_graph {
 // unless commit() is reached, every allocation made since construction is
 // freed and its handle member reset, so a later call never finds a group
 // half allocated (its first pointer set, the others null).
+LaneServer::LaneServer(tga_graph* g_) : g(g_) { g->servers.push_back(this); }
+LaneServer::~LaneServer() {
+  if (g) {
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->stream);
+    for (void* p : mem) g->release_ptr(p);
+    g->servers.erase(std::find(g->servers.begin(), g->servers.end(), this));
+  }
+  if (ev_wtab) {
+    (void)hipEventSynchronize(ev_wtab);
+    (void)hipEventDestroy(ev_wtab);
+  }
+  if (h_wtab) (void)hipHostFree(h_wtab);
+}
+template <class T>
+int LaneServer::alloc(T*& p, size_t count) {
+  const int rc = g->alloc(p, count);
+  if (!rc) mem.push_back((void*)p);
+  return rc;
+}
+
 struct AllocTxn {
   tga_graph* g;
   size_t mark;
@@ -3671,14 +3785,15 @@ extern "C" int tga_pagerank_batch(tga_graph* g, const tga_window* windows, int32
   const float inv = 1.0f / (float)N;
   const float base = (1.0f - damping) / (float)N;
   g->kbegin();
-  k_prb_du<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->vdeg, g->coff, g->ct, g->prb_win, nq, g->prb_du);
+  k_prb_du<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->vdeg, g->coff, g->ct, g->prb_win, nq, g->prb_du,
+                                                ~0ull);
   if (g->prb_ndu_items)
     k_prb_du_long<<<g->grid((int64_t)g->prb_ndu_items * 64), kBlock, 0, st>>>(
-        g->prb_du_items, g->prb_ndu_items, g->coff, g->vdeg, g->ct, g->prb_win, nq, g->prb_du);
+        g->prb_du_items, g->prb_ndu_items, g->coff, g->vdeg, g->ct, g->prb_win, nq, g->prb_du, ~0ull);
   if (g->prb_nmulti)
-    k_prb_wmulti<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
-                                                                         g->coff, g->ct, g->prb_win, nq, g->prb_wm);
-  k_prb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, inv, g->prb_rank);
+    k_prb_wmulti<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+        g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, g->prb_win, nq, g->prb_wm, ~0ull);
+  k_prb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, inv, g->prb_rank, ~0ull);
   HIP_TRY(hipGetLastError());
   // The iterations replay one captured graph (cached per batch shape).
   auto iterations = [&]() -> int {
@@ -3757,9 +3872,9 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
   g->kbegin();
   if (g->prb_nmulti)
-    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
-                                                                       g->coff, g->ct, g->prb_win, nq, g->ccb_mask);
-  k_ccb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->ccb_label);
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+        g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, g->prb_win, nq, g->ccb_mask, ~0ull);
+  k_ccb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->ccb_label, ~0ull);
   HIP_TRY(hipGetLastError());
   // Rounds run in groups of kCcbGroup, replayed from one captured graph:
   // flag r of a group records whether round r lowered a label, and the
@@ -3772,11 +3887,11 @@ extern "C" int tga_connected_components_batch(tga_graph* g, const tga_window* wi
       const int32_t* run_if = r ? g->ccb_changed + r : nullptr;
       k_ccb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
           g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
-          g->ccb_mask, g->prb_win, nq, g->ccb_label, g->ccb_partial, g->ccb_changed + r + 1, run_if);
+          g->ccb_mask, g->prb_win, nq, g->ccb_label, g->ccb_partial, g->ccb_changed + r + 1, run_if, nullptr);
       if (g->prb_nmrows)
         k_ccb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(
             g->prb_mrows, g->prb_nm, g->prb_seg_off, g->ccb_partial, nq, g->ccb_label, g->ccb_changed + r + 1,
-            run_if);
+            run_if, nullptr);
       HIP_TRY(hipGetLastError());
     }
     return TGA_OK;
@@ -3841,8 +3956,8 @@ extern "C" int tga_hits_batch(tga_graph* g, const tga_window* windows, int32_t n
   const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
   g->kbegin();
   if (g->prb_nmulti)
-    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
-                                                                       g->coff, g->ct, g->prb_win, nq, g->hb_mask);
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+        g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, g->prb_win, nq, g->hb_mask, ~0ull);
   k_cols_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, 1.0f, g->hb_hub, g->hb_auth);
   HIP_TRY(hipGetLastError());
   auto iterations = [&]() -> int {
@@ -3981,39 +4096,23 @@ extern "C" int tga_bfs_batch(tga_graph* g, const tga_window* windows, const int3
 // queries are admitted into free lanes between steps, every step advances all
 // running lanes by one level in one decode pass, and a lane whose frontier
 // empties is finished (its level row is final) until its result is taken.
-struct tga_bfs_server {
-  tga_graph* g = nullptr;
+struct tga_bfs_server : LaneServer {
   uint64_t *front = nullptr, *vis = nullptr, *next = nullptr, *mask = nullptr;
   int64_t* bound = nullptr;
   int32_t *src = nullptr, *start = nullptr, *levels = nullptr;
   unsigned long long* alive = nullptr;
   int32_t nb = 0;
   int32_t step = 0;
-  uint64_t running = 0, pending = 0, done = 0;
   int32_t h_src[kMaxQueries] = {};
   int32_t h_start[kMaxQueries] = {};
-  tga_window h_win[kMaxQueries] = {};
-  std::vector<void*> mem;
-  template <class T>
-  int alloc(T*& p, size_t count) {
-    void* q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return fail(TGA_ENOMEM, "hipMalloc failed");
-    mem.push_back(q);
-    p = (T*)q;
-    return TGA_OK;
-  }
-  ~tga_bfs_server() {
-    if (g && g->stream) (void)hipStreamSynchronize(g->stream);
-    for (void* p : mem) (void)hipFree(p);
-  }
+  using LaneServer::LaneServer;
 };
 
 extern "C" int tga_bfs_server_open(tga_graph* g, tga_bfs_server** out) {
   if (!g || !out) return fail(TGA_EINVAL, "tga_bfs_server_open: NULL argument");
   *out = nullptr;
   HIP_TRY(hipSetDevice(g->device));
-  auto* s = new tga_bfs_server();
-  s->g = g;
+  auto* s = new tga_bfs_server(g);
   const size_t n = (size_t)g->n;
   int rc;
   if ((rc = s->alloc(s->front, n)) || (rc = s->alloc(s->vis, n)) || (rc = s->alloc(s->next, n)) ||
@@ -4037,11 +4136,10 @@ extern "C" int tga_bfs_server_open(tga_graph* g, tga_bfs_server** out) {
 extern "C" void tga_bfs_server_close(tga_bfs_server* s) { delete s; }
 
 extern "C" int tga_bfs_server_admit(tga_bfs_server* s, int32_t src, tga_window w, int32_t* lane) {
-  if (!s || !lane || bad_window(w) || src < 0 || src >= s->g->n)
-    return fail(TGA_EINVAL, "tga_bfs_server_admit: bad arguments");
-  const uint64_t used = s->running | s->pending | s->done;
-  if (~used == 0) return fail(TGA_EBUSY, "tga_bfs_server_admit: all 64 lanes are busy");
-  const int q = __builtin_ctzll(~used);
+  if (!s || !s->g || !lane || bad_window(w) || src < 0 || src >= s->g->n)
+    return fail(TGA_EINVAL, "tga_bfs_server_admit: bad arguments (or the handle is closed)");
+  const int q = s->free_lane();
+  if (q < 0) return fail(TGA_EBUSY, "tga_bfs_server_admit: all 64 lanes are busy");
   s->h_src[q] = src;
   s->h_win[q] = w;
   s->pending |= 1ull << q;
@@ -4050,7 +4148,7 @@ extern "C" int tga_bfs_server_admit(tga_bfs_server* s, int32_t src, tga_window w
 }
 
 extern "C" int tga_bfs_server_step(tga_bfs_server* s, uint64_t* finished) {
-  if (!s || !finished) return fail(TGA_EINVAL, "tga_bfs_server_step: NULL argument");
+  if (!s || !s->g || !finished) return fail(TGA_EINVAL, "tga_bfs_server_step: NULL argument (or the handle is closed)");
   tga_graph* g = s->g;
   HIP_TRY(hipSetDevice(g->device));
   const int64_t n = g->n;
@@ -4109,8 +4207,8 @@ extern "C" int tga_bfs_server_step(tga_bfs_server* s, uint64_t* finished) {
 }
 
 extern "C" int tga_bfs_server_result(tga_bfs_server* s, int32_t lane, int32_t* level, uint32_t flags) {
-  if (!s || !level || lane < 0 || lane >= kMaxQueries || !(s->done >> lane & 1))
-    return fail(TGA_EINVAL, "tga_bfs_server_result: lane has no finished query");
+  if (!s || !s->g || !level || lane < 0 || lane >= kMaxQueries || !(s->done >> lane & 1))
+    return fail(TGA_EINVAL, "tga_bfs_server_result: lane has no finished query (or the handle is closed)");
   tga_graph* g = s->g;
   HIP_TRY(hipSetDevice(g->device));
   int rc = g->out_copy(level, s->levels + (int64_t)lane * g->n, (size_t)g->n * 4, flags);
@@ -4118,6 +4216,311 @@ extern "C" int tga_bfs_server_result(tga_bfs_server* s, int32_t lane, int32_t* l
   HIP_TRY(hipStreamSynchronize(g->stream));
   s->done &= ~(1ull << lane);
   return TGA_OK;
+}
+
+// ------------------------------------------------ CC and PageRank servers ---
+//
+// Continuous admission into the batched column state of
+// tga_connected_components_batch / tga_pagerank_batch: all 64 lanes are
+// processed every step (free lanes hold empty windows), and a query admitted
+// between steps starts in its own lane at the next one while the others go
+// on.  Lanes are independent columns, so every query's result is the one its
+// batched (and plain) call returns.
+
+// Window table of a server's lanes: queries of running or pending lanes,
+// empty windows [1, 0] elsewhere.
+// The table goes through a pinned buffer, rewritten only once its last
+// upload has been done (the step does not wait for the device).
+static int upload_windows(LaneServer* s, int64_t* win) {
+  if (!s->h_wtab) {
+    HIP_TRY(hipHostMalloc((void**)&s->h_wtab, sizeof(int64_t) * 2 * 64, hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_wtab, hipEventDisableTiming));
+  } else {
+    HIP_TRY(hipEventSynchronize(s->ev_wtab));
+  }
+  const uint64_t live = s->running | s->pending | s->done;
+  for (int q = 0; q < 64; ++q) {
+    const bool on = live >> q & 1;
+    s->h_wtab[2 * q] = on ? s->h_win[q].start : 1;
+    s->h_wtab[2 * q + 1] = on ? s->h_win[q].end : 0;
+  }
+  HIP_TRY(hipMemcpyAsync(win, s->h_wtab, sizeof(int64_t) * 2 * 64, hipMemcpyHostToDevice, s->g->stream));
+  HIP_TRY(hipEventRecord(s->ev_wtab, s->g->stream));
+  return TGA_OK;
+}
+
+template <class T>
+static int lane_result(LaneServer* s, int32_t lane, const T* cols, T*& stage, T* out, uint32_t flags) {
+  tga_graph* g = s->g;
+  HIP_TRY(hipSetDevice(g->device));
+  T* dst = out;
+  if (!(flags & TGA_OUT_DEVICE)) {
+    if (!stage) {
+      const int rc = s->alloc(stage, (size_t)g->n);
+      if (rc) return rc;
+    }
+    dst = stage;
+  }
+  k_col_out<T><<<g->grid(g->n), kBlock, 0, g->stream>>>((int32_t)g->n, lane, cols, dst);
+  HIP_TRY(hipGetLastError());
+  if (!(flags & TGA_OUT_DEVICE))
+    HIP_TRY(hipMemcpyAsync(out, dst, sizeof(T) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  s->done &= ~(1ull << lane);
+  return TGA_OK;
+}
+
+struct tga_cc_server : LaneServer {
+  int32_t *label = nullptr, *partial = nullptr, *changed = nullptr, *stage = nullptr;
+  uint64_t* mask = nullptr;
+  unsigned long long* lane_changed = nullptr;  // per round of a group: lanes that lowered a label
+  int64_t* win = nullptr;
+  tga_graph::Graph gr;
+  using LaneServer::LaneServer;
+  ~tga_cc_server() override {
+    if (g && gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  }
+};
+
+extern "C" int tga_cc_server_open(tga_graph* g, tga_cc_server** out) {
+  if (!g || !out) return fail(TGA_EINVAL, "tga_cc_server_open: NULL argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(g->device));
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) return fail(TGA_ENOMEM, "tga_cc_server_open: no room for the run structure");
+  auto* s = new tga_cc_server(g);
+  const int64_t n = g->n;
+  if ((rc = s->alloc(s->label, (size_t)n * 64)) ||
+      (rc = s->alloc(s->partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
+      (rc = s->alloc(s->mask, (size_t)std::max<int32_t>(g->prb_nmulti, 1))) ||
+      (rc = s->alloc(s->changed, kCcbGroup + 1)) || (rc = s->alloc(s->lane_changed, kCcbGroup)) ||
+      (rc = s->alloc(s->win, 2 * 64))) {
+    delete s;
+    return rc;
+  }
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  k_fill<uint64_t><<<g->grid(g->prb_nmulti), kBlock, 0, g->stream>>>(s->mask, std::max<int32_t>(g->prb_nmulti, 1), 0);
+  k_ccb_init<<<g->grid(n * 64), kBlock, 0, g->stream>>>(A, (int32_t)n, s->label, ~0ull);
+  if (hipGetLastError() != hipSuccess || (rc = upload_windows(s, s->win))) {
+    delete s;
+    return rc ? rc : fail(TGA_EDEVICE, "tga_cc_server_open: initialisation failed");
+  }
+  *out = s;
+  return TGA_OK;
+}
+
+extern "C" void tga_cc_server_close(tga_cc_server* s) { delete s; }
+
+extern "C" int tga_cc_server_admit(tga_cc_server* s, tga_window w, int32_t* lane) {
+  if (!s || !s->g || !lane || bad_window(w))
+    return fail(TGA_EINVAL, "tga_cc_server_admit: bad arguments (or the handle is closed)");
+  const int q = s->free_lane();
+  if (q < 0) return fail(TGA_EBUSY, "tga_cc_server_admit: all 64 lanes are busy");
+  s->h_win[q] = w;
+  s->pending |= 1ull << q;
+  *lane = q;
+  return TGA_OK;
+}
+
+// One step: admits the queued queries (their link masks and initial labels),
+// then runs kCcbGroup rounds from the server's captured graph.  A lane whose
+// labels no round of the group lowered is at its fixpoint: it finishes.
+extern "C" int tga_cc_server_step(tga_cc_server* s, uint64_t* finished) {
+  if (!s || !s->g || !finished) return fail(TGA_EINVAL, "tga_cc_server_step: NULL argument (or the handle is closed)");
+  tga_graph* g = s->g;
+  HIP_TRY(hipSetDevice(g->device));
+  hipStream_t st = g->stream;
+  const int64_t n = g->n;
+  *finished = 0;
+  int rc;
+  if (s->pending) {
+    if ((rc = upload_windows(s, s->win))) return rc;
+    const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+    g->kbegin();
+    if (g->prb_nmulti)
+      k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+          g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, s->win, 64, s->mask, s->pending);
+    k_ccb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, s->label, s->pending);
+    g->kend();
+    HIP_TRY(hipGetLastError());
+    s->running |= s->pending;
+    s->pending = 0;
+  }
+  if (!s->running) return TGA_OK;
+  uint64_t conv = ~0ull;  // no segments: every window's labels are the initial ones
+  if (g->prb_nseg) {
+    auto rounds = [&]() -> int {
+      for (int r = 0; r < kCcbGroup; ++r) {
+        k_fill<int32_t><<<1, kBlock, 0, st>>>(s->changed + r + 1, 1, 0);
+        k_fill<unsigned long long><<<1, kBlock, 0, st>>>(s->lane_changed + r, 1, 0ull);
+        const int32_t* run_if = r ? s->changed + r : nullptr;
+        k_ccb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+            g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_midx,
+            s->mask, s->win, 64, s->label, s->partial, s->changed + r + 1, run_if, s->lane_changed + r);
+        if (g->prb_nmrows)
+          k_ccb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(
+              g->prb_mrows, g->prb_nm, g->prb_seg_off, s->partial, 64, s->label, s->changed + r + 1, run_if,
+              s->lane_changed + r);
+        HIP_TRY(hipGetLastError());
+      }
+      return TGA_OK;
+    };
+    g->kbegin();
+    if ((rc = g->replay(s->gr, 1, rounds))) return rc;
+    g->kend();
+    unsigned long long lc[kCcbGroup];
+    HIP_TRY(hipMemcpyAsync(lc, s->lane_changed, sizeof lc, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    conv = 0;
+    for (int r = 0; r < kCcbGroup; ++r) conv |= ~(uint64_t)lc[r];
+  }
+  *finished = s->running & conv;
+  s->done |= *finished;
+  s->running &= ~*finished;
+  return TGA_OK;
+}
+
+extern "C" int tga_cc_server_result(tga_cc_server* s, int32_t lane, int32_t* label, uint32_t flags) {
+  if (!s || !s->g || !label || lane < 0 || lane >= 64 || !(s->done >> lane & 1))
+    return fail(TGA_EINVAL, "tga_cc_server_result: lane has no finished query (or the handle is closed)");
+  return lane_result<int32_t>(s, lane, s->label, s->stage, label, flags);
+}
+
+struct tga_pr_server : LaneServer {
+  float damping = 0.85f;
+  int32_t* du = nullptr;
+  float *rank = nullptr, *coef = nullptr, *wm = nullptr, *stage = nullptr;
+  double *acc = nullptr, *partial = nullptr, *sums = nullptr, *part = nullptr;
+  int64_t* win = nullptr;
+  int32_t h_iters[64] = {};
+  int32_t h_it[64] = {};
+  int32_t h_pend_iters[64] = {};
+  using LaneServer::LaneServer;
+};
+
+extern "C" int tga_pr_server_open(tga_graph* g, float damping, tga_pr_server** out) {
+  if (!g || !out) return fail(TGA_EINVAL, "tga_pr_server_open: NULL argument");
+  *out = nullptr;
+  if (g->ae - g->ab <= 0) return fail(TGA_EINVAL, "tga_pr_server_open: no active vertices");
+  HIP_TRY(hipSetDevice(g->device));
+  bool ok = false;
+  int rc = ensure_prb(g, &ok);
+  if (rc) return rc;
+  if (!ok) return fail(TGA_ENOMEM, "tga_pr_server_open: no room for the run structure");
+  auto* s = new tga_pr_server(g);
+  s->damping = damping;
+  const int64_t n = g->n;
+  if ((rc = s->alloc(s->du, (size_t)n * 64)) || (rc = s->alloc(s->rank, (size_t)n * 64)) ||
+      (rc = s->alloc(s->coef, (size_t)n * 64)) || (rc = s->alloc(s->acc, (size_t)n * 64)) ||
+      (rc = s->alloc(s->wm, (size_t)std::max<int32_t>(g->prb_nmulti, 1) * 64)) ||
+      (rc = s->alloc(s->partial, (size_t)std::max<int64_t>(g->prb_nseg, 1) * 64)) ||
+      (rc = s->alloc(s->sums, 64)) || (rc = s->alloc(s->part, (size_t)kPrbRedBlocks * 64)) ||
+      (rc = s->alloc(s->win, 2 * 64))) {
+    delete s;
+    return rc;
+  }
+  hipStream_t st = g->stream;
+  k_fill<int32_t><<<g->grid(n * 64), kBlock, 0, st>>>(s->du, n * 64, 0);
+  k_fill<float><<<g->grid(n * 64), kBlock, 0, st>>>(s->rank, n * 64, 0.0f);
+  k_fill<double><<<g->grid(n * 64), kBlock, 0, st>>>(s->acc, n * 64, 0.0);
+  k_fill<float><<<g->grid((int64_t)std::max<int32_t>(g->prb_nmulti, 1) * 64), kBlock, 0, st>>>(
+      s->wm, (int64_t)std::max<int32_t>(g->prb_nmulti, 1) * 64, 0.0f);
+  k_fill<double><<<1, kBlock, 0, st>>>(s->sums, 64, 0.0);
+  if (hipGetLastError() != hipSuccess || (rc = upload_windows(s, s->win))) {
+    delete s;
+    return rc ? rc : fail(TGA_EDEVICE, "tga_pr_server_open: initialisation failed");
+  }
+  *out = s;
+  return TGA_OK;
+}
+
+extern "C" void tga_pr_server_close(tga_pr_server* s) { delete s; }
+
+extern "C" int tga_pr_server_admit(tga_pr_server* s, tga_window w, int32_t iters, int32_t* lane) {
+  if (!s || !s->g || !lane || bad_window(w) || iters < 0)
+    return fail(TGA_EINVAL, "tga_pr_server_admit: bad arguments (or the handle is closed)");
+  const int q = s->free_lane();
+  if (q < 0) return fail(TGA_EBUSY, "tga_pr_server_admit: all 64 lanes are busy");
+  s->h_win[q] = w;
+  s->h_pend_iters[q] = iters;
+  s->pending |= 1ull << q;
+  *lane = q;
+  return TGA_OK;
+}
+
+// One step: admits the queued queries (window degrees, multi-contact
+// weights, initial ranks), then runs one iteration for every running lane;
+// lanes that reach their iteration count keep their ranks and finish.  The
+// step only enqueues work (the iteration counts are known on the host).
+extern "C" int tga_pr_server_step(tga_pr_server* s, uint64_t* finished) {
+  if (!s || !s->g || !finished) return fail(TGA_EINVAL, "tga_pr_server_step: NULL argument (or the handle is closed)");
+  tga_graph* g = s->g;
+  HIP_TRY(hipSetDevice(g->device));
+  hipStream_t st = g->stream;
+  const int64_t n = g->n;
+  const int64_t N = g->ae - g->ab;
+  const Window A{TGA_FULL_WINDOW_START, TGA_FULL_WINDOW_END, g->ab, g->ae};
+  const float damping = s->damping;
+  const float inv = 1.0f / (float)N;
+  const float base = (1.0f - damping) / (float)N;
+  *finished = 0;
+  int rc;
+  g->kbegin();
+  if (s->pending) {
+    if ((rc = upload_windows(s, s->win))) return rc;
+    const uint64_t P = s->pending;
+    k_prb_du<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, g->vdeg, g->coff, g->ct, s->win, 64, s->du, P);
+    if (g->prb_ndu_items)
+      k_prb_du_long<<<g->grid((int64_t)g->prb_ndu_items * 64), kBlock, 0, st>>>(
+          g->prb_du_items, g->prb_ndu_items, g->coff, g->vdeg, g->ct, s->win, 64, s->du, P);
+    if (g->prb_nmulti)
+      k_prb_wmulti<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+          g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, s->win, 64, s->wm, P);
+    k_prb_init<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, inv, s->rank, P);
+    HIP_TRY(hipGetLastError());
+    for (uint64_t b = P; b; b &= b - 1) {
+      const int q = __builtin_ctzll(b);
+      s->h_iters[q] = s->h_pend_iters[q];
+      s->h_it[q] = 0;
+      if (s->h_iters[q] == 0) *finished |= 1ull << q;  // the initial ranks are the result
+    }
+    s->running |= P & ~*finished;
+    s->pending = 0;
+  }
+  if (s->running) {
+    uint64_t fresh = 0, last = 0;
+    for (uint64_t b = s->running; b; b &= b - 1) {
+      const int q = __builtin_ctzll(b);
+      if (s->h_it[q] == 0) fresh |= 1ull << q;
+      if (++s->h_it[q] == s->h_iters[q]) last |= 1ull << q;
+    }
+    k_prb_coef_step<<<g->grid(n * 64), kBlock, 0, st>>>(A, (int32_t)n, s->rank, s->sums, s->du, damping, s->coef,
+                                                        (double)base, s->acc, fresh);
+    if (g->prb_nseg)
+      k_prb_pull<<<g->grid(g->prb_nseg * 64), kBlock, 0, st>>>(
+          g->prb_nseg, g->prb_seg_row, g->prb_seg_beg, g->prb_seg_len, g->prb_seg_off, g->prb_runs, g->prb_f0,
+          g->prb_midx, s->wm, s->coef, s->win, 64, (double)base, s->acc, s->partial);
+    if (g->prb_nmrows)
+      k_prb_fix<<<g->grid((int64_t)g->prb_nmrows * 64), kBlock, 0, st>>>(g->prb_mrows, g->prb_nm, g->prb_seg_off,
+                                                                       s->partial, (double)base, s->acc);
+    k_prb_colsum<<<kPrbRedBlocks, kBlock, 0, st>>>((int32_t)n, s->acc, s->part);
+    k_prb_colsum2<<<1, 64, 0, st>>>(s->part, kPrbRedBlocks, s->sums);
+    if (last) k_prb_fin<<<g->grid(n), kBlock, 0, st>>>((int32_t)n, last, s->acc, s->sums, s->rank);
+    HIP_TRY(hipGetLastError());
+    *finished |= last;
+    s->running &= ~last;
+  }
+  g->kend();
+  s->done |= *finished;
+  return TGA_OK;
+}
+
+extern "C" int tga_pr_server_result(tga_pr_server* s, int32_t lane, float* rank, uint32_t flags) {
+  if (!s || !s->g || !rank || lane < 0 || lane >= 64 || !(s->done >> lane & 1))
+    return fail(TGA_EINVAL, "tga_pr_server_result: lane has no finished query (or the handle is closed)");
+  return lane_result<float>(s, lane, s->rank, s->stage, rank, flags);
 }
 
 extern "C" int tga_earliest_arrival(tga_graph* g, tga_window w, int32_t src, int32_t* arrival, uint32_t flags) {
@@ -4526,8 +4929,8 @@ extern "C" int tga_graph_coloring_batch(tga_graph* g, const tga_window* windows,
   const int64_t nl = ((int64_t)off.size() - 1) / 2;
   g->kbegin();
   if (g->prb_nmulti)
-    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(g->prb_mlist, g->prb_nmulti, g->prb_runs,
-                                                                       g->coff, g->ct, g->prb_win, nq, g->ccb_mask);
+    k_ccb_mask<<<g->grid((int64_t)g->prb_nmulti * 64), kBlock, 0, st>>>(
+        g->prb_mlist, g->prb_nmulti, g->prb_runs, g->coff, g->ct, g->prb_win, nq, g->ccb_mask, ~0ull);
   k_fill<int32_t><<<g->grid(n * 64), kBlock, 0, st>>>(g->ccb_label, n * 64, 0);
   HIP_TRY(hipGetLastError());
   int32_t* const fl = g->gcb_flags;

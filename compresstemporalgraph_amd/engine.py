@@ -82,6 +82,18 @@ def lib():
         L.tga_bfs_server_result.argtypes = [vp, C.c_int32, vp, C.c_uint32]
         L.tga_bfs_server_close.argtypes = [vp]
         L.tga_bfs_server_close.restype = None
+        L.tga_cc_server_open.argtypes = [vp, C.POINTER(vp)]
+        L.tga_cc_server_admit.argtypes = [vp, Window, p32]
+        L.tga_cc_server_step.argtypes = [vp, pu64]
+        L.tga_cc_server_result.argtypes = [vp, C.c_int32, vp, C.c_uint32]
+        L.tga_cc_server_close.argtypes = [vp]
+        L.tga_cc_server_close.restype = None
+        L.tga_pr_server_open.argtypes = [vp, C.c_float, C.POINTER(vp)]
+        L.tga_pr_server_admit.argtypes = [vp, Window, C.c_int32, p32]
+        L.tga_pr_server_step.argtypes = [vp, pu64]
+        L.tga_pr_server_result.argtypes = [vp, C.c_int32, vp, C.c_uint32]
+        L.tga_pr_server_close.argtypes = [vp]
+        L.tga_pr_server_close.restype = None
         L.tga_earliest_arrival.argtypes = [vp, Window, C.c_int32, vp, C.c_uint32]
         L.tga_latest_departure.argtypes = [vp, Window, C.c_int32, vp, vp, C.c_uint32]
         L.tga_connected_components.argtypes = [vp, Window, vp, C.c_uint32]
@@ -275,6 +287,16 @@ class TemporalGraph:
         """Opens a BFS server (continuous batching of up to 64 queries)."""
         return BfsServer(self)
 
+    def cc_server(self):
+        """Opens a connected-components server (continuous admission of up to
+        64 window queries into the batched label columns)."""
+        return CcServer(self)
+
+    def pagerank_server(self, damping: float = 0.85):
+        """Opens a PageRank server (continuous admission of up to 64 window
+        queries, one iteration of every running query per step)."""
+        return PrServer(self, damping)
+
     def earliest_arrival(self, src: int, window=None, out=None):
         o = _Out(self.n, np.int32, out)
         _check(lib().tga_earliest_arrival(self.h, _win(window), src, o.ptr, o.flags))
@@ -408,38 +430,39 @@ class TemporalGraph:
         return self.graph_coloring(kw.get("window"), kw.get("rounds", 512))
 
 
-class BfsServer:
-    """Continuous batching of BFS queries on one TemporalGraph: admit() puts a
-    query into a free lane, step() advances every running query one level in
-    a single decode pass and returns the lanes that finished, result(lane)
-    returns a finished query's levels (equal to TemporalGraph.bfs) and frees
-    the lane."""
+class _LaneServer:
+    """Continuous batching on one TemporalGraph: admit() puts a query into a
+    free lane, step() advances every running query and returns the lanes
+    that finished, result(lane) returns a finished query's output (equal to
+    the plain call's) and frees the lane."""
 
     LANES = 64
+    _kind = ""
+    _dtype = np.int32
 
-    def __init__(self, graph: "TemporalGraph"):
+    def __init__(self, graph: "TemporalGraph", *open_args):
         self.graph = graph
         self.h = C.c_void_p()
-        _check(lib().tga_bfs_server_open(graph.h, C.byref(self.h)))
+        _check(getattr(lib(), f"tga_{self._kind}_server_open")(graph.h, *open_args, C.byref(self.h)))
 
-    def admit(self, src: int, window=None) -> int:
+    def _admit(self, *args) -> int:
         lane = C.c_int32()
-        _check(lib().tga_bfs_server_admit(self.h, int(src), _win(window), C.byref(lane)))
+        _check(getattr(lib(), f"tga_{self._kind}_server_admit")(self.h, *args, C.byref(lane)))
         return lane.value
 
     def step(self) -> list:
         fin = C.c_uint64()
-        _check(lib().tga_bfs_server_step(self.h, C.byref(fin)))
+        _check(getattr(lib(), f"tga_{self._kind}_server_step")(self.h, C.byref(fin)))
         return [q for q in range(self.LANES) if fin.value >> q & 1]
 
     def result(self, lane: int, out=None):
-        o = _Out(self.graph.n, np.int32, out)
-        _check(lib().tga_bfs_server_result(self.h, int(lane), o.ptr, o.flags))
+        o = _Out(self.graph.n, self._dtype, out)
+        _check(getattr(lib(), f"tga_{self._kind}_server_result")(self.h, int(lane), o.ptr, o.flags))
         return o.arr
 
     def close(self) -> None:
         if self.h:
-            lib().tga_bfs_server_close(self.h)
+            getattr(lib(), f"tga_{self._kind}_server_close")(self.h)
             self.h = C.c_void_p()
 
     def __enter__(self):
@@ -447,3 +470,39 @@ class BfsServer:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class BfsServer(_LaneServer):
+    """BFS queries: each step advances every running query one level in a
+    single decode pass; result() equals TemporalGraph.bfs."""
+
+    _kind = "bfs"
+
+    def admit(self, src: int, window=None) -> int:
+        return self._admit(int(src), _win(window))
+
+
+class CcServer(_LaneServer):
+    """Connected-components queries: each step runs a group of label rounds
+    over the batched run structure; a query finishes at its fixpoint and
+    result() equals TemporalGraph.connected_components."""
+
+    _kind = "cc"
+
+    def admit(self, window=None) -> int:
+        return self._admit(_win(window))
+
+
+class PrServer(_LaneServer):
+    """PageRank queries (the server's damping, each query its iteration
+    count): each step runs one iteration of every running query; result()
+    equals the query's row of TemporalGraph.pagerank_batch."""
+
+    _kind = "pr"
+    _dtype = np.float32
+
+    def __init__(self, graph: "TemporalGraph", damping: float = 0.85):
+        super().__init__(graph, C.c_float(damping))
+
+    def admit(self, window=None, iters: int = 20) -> int:
+        return self._admit(_win(window), int(iters))

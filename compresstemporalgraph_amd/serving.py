@@ -1,18 +1,20 @@
 """Serving form: BFS, CC, PageRank (PR) and degree-centrality (DC) window
-queries arriving over time, served by continuous batching on one GPU.
+queries arriving over time, served by continuous batching on one handle.
 
-Requests are (arrival time, kind, source, window).  The loop admits every BFS
-request that has arrived into a free lane of the BFS server (up to 64 at
-once, fewer when the per-query state would exceed the memory budget),
-advances all running queries by one BFS level per decode pass, and completes
-a query as soon as its frontier empties -- so a query that arrives while
-others are on the GPU joins them at the next level instead of waiting for the
-batch to drain.  DC and PR requests that have arrived are answered together
-by one batched call per kind (up to 64 windows; PR 20 iterations), and so
-are CC requests.  Given more handles of the graph (one per kind, or shared),
-these run in worker threads beside the BFS loop, else between its steps.  The plain baseline serves the
-same arrivals one query at a time (TemporalGraph.bfs, connected_components,
-pagerank, degree_centrality), in arrival order.
+Requests are (arrival time, kind, source, window).  BFS, CC and PR each have a
+query server of 64 lanes: a request that has arrived is admitted into a free
+lane of its kind's server and joins the queries already running there at the
+server's next step -- one BFS level, one group of CC label rounds, or one PR
+iteration (20 per query) -- so it never waits for a batch to drain.  The loop
+steps the servers that have running queries in turn, all on the handle's one
+stream, and completes a query as soon as its server reports it finished.  DC
+(a single pass per query) is answered by one batched call for the DC requests
+that have arrived (up to 64).  Each kind's device state is charged against
+the memory budget when its server opens (the handle's device bytes before
+and after); a kind whose state does not fit is answered by plain calls, one
+per turn of the loop.  The plain baseline serves the same arrivals one query
+at a time (TemporalGraph.bfs, connected_components, pagerank,
+degree_centrality), in arrival order.
 
 Latency is completion time minus arrival time; throughput is completed
 queries over the span from the first arrival to the last completion.
@@ -20,11 +22,12 @@ queries over the span from the first arrival to the last completion.
 from __future__ import annotations
 
 import time
+from collections import deque
 from dataclasses import dataclass
 
 import numpy as np
 
-from .engine import BfsServer, TemporalGraph
+from .engine import TemporalGraph
 
 
 @dataclass
@@ -62,13 +65,6 @@ def mixed_requests(rate: float, count: int, n: int, t_lo: int, t_hi: int, dc_sha
                 r.kind = kind
                 break
     return req
-
-
-def lanes_for_budget(n: int, budget_bytes: int) -> int:
-    """Queries that fit the memory budget: each holds an int32 level row of n
-    entries (the three shared u64 masks are charged once)."""
-    shared = 3 * 8 * n
-    return int(max(1, min(BfsServer.LANES, (budget_bytes - shared) // (4 * n))))
 
 
 def _stats(req, done_at) -> dict:
@@ -150,146 +146,125 @@ def _plain_one(g: TemporalGraph, r, device_out: bool):
 # costs about the same whatever its width (passes over the graph for 64
 # columns), so below these counts the plain calls are cheaper (on a 2^25
 # contact graph a windowed batch costs about 24 plain PR, 9 plain CC or 2
-# plain DC queries).  CC waits for 16: a batch made while the other kinds
-# run takes several times its time alone and holds back every queue, so CC
-# batches only under a real backlog.
+# plain DC queries).  The serving loop batches DC by this rule; BFS, CC and
+# PR go through their servers.
 MIN_BATCH = {"DC": 2, "PR": 16, "CC": 16}
 
-
-def _answer(g: TemporalGraph, req: list, idx: list, device_out: bool) -> list:
-    """Results for the non-BFS requests idx, batched per kind (up to 64
-    each; plain calls below MIN_BATCH); returns [(i, result)]."""
-    out = []
-    for kind, fn in (("DC", _dc_batch), ("PR", _pr_batch), ("CC", _cc_batch)):
-        sel = [i for i in idx if req[i].kind == kind]
-        for b in range(0, len(sel), 64):
-            part = sel[b:b + 64]
-            if len(part) >= MIN_BATCH[kind]:
-                out += list(zip(part, fn(g, req, part, device_out)))
-            else:
-                out += [(i, _plain_one(g, req[i], device_out)) for i in part]
-    return out
+SERVED = ("BFS", "CC", "PR")  # kinds with a query server
 
 
-class _BatchWorker:
-    """Answers the non-BFS requests on a second handle of the same graph, in
-    a thread of its own: each wake-up takes every pending request (DC and PR
-    batched per kind, CC one at a time).  The handle has its own stream, so
-    this work overlaps the BFS levels of the main loop on the device (the
-    engine's calls release the interpreter lock while they wait for the
-    GPU)."""
+def _open_server(g: TemporalGraph, kind: str):
+    if kind == "BFS":
+        return g.bfs_server()
+    if kind == "CC":
+        return g.cc_server()
+    return g.pagerank_server(damping=0.85)
 
-    def __init__(self, g: TemporalGraph, req: list, device_out: bool, t0: float, done_at: list, keep):
-        import threading
-        self.g, self.req, self.device_out, self.t0 = g, req, device_out, t0
-        self.done_at, self.keep = done_at, keep
-        self.cv = threading.Condition()
-        self.pending, self.open, self.outstanding, self.error = [], True, 0, None
-        self.thread = threading.Thread(target=self._run, daemon=True)
-        self.thread.start()
 
-    def put(self, i: int) -> None:
-        with self.cv:
-            self.pending.append(i)
-            self.outstanding += 1
-            self.cv.notify()
+def _admit(srv, kind: str, r) -> int:
+    if kind == "BFS":
+        return srv.admit(r.src, r.window)
+    if kind == "CC":
+        return srv.admit(r.window)
+    return srv.admit(r.window, iters=PR_ITERS)
 
-    def busy(self) -> bool:
-        with self.cv:
-            if self.error is not None:
-                raise self.error
-            return self.outstanding > 0
 
-    def close(self) -> None:
-        with self.cv:
-            self.open = False
-            self.cv.notify()
-        self.thread.join()
-
-    def _run(self) -> None:
-        while True:
-            with self.cv:
-                while self.open and not self.pending:
-                    self.cv.wait()
-                if not self.pending:
-                    return
-                idx, self.pending = self.pending, []
-            try:
-                res = _answer(self.g, self.req, idx, self.device_out)
-            except Exception as e:  # surfaced by busy()
-                with self.cv:
-                    self.error, self.outstanding = e, 0
-                return
-            t = time.perf_counter() - self.t0
-            with self.cv:
-                for i, row in res:
-                    self.done_at[i] = t
-                    if self.keep is not None:
-                        self.keep[i] = row
-                self.outstanding -= len(idx)
+def open_servers(g: TemporalGraph, kinds, budget_bytes: int):
+    """Opens the servers of `kinds` (in SERVED order) while their device state
+    fits the budget; returns ({kind: server}, {kind: bytes charged}).  A
+    server's charge is the handle's device bytes after its opening minus
+    before (its lanes, plus the batched run structure CC and PR share when the
+    handle has not built it yet, charged to whichever opens first; it stays
+    with the handle); one that does not fit is closed again and its kind is
+    left to plain calls.  DC is charged its widest batch: count and
+    centrality rows of 64 queries (fewer when the rest of the budget is
+    smaller)."""
+    servers, charged = {}, {}
+    left = budget_bytes
+    for kind in SERVED:
+        if kind not in kinds:
+            continue
+        before = g.device_bytes()
+        srv = _open_server(g, kind)
+        cost = g.device_bytes() - before
+        if cost > left:
+            srv.close()
+            continue
+        servers[kind], charged[kind] = srv, cost
+        left -= cost
+    if "DC" in kinds:
+        rows = int(max(1, min(64, left // (8 * g.n))))
+        charged["DC"] = rows * 8 * g.n
+    return servers, charged
 
 
 def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep=None,
-                  device_out: bool = False, workers: dict = None) -> dict:
-    """Continuous batching.  `keep` (a dict) receives the results per request
-    index when given (for identity checks).  `workers` maps the kinds "DC",
-    "PR" and "CC" to other handles of the same graph: each distinct handle
-    gets a worker thread that answers its kinds beside the BFS loop (the
-    handles' streams overlap on the device); kinds without one are answered
-    between the BFS steps."""
-    lanes = lanes_for_budget(g.n, budget_bytes)
+                  device_out: bool = False) -> dict:
+    """Continuous batching on one handle.  `keep` (a dict) receives the
+    results per request index when given (for identity checks)."""
     sink = _sink(g, device_out)
+    pr_sink = sink
+    if device_out:
+        import torch
+        pr_sink = lambda: torch.empty(g.n, dtype=torch.float32, device=f"cuda:{g.device}")
     done_at = [0.0] * len(req)
     order = sorted(range(len(req)), key=lambda i: req[i].arrival)
-    with g.bfs_server() as srv:
-        lane_of, dc_wait = {}, []
-        nxt = 0
-        t0 = time.perf_counter()
-        by_handle = {}
-        for kind, h in (workers or {}).items():
-            if id(h) not in by_handle:
-                by_handle[id(h)] = _BatchWorker(h, req, device_out, t0, done_at, keep)
-        worker_of = {kind: by_handle[id(h)] for kind, h in (workers or {}).items()}
-        pool = list(by_handle.values())
-        try:
-            while len(done_at) and (nxt < len(order) or lane_of or dc_wait or any(w.busy() for w in pool)):
-                now = time.perf_counter() - t0
-                while nxt < len(order) and req[order[nxt]].arrival <= now:
-                    i = order[nxt]
-                    if req[i].kind != "BFS":
-                        if req[i].kind in worker_of:
-                            worker_of[req[i].kind].put(i)
-                        else:
-                            dc_wait.append(i)
-                    elif len(lane_of) < lanes:
-                        lane_of[srv.admit(req[i].src, req[i].window)] = i
-                    else:
-                        break  # BFS lanes full: this request waits (arrival order kept)
-                    nxt += 1
-                if not lane_of and not dc_wait:  # idle until the next arrival (or the workers)
-                    wake = req[order[nxt]].arrival - now if nxt < len(order) else 1e-4
-                    time.sleep(max(0.0, min(wake, 1e-3) if pool else wake))
+    kinds = {r.kind for r in req}
+    servers, charged = open_servers(g, kinds, budget_bytes)
+    dc_rows = charged.get("DC", 8 * g.n * 64) // (8 * g.n)
+    queue = {k: deque() for k in ("BFS", "CC", "PR", "DC")}
+    lane_of = {k: {} for k in servers}
+    left = len(req)
+    nxt = 0
+    t0 = time.perf_counter()
+
+    def finish(i, res):
+        nonlocal left
+        done_at[i] = time.perf_counter() - t0
+        left -= 1
+        if keep is not None:
+            keep[i] = res
+
+    try:
+        while left:
+            now = time.perf_counter() - t0
+            while nxt < len(order) and req[order[nxt]].arrival <= now:
+                queue[req[order[nxt]].kind].append(order[nxt])
+                nxt += 1
+            for kind, srv in servers.items():  # admission in arrival order per kind
+                q, lanes = queue[kind], lane_of[kind]
+                while q and len(lanes) < srv.LANES:
+                    i = q.popleft()
+                    lanes[_admit(srv, kind, req[i])] = i
+            worked = False
+            if queue["DC"]:
+                idx = [queue["DC"].popleft() for _ in range(min(dc_rows, len(queue["DC"])))]
+                res = (_dc_batch(g, req, idx, device_out) if len(idx) >= MIN_BATCH["DC"]
+                       else [_plain_one(g, req[i], device_out) for i in idx])
+                for i, row in zip(idx, res):
+                    finish(i, row)
+                worked = True
+            for kind in SERVED:  # kinds left to plain calls: one per turn
+                if kind not in servers and queue[kind]:
+                    i = queue[kind].popleft()
+                    finish(i, _plain_one(g, req[i], device_out))
+                    worked = True
+            for kind, srv in servers.items():
+                lanes = lane_of[kind]
+                if not lanes:
                     continue
-                if dc_wait:
-                    idx, dc_wait = dc_wait, []
-                    res = _answer(g, req, idx, device_out)
-                    t = time.perf_counter() - t0
-                    for i, row in res:
-                        done_at[i] = t
-                        if keep is not None:
-                            keep[i] = row
-                if lane_of:
-                    for lane in srv.step():
-                        i = lane_of.pop(lane)
-                        levels = srv.result(lane, out=sink())
-                        done_at[i] = time.perf_counter() - t0
-                        if keep is not None:
-                            keep[i] = levels
-        finally:
-            for w in pool:
-                w.close()
+                worked = True
+                for lane in srv.step():
+                    i = lanes.pop(lane)
+                    finish(i, srv.result(lane, out=(pr_sink if kind == "PR" else sink)()))
+            if not worked and left and nxt < len(order):  # idle until the next arrival
+                time.sleep(max(0.0, req[order[nxt]].arrival - (time.perf_counter() - t0)))
+    finally:
+        for srv in servers.values():
+            srv.close()
     out = _stats(req, done_at)
-    out["lanes"] = lanes
+    out["lanes"] = {k: srv.LANES for k, srv in servers.items()}
+    out["state_bytes"] = charged
     return out
 
 

@@ -170,6 +170,30 @@ int tga_bfs_server_step(tga_bfs_server* s, uint64_t* finished);
 int tga_bfs_server_result(tga_bfs_server* s, int32_t lane, int32_t* level, uint32_t flags);
 void tga_bfs_server_close(tga_bfs_server* s);
 
+/* CC and PageRank servers: continuous admission into the 64 columns of
+ * tga_connected_components_batch / tga_pagerank_batch (no counterpart in the
+ * reference).  admit / step / result / close work as for the BFS server.  A
+ * CC step runs a group of label rounds and finishes the lanes that reached
+ * their fixpoint; a PR step runs one iteration of every running lane (each
+ * with the iteration count it was admitted with; damping is the server's)
+ * and finishes the lanes that reached their count, without waiting for the
+ * device.  Results equal tga_connected_components / tga_pagerank_batch's for
+ * the same window, whatever ran beside them.  Server state is allocated on
+ * the handle (tga_info's device bytes include it); close servers before
+ * their handle (after tga_close their calls fail with TGA_EINVAL). */
+typedef struct tga_cc_server tga_cc_server;
+int tga_cc_server_open(tga_graph* g, tga_cc_server** out);
+int tga_cc_server_admit(tga_cc_server* s, tga_window w, int32_t* lane);
+int tga_cc_server_step(tga_cc_server* s, uint64_t* finished);
+int tga_cc_server_result(tga_cc_server* s, int32_t lane, int32_t* label, uint32_t flags);
+void tga_cc_server_close(tga_cc_server* s);
+typedef struct tga_pr_server tga_pr_server;
+int tga_pr_server_open(tga_graph* g, float damping, tga_pr_server** out);
+int tga_pr_server_admit(tga_pr_server* s, tga_window w, int32_t iters, int32_t* lane);
+int tga_pr_server_step(tga_pr_server* s, uint64_t* finished);
+int tga_pr_server_result(tga_pr_server* s, int32_t lane, float* rank, uint32_t flags);
+void tga_pr_server_close(tga_pr_server* s);
+
 /* Earliest arrival (the reference GPU BFS rule, bfs.cu:1452-1471): arrival[src]
  * = max(ws, 0); the first window contact (x, r, t) of each pair relaxes r when
  * arrival[x] <= t < arrival[r]; INT32_MAX when unreached. */

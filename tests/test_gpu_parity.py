@@ -181,6 +181,77 @@ def test_bfs_server_mixed_admission(pair):
         np.testing.assert_array_equal(got[i], o.bfs(s, window=w), err_msg=f"query {i}")
 
 
+def _server_queries(ds, count, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = int(ds.t.min()), int(ds.t.max())
+    wins = []
+    for q in range(count):
+        k = q % 4
+        if k == 0:
+            wins.append(None)
+        elif k == 1:
+            a, b = sorted(int(v) for v in rng.integers(lo, hi + 1, 2))
+            wins.append((a, b))
+        elif k == 2:
+            t = int(ds.t[rng.integers(0, len(ds.t))])
+            wins.append((t, t))
+        else:
+            wins.append((hi + 1, hi + 10))
+    return rng, wins
+
+
+def _drive(srv, queries, rng, admit):
+    """Admits a few queries before each step (more queries than lanes over
+    the run, so lanes are reused) and collects every finished lane."""
+    lane_of, nxt, got = {}, 0, {}
+    while nxt < len(queries) or lane_of:
+        for _ in range(int(rng.integers(0, 6))):
+            if nxt < len(queries) and len(lane_of) < 64:
+                lane_of[admit(srv, queries[nxt])] = nxt
+                nxt += 1
+        if not lane_of:
+            continue
+        for lane in srv.step():
+            got[lane_of.pop(lane)] = srv.result(lane)
+    assert len(got) == len(queries)
+    return got
+
+
+def test_cc_server_mixed_admission(pair):
+    """CC queries admitted at different steps into reused lanes: each result
+    equals the oracle's labels for its window alone."""
+    ds, g = pair
+    rng, wins = _server_queries(ds, 90, 31)
+    with g.cc_server() as srv:
+        got = _drive(srv, wins, rng, lambda s, w: s.admit(w))
+    o = ds.oracle()
+    for i, w in enumerate(wins):
+        np.testing.assert_array_equal(got[i], o.cc(window=w), err_msg=f"query {i}")
+
+
+def test_pagerank_server_mixed_admission(pair):
+    """PageRank queries with their own iteration counts (0 to 20) admitted at
+    different steps: each result is bit-identical to its row of
+    pagerank_batch (the same per-column arithmetic) and within the PageRank
+    bar of the oracle."""
+    ds, g = pair
+    rng, wins = _server_queries(ds, 90, 37)
+    iters = [int(v) for v in rng.choice([0, 1, 3, 20], len(wins))]
+    with g.pagerank_server(damping=0.85) as srv:
+        got = _drive(srv, list(zip(wins, iters)), rng, lambda s, q: s.admit(q[0], iters=q[1]))
+    for it in sorted(set(iters)):
+        idx = [i for i in range(len(wins)) if iters[i] == it]
+        for b in range(0, len(idx), 64):
+            part = idx[b:b + 64]
+            ref = g.pagerank_batch([wins[i] for i in part], iters=it)
+            for j, i in enumerate(part):
+                np.testing.assert_array_equal(got[i], ref[j], err_msg=f"query {i} iters {it}")
+    o = ds.oracle()
+    for i in range(0, len(wins), 7):
+        ref32, _ = o.pagerank(iters[i], damping=0.85, window=wins[i], dtype=np.float32)
+        assert rel_linf(got[i], ref32) <= REL_TOL, f"query {i}"
+
+
 def test_earliest_arrival(pair):
     ds, g = pair
     for w in windows_for(ds)[:2]:
@@ -353,12 +424,13 @@ def test_hits(pair, path, monkeypatch):
         assert rel_linf(auth, a32) <= REL_TOL
 
 
-@pytest.mark.parametrize("worker", [False, True])
-def test_served_mix(pair, worker):
-    """A BFS / CC / PR / DC request stream through the serving loop (BFS in
-    server lanes, DC, PR and CC in batched calls; between the BFS steps or on
-    a second handle in a worker thread): every result equals the
-    oracle's answer to that query alone (PageRank within its bar)."""
+@pytest.mark.parametrize("budget", ["full", "bfs_only"])
+def test_served_mix(pair, budget):
+    """A BFS / CC / PR / DC request stream through the serving loop (BFS, CC
+    and PR admitted into their servers' lanes, DC in batched calls; with a
+    budget that holds only the BFS server, CC and PR by plain calls): every
+    result equals the oracle's answer to that query alone (PageRank within
+    its bar)."""
     from compresstemporalgraph_amd import serving
     ds, g = pair
     lo, hi = int(ds.t.min()), int(ds.t.max())
@@ -368,12 +440,14 @@ def test_served_mix(pair, worker):
         r.src = int(ds.src[r.src % len(ds.src)])
         r.window = (r.window[0], min(r.window[1], hi))
     keep = {}
-    g2 = TemporalGraph(ds.dir, ds.name) if worker else None
-    try:
-        serving.serve_batched(g, req, keep=keep, workers={"DC": g2, "PR": g2, "CC": g2} if g2 else None)
-    finally:
-        if g2 is not None:
-            g2.close()
+    g.connected_components_batch([None])  # the run structure: handle state, built once
+    before = g.device_bytes()
+    with g.bfs_server():
+        bfs_bytes = g.device_bytes() - before
+    st = serving.serve_batched(g, req, keep=keep,
+                               budget_bytes=(8 << 30) if budget == "full" else bfs_bytes + 8 * ds.n * 4)
+    assert set(st["lanes"]) == ({"BFS", "CC", "PR"} if budget == "full" else {"BFS"})
+    assert g.device_bytes() == before  # the servers' state is freed at the end
     o = ds.oracle()
     assert sorted(keep) == list(range(len(req)))
     for i, r in enumerate(req):

@@ -145,6 +145,14 @@ def test_entry_points_reject_null_handles():
         "tga_bfs_server_admit": lambda: L.tga_bfs_server_admit(None, 0, w, C.byref(i32)),
         "tga_bfs_server_step": lambda: L.tga_bfs_server_step(None, C.byref(u64)),
         "tga_bfs_server_result": lambda: L.tga_bfs_server_result(None, 0, buf, 0),
+        "tga_cc_server_open": lambda: L.tga_cc_server_open(None, C.byref(C.c_void_p())),
+        "tga_cc_server_admit": lambda: L.tga_cc_server_admit(None, w, C.byref(i32)),
+        "tga_cc_server_step": lambda: L.tga_cc_server_step(None, C.byref(u64)),
+        "tga_cc_server_result": lambda: L.tga_cc_server_result(None, 0, buf, 0),
+        "tga_pr_server_open": lambda: L.tga_pr_server_open(None, 0.85, C.byref(C.c_void_p())),
+        "tga_pr_server_admit": lambda: L.tga_pr_server_admit(None, w, 1, C.byref(i32)),
+        "tga_pr_server_step": lambda: L.tga_pr_server_step(None, C.byref(u64)),
+        "tga_pr_server_result": lambda: L.tga_pr_server_result(None, 0, fbuf, 0),
         "tga_earliest_arrival": lambda: L.tga_earliest_arrival(None, w, 0, buf, 0),
         "tga_latest_departure": lambda: L.tga_latest_departure(None, w, 0, buf, None, 0),
         "tga_connected_components": lambda: L.tga_connected_components(None, w, buf, 0),

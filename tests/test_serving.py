@@ -33,14 +33,6 @@ def test_mixed_requests_share():
     assert abs(kinds.count("DC") / len(kinds) - 0.25) < 0.03
 
 
-def test_lanes_for_budget():
-    n = 1 << 20
-    shared = 3 * 8 * n
-    assert serving.lanes_for_budget(n, 8 << 30) == 64
-    assert serving.lanes_for_budget(n, shared + 10 * 4 * n + 5) == 10
-    assert serving.lanes_for_budget(n, 0) == 1
-
-
 def test_stats_percentiles_and_kinds():
     req = [serving.Request(arrival=float(i), src=0, window=(0, 1), kind="DC" if i % 4 == 0 else "BFS")
            for i in range(100)]
@@ -56,21 +48,36 @@ def test_stats_percentiles_and_kinds():
 
 
 class _FakeServer:
-    """Finishes the query admitted to a lane after `src % 3 + 1` steps."""
+    """Finishes the query admitted to a lane after `key % 3 + 1` steps (key:
+    the BFS source, else the window start)."""
 
     LANES = 64
 
-    def __init__(self, log):
-        self.left, self.src, self.log = {}, {}, log
+    def __init__(self, kind, log, graph, cost):
+        self.kind, self.log, self.graph, self.cost = kind, log, graph, cost
+        self.left, self.query = {}, {}
+        graph.bytes += cost
+        self.closed = False
 
-    def admit(self, src, window):
-        lane = min(set(range(self.LANES)) - set(self.left))
-        self.left[lane], self.src[lane] = src % 3 + 1, (src, window)
-        self.log.append(("admit", src))
+    def _take(self, key, query):
+        lane = min(set(range(self.LANES)) - set(self.left) - set(self.query))
+        self.left[lane], self.query[lane] = key % 3 + 1, query
+        self.log.append(("admit", self.kind, query))
         return lane
+
+    def admit(self, *args, **kw):
+        if self.kind == "BFS":
+            src, window = args
+            return self._take(src, ("bfs", src, window))
+        window = args[0]
+        if self.kind == "PR":
+            assert kw["iters"] == serving.PR_ITERS
+            return self._take(window[0], ("pr", window))
+        return self._take(window[0], ("cc", window))
 
     def step(self):
         assert self.left, "step with no running query"
+        self.log.append(("step", self.kind))
         fin = []
         for lane in list(self.left):
             self.left[lane] -= 1
@@ -80,24 +87,42 @@ class _FakeServer:
         return fin
 
     def result(self, lane, out=None):
-        return ("bfs",) + self.src.pop(lane)
+        assert lane not in self.left
+        return self.query.pop(lane)
 
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        pass
+    def close(self):
+        assert not self.closed
+        self.closed = True
+        self.graph.bytes -= self.cost
 
 
 class _FakeGraph:
     n = 10
     device = 0
+    COST = {"BFS": 300, "CC": 500, "PR": 900}
 
     def __init__(self):
-        self.log, self.dc_calls, self.dc_plain = [], [], 0
+        self.log, self.dc_calls, self.dc_plain, self.plain = [], [], 0, []
+        self.bytes = 1000  # the handle's own state
+        self.servers = []
+
+    def device_bytes(self):
+        return self.bytes
+
+    def _server(self, kind):
+        s = _FakeServer(kind, self.log, self, self.COST[kind])
+        self.servers.append(s)
+        return s
 
     def bfs_server(self):
-        return _FakeServer(self.log)
+        return self._server("BFS")
+
+    def cc_server(self):
+        return self._server("CC")
+
+    def pagerank_server(self, damping=0.85):
+        assert damping == 0.85
+        return self._server("PR")
 
     def degree_centrality_batch(self, wins, out=None, out_centrality=None):
         assert 1 <= len(wins) <= 64
@@ -105,32 +130,30 @@ class _FakeGraph:
         return [("cnt", w) for w in wins], [("cen", w) for w in wins]
 
     def bfs(self, src, window=None, out=None):
+        self.plain.append("BFS")
         return ("bfs", src, window)
 
     def degree_centrality(self, window=None, centrality=False, out=None, out_centrality=None):
         self.dc_plain += 1
         return ("cnt", window), ("cen", window)
 
-    def pagerank_batch(self, wins, iters=20, out=None):
-        assert 1 <= len(wins) <= 64 and iters == serving.PR_ITERS
-        self.pr_calls = getattr(self, "pr_calls", []) + [list(wins)]
-        return [("pr", w) for w in wins]
-
     def pagerank(self, window=None, iters=20, out=None):
+        self.plain.append("PR")
         return ("pr", window)
 
     def connected_components(self, window=None, out=None):
+        self.plain.append("CC")
         return ("cc", window)
 
-    def connected_components_batch(self, wins, out=None):
-        assert serving.MIN_BATCH["CC"] <= len(wins) <= 64
-        return [("cc", w) for w in wins]
+
+WANT = {"BFS": lambda r: ("bfs", r.src, r.window), "CC": lambda r: ("cc", r.window),
+        "PR": lambda r: ("pr", r.window), "DC": lambda r: (("cnt", r.window), ("cen", r.window))}
 
 
 def test_serve_loop_bookkeeping():
     """Every request is answered exactly once with its own result, in both
     loops; DC requests go through batched calls of at most 64; BFS requests
-    are admitted in arrival order."""
+    are admitted in arrival order; the server is closed at the end."""
     req = serving.mixed_requests(rate=1e6, count=300, n=1000, t_lo=0, t_hi=1 << 20, dc_share=0.3, seed=2)
     g = _FakeGraph()
     kb, kp = {}, {}
@@ -138,35 +161,17 @@ def test_serve_loop_bookkeeping():
     sp = serving.serve_plain(g, req, keep=kp)
     assert sorted(kb) == list(range(len(req))) == sorted(kp)
     for i, r in enumerate(req):
-        if r.kind == "DC":
-            assert kb[i] == (("cnt", r.window), ("cen", r.window)) == kp[i]
-        else:
-            assert kb[i] == ("bfs", r.src, r.window) == kp[i]
+        assert kb[i] == WANT[r.kind](r) == kp[i]
     # every DC request went through a batched call (of 2..64) or, alone, a
     # plain one (serve_plain's calls are on the same stand-in: subtract them)
     ndc = sum(r.kind == "DC" for r in req)
     assert sum(len(c) for c in g.dc_calls) + g.dc_plain - ndc == ndc
     assert all(serving.MIN_BATCH["DC"] <= len(c) <= 64 for c in g.dc_calls)
     order = sorted((r.arrival, r.src) for r in req if r.kind == "BFS")
-    assert [s for k, s in g.log if k == "admit"] == [s for _, s in order]
-    assert sb["queries"] == sp["queries"] == len(req) and sb["lanes"] == 64
-
-
-def test_serve_loop_dc_worker():
-    """The same stream with DC batches on a second handle in a worker thread:
-    every request answered once with its own result, DC calls only on the
-    second handle, at most 64 windows each."""
-    req = serving.mixed_requests(rate=1e5, count=300, n=1000, t_lo=0, t_hi=1 << 20, dc_share=0.3, seed=5)
-    g, g2 = _FakeGraph(), _FakeGraph()
-    kb = {}
-    st = serving.serve_batched(g, req, keep=kb, workers={"DC": g2})
-    assert sorted(kb) == list(range(len(req))) and st["queries"] == len(req)
-    for i, r in enumerate(req):
-        want = (("cnt", r.window), ("cen", r.window)) if r.kind == "DC" else ("bfs", r.src, r.window)
-        assert kb[i] == want
-    assert not g.dc_calls and not g.dc_plain
-    assert sum(len(c) for c in g2.dc_calls) + g2.dc_plain == sum(r.kind == "DC" for r in req)
-    assert all(serving.MIN_BATCH["DC"] <= len(c) <= 64 for c in g2.dc_calls)
+    assert [q[1] for k, kind, q in (e for e in g.log if e[0] == "admit")] == [s for _, s in order]
+    assert sb["queries"] == sp["queries"] == len(req) and sb["lanes"] == {"BFS": 64}
+    assert [s.kind for s in g.servers] == ["BFS"] and all(s.closed for s in g.servers)
+    assert sb["state_bytes"]["BFS"] == 300 and g.bytes == 1000
 
 
 def test_mixed_requests_four_kinds():
@@ -178,20 +183,40 @@ def test_mixed_requests_four_kinds():
 
 
 def test_serve_loop_all_kinds():
-    """BFS, CC, PR and DC requests, inline and with the worker: each answered
-    once with its own result; PR and DC in batched calls of at most 64."""
+    """BFS, CC, PR and DC requests: each answered once with its own result;
+    BFS, CC and PR through their servers (admitted in arrival order per kind,
+    never more than 64 running), DC in batched calls of at most 64."""
     req = serving.mixed_requests(rate=1e5, count=400, n=1000, t_lo=0, t_hi=1 << 20, seed=6,
                                  shares={"DC": 0.1, "PR": 0.3, "CC": 0.2})
-    want = {"BFS": lambda r: ("bfs", r.src, r.window), "CC": lambda r: ("cc", r.window),
-            "PR": lambda r: ("pr", r.window), "DC": lambda r: (("cnt", r.window), ("cen", r.window))}
-    for workers in (0, 1, 3):
-        g, g2, g3, g4 = _FakeGraph(), _FakeGraph(), _FakeGraph(), _FakeGraph()
-        kb, kp = {}, {}
-        wmap = {0: None, 1: {"DC": g2, "PR": g2, "CC": g2}, 3: {"DC": g2, "PR": g3, "CC": g4}}[workers]
-        serving.serve_batched(g, req, keep=kb, workers=wmap)
-        serving.serve_plain(g, req, keep=kp)
-        for i, r in enumerate(req):
-            assert kb[i] == want[r.kind](r) == kp[i], (workers, i, r.kind)
-        gb = {0: g, 1: g2, 3: g3}[workers]
-        calls = getattr(gb, "pr_calls", [])
-        assert all(serving.MIN_BATCH["PR"] <= len(c) <= 64 for c in calls)
+    g = _FakeGraph()
+    kb, kp = {}, {}
+    st = serving.serve_batched(g, req, keep=kb)
+    serving.serve_plain(g, req, keep=kp)
+    for i, r in enumerate(req):
+        assert kb[i] == WANT[r.kind](r) == kp[i], (i, r.kind)
+    assert sorted(s.kind for s in g.servers) == ["BFS", "CC", "PR"]
+    assert st["state_bytes"] == {"BFS": 300, "CC": 500, "PR": 900, "DC": 64 * 8 * g.n}
+    for kind in ("BFS", "CC", "PR"):
+        admitted = [e[2] for e in g.log if e[0] == "admit" and e[1] == kind]
+        assert admitted == [WANT[kind](r) for r in sorted(req, key=lambda r: r.arrival) if r.kind == kind]
+    # the batched loop made no plain call of a served kind
+    assert len(g.plain) == sum(r.kind != "DC" for r in req)
+
+
+def test_serve_loop_budget_fallback():
+    """A budget that holds the BFS server but not the CC or PR state: those
+    kinds are answered by plain calls (their servers are opened, measured and
+    closed again), and DC batches shrink to what is left."""
+    req = serving.mixed_requests(rate=1e5, count=200, n=1000, t_lo=0, t_hi=1 << 20, seed=3,
+                                 shares={"DC": 0.2, "PR": 0.2, "CC": 0.2})
+    g = _FakeGraph()
+    kb = {}
+    budget = 300 + 5 * 8 * g.n + 50  # BFS, then 450 left: neither CC (500) nor PR (900)
+    st = serving.serve_batched(g, req, keep=kb, budget_bytes=budget)
+    for i, r in enumerate(req):
+        assert kb[i] == WANT[r.kind](r), (i, r.kind)
+    assert st["lanes"] == {"BFS": 64}
+    assert st["state_bytes"]["DC"] == 5 * 8 * g.n
+    assert all(len(c) <= 5 for c in g.dc_calls)
+    assert sorted(g.plain) == sorted(r.kind for r in req if r.kind in ("CC", "PR"))
+    assert all(s.closed for s in g.servers) and g.bytes == 1000
