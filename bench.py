@@ -716,20 +716,14 @@ def main() -> int:
         datasets.ensure(root, legs_on)
         # The query legs keep results in torch device tensors.  torch brings
         # its own HIP runtime (which must initialise before the engine's in a
-        # process, and whose streams share the hardware queues), so they run
-        # in a child process, before this process touches the GPU: a live
-        # context here would hold hardware queues the served mix needs.
+        # process), so they run in a child process, before this process
+        # touches the GPU.  The served mix runs on one handle (one stream),
+        # so HIP's default of four hardware queues is enough.
         import subprocess
         cmd = [sys.executable, os.path.abspath(__file__), "--query-legs", "--workload", legs_on]
         if args.data_root:
             cmd += ["--data-root", args.data_root]
-        # Sixteen hardware queues: the served mix runs its batches on three
-        # more handles beside the BFS loop, and with HIP's default of four the
-        # handles' streams share queues, so a long kernel of one stalls the
-        # launches queued behind it on another.
-        # Set, not defaulted: the environment may already carry HIP's default
-        # of four.
-        env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
+        env = dict(os.environ)
         child = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, env=env)
         if child.returncode != 0:
             raise SystemExit(f"query legs failed (exit {child.returncode})")

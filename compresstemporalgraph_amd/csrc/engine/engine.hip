@@ -4278,7 +4278,8 @@ struct tga_cc_server : LaneServer {
   tga_graph::Graph gr;
   using LaneServer::LaneServer;
   ~tga_cc_server() override {
-    if (g && gr.exec) (void)hipGraphExecDestroy(gr.exec);
+    if (g) (void)hipStreamSynchronize(g->stream);
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
   }
 };
 
@@ -4452,8 +4453,9 @@ extern "C" int tga_pr_server_admit(tga_pr_server* s, tga_window w, int32_t iters
 
 // One step: admits the queued queries (window degrees, multi-contact
 // weights, initial ranks), then runs one iteration for every running lane;
-// lanes that reach their iteration count keep their ranks and finish.  The
-// step only enqueues work (the iteration counts are known on the host).
+// lanes that reach their iteration count keep their ranks and finish (the
+// counts are known on the host: the device is waited for only so that a
+// step's time is its own, as for the other servers).
 extern "C" int tga_pr_server_step(tga_pr_server* s, uint64_t* finished) {
   if (!s || !s->g || !finished) return fail(TGA_EINVAL, "tga_pr_server_step: NULL argument (or the handle is closed)");
   tga_graph* g = s->g;
@@ -4513,6 +4515,7 @@ extern "C" int tga_pr_server_step(tga_pr_server* s, uint64_t* finished) {
     s->running &= ~last;
   }
   g->kend();
+  HIP_TRY(hipStreamSynchronize(st));
   s->done |= *finished;
   return TGA_OK;
 }

@@ -5,11 +5,11 @@ Requests are (arrival time, kind, source, window).  BFS, CC and PR each have a
 query server of 64 lanes: a request that has arrived is admitted into a free
 lane of its kind's server and joins the queries already running there at the
 server's next step -- one BFS level, one group of CC label rounds, or one PR
-iteration (20 per query) -- so it never waits for a batch to drain.  The loop
-steps the servers that have running queries in turn, all on the handle's one
-stream, and completes a query as soon as its server reports it finished.  DC
-(a single pass per query) is answered by one batched call for the DC requests
-that have arrived (up to 64).  Each kind's device state is charged against
+iteration (20 per query) -- so it never waits for a batch to drain.  DC (a
+single pass per query) is answered by batched calls of the DC requests that
+have arrived (up to 64).  The kinds share the handle's one stream by weighted
+fair queuing (serve_batched), and a query completes as soon as its server
+reports it finished.  Each kind's device state is charged against
 the memory budget when its server opens (the handle's device bytes before
 and after); a kind whose state does not fit is answered by plain calls, one
 per turn of the loop.  The plain baseline serves the same arrivals one query
@@ -198,10 +198,26 @@ def open_servers(g: TemporalGraph, kinds, budget_bytes: int):
     return servers, charged
 
 
+# An idle CC or PR server starts once this many queries of its kind wait:
+# its step costs about the same at any lane count, so fewer are cheaper as
+# plain calls.  A running server takes every arrival of its kind.  The BFS
+# server's step costs what its frontiers hold, so it always serves.
+SERVE_FROM = {"BFS": 1, "CC": 8, "PR": 16}
+
+
 def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep=None,
                   device_out: bool = False) -> dict:
     """Continuous batching on one handle.  `keep` (a dict) receives the
-    results per request index when given (for identity checks)."""
+    results per request index when given (for identity checks).
+
+    The kinds with work share the GPU by weighted fair queuing: each turn
+    runs one action (a server step, a DC batch, or one plain call) of the
+    kind whose virtual time is least, and charges it the action's measured
+    time divided by the number of queries the action advanced -- so every
+    query in the system gets an equal share of device time, and a kind of
+    cheap steps (BFS levels) is not held to the pace of costly ones (PR
+    iterations, CC round groups).  A kind that becomes busy again starts at
+    the virtual time of the last action, with no credit for its idle time."""
     sink = _sink(g, device_out)
     pr_sink = sink
     if device_out:
@@ -214,6 +230,9 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
     dc_rows = charged.get("DC", 8 * g.n * 64) // (8 * g.n)
     queue = {k: deque() for k in ("BFS", "CC", "PR", "DC")}
     lane_of = {k: {} for k in servers}
+    vt = {k: 0.0 for k in queue}
+    vnow = 0.0
+    busy = set()
     left = len(req)
     nxt = 0
     t0 = time.perf_counter()
@@ -225,6 +244,25 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
         if keep is not None:
             keep[i] = res
 
+    def act(kind) -> int:
+        """One action of `kind`; returns the number of queries it advanced."""
+        if kind in servers and lane_of[kind]:
+            srv, lanes = servers[kind], lane_of[kind]
+            n = len(lanes)
+            for lane in srv.step():
+                i = lanes.pop(lane)
+                finish(i, srv.result(lane, out=(pr_sink if kind == "PR" else sink)()))
+            return n
+        q = queue[kind]
+        if kind == "DC" and len(q) >= MIN_BATCH["DC"]:
+            idx = [q.popleft() for _ in range(min(dc_rows, len(q)))]
+            for i, row in zip(idx, _dc_batch(g, req, idx, device_out)):
+                finish(i, row)
+            return len(idx)
+        i = q.popleft()
+        finish(i, _plain_one(g, req[i], device_out))
+        return 1
+
     try:
         while left:
             now = time.perf_counter() - t0
@@ -233,32 +271,24 @@ def serve_batched(g: TemporalGraph, req: list, budget_bytes: int = 8 << 30, keep
                 nxt += 1
             for kind, srv in servers.items():  # admission in arrival order per kind
                 q, lanes = queue[kind], lane_of[kind]
+                if not lanes and len(q) < SERVE_FROM[kind]:
+                    continue  # idle server, short queue: plain calls
                 while q and len(lanes) < srv.LANES:
                     i = q.popleft()
                     lanes[_admit(srv, kind, req[i])] = i
-            worked = False
-            if queue["DC"]:
-                idx = [queue["DC"].popleft() for _ in range(min(dc_rows, len(queue["DC"])))]
-                res = (_dc_batch(g, req, idx, device_out) if len(idx) >= MIN_BATCH["DC"]
-                       else [_plain_one(g, req[i], device_out) for i in idx])
-                for i, row in zip(idx, res):
-                    finish(i, row)
-                worked = True
-            for kind in SERVED:  # kinds left to plain calls: one per turn
-                if kind not in servers and queue[kind]:
-                    i = queue[kind].popleft()
-                    finish(i, _plain_one(g, req[i], device_out))
-                    worked = True
-            for kind, srv in servers.items():
-                lanes = lane_of[kind]
-                if not lanes:
-                    continue
-                worked = True
-                for lane in srv.step():
-                    i = lanes.pop(lane)
-                    finish(i, srv.result(lane, out=(pr_sink if kind == "PR" else sink)()))
-            if not worked and left and nxt < len(order):  # idle until the next arrival
+            ready = [k for k in queue if queue[k] or (k in lane_of and lane_of[k])]
+            if not ready:  # idle until the next arrival
                 time.sleep(max(0.0, req[order[nxt]].arrival - (time.perf_counter() - t0)))
+                continue
+            for k in ready:
+                if k not in busy:
+                    vt[k] = max(vt[k], vnow)
+            busy = set(ready)
+            kind = min(ready, key=lambda k: vt[k])
+            t = time.perf_counter()
+            n = act(kind)
+            vt[kind] += (time.perf_counter() - t) / max(1, n)
+            vnow = vt[kind]
     finally:
         for srv in servers.values():
             srv.close()

@@ -176,8 +176,7 @@ void tga_bfs_server_close(tga_bfs_server* s);
  * CC step runs a group of label rounds and finishes the lanes that reached
  * their fixpoint; a PR step runs one iteration of every running lane (each
  * with the iteration count it was admitted with; damping is the server's)
- * and finishes the lanes that reached their count, without waiting for the
- * device.  Results equal tga_connected_components / tga_pagerank_batch's for
+ * and finishes the lanes that reached their count.  Results equal tga_connected_components / tga_pagerank_batch's for
  * the same window, whatever ran beside them.  Server state is allocated on
  * the handle (tga_info's device bytes include it); close servers before
  * their handle (after tga_close their calls fail with TGA_EINVAL). */

@@ -182,10 +182,12 @@ def test_mixed_requests_four_kinds():
         assert abs(kinds.count(k) / len(kinds) - share) < 0.03, k
 
 
-def test_serve_loop_all_kinds():
+def test_serve_loop_all_kinds(monkeypatch):
     """BFS, CC, PR and DC requests: each answered once with its own result;
-    BFS, CC and PR through their servers (admitted in arrival order per kind,
-    never more than 64 running), DC in batched calls of at most 64."""
+    with every server started by a single query, BFS, CC and PR all go
+    through their servers (admitted in arrival order per kind), DC in
+    batched calls of at most 64."""
+    monkeypatch.setattr(serving, "SERVE_FROM", {"BFS": 1, "CC": 1, "PR": 1})
     req = serving.mixed_requests(rate=1e5, count=400, n=1000, t_lo=0, t_hi=1 << 20, seed=6,
                                  shares={"DC": 0.1, "PR": 0.3, "CC": 0.2})
     g = _FakeGraph()
@@ -201,6 +203,54 @@ def test_serve_loop_all_kinds():
         assert admitted == [WANT[kind](r) for r in sorted(req, key=lambda r: r.arrival) if r.kind == kind]
     # the batched loop made no plain call of a served kind
     assert len(g.plain) == sum(r.kind != "DC" for r in req)
+
+
+def test_serve_loop_plain_below_threshold():
+    """An idle CC or PR server starts only once SERVE_FROM queries of its kind
+    wait: a burst (all arrived at once) goes through the servers, sparse
+    arrivals (each answered before the next comes) through plain calls."""
+    burst = serving.mixed_requests(rate=1e9, count=200, n=1000, t_lo=0, t_hi=1 << 20, seed=4,
+                                   shares={"PR": 0.5, "CC": 0.5})
+    g = _FakeGraph()
+    kb = {}
+    serving.serve_batched(g, burst, keep=kb)
+    assert all(kb[i] == WANT[r.kind](r) for i, r in enumerate(burst))
+    assert not g.plain and sorted(s.kind for s in g.servers) == ["CC", "PR"]
+    sparse = serving.mixed_requests(rate=500, count=12, n=1000, t_lo=0, t_hi=1 << 20, seed=4,
+                                    shares={"PR": 0.5, "CC": 0.5})
+    g = _FakeGraph()
+    kb = {}
+    serving.serve_batched(g, sparse, keep=kb)
+    assert all(kb[i] == WANT[r.kind](r) for i, r in enumerate(sparse))
+    assert sorted(g.plain) == sorted(r.kind for r in sparse)
+    assert not [e for e in g.log if e[0] == "admit"]
+
+
+def test_serve_loop_fair_share(monkeypatch):
+    """Weighted fair queuing: while PR steps cost 3 ms each and BFS steps
+    almost nothing, the loop keeps stepping BFS between PR steps instead of
+    alternating the two."""
+    import time as _t
+    monkeypatch.setattr(serving, "SERVE_FROM", {"BFS": 1, "CC": 1, "PR": 1})
+    req = [serving.Request(arrival=0.0, src=3 * k + 2, window=(3 * k + 2, 9), kind="BFS" if k % 2 else "PR")
+           for k in range(40)]
+    g = _FakeGraph()
+    orig = _FakeServer.step
+
+    def step(self):
+        if self.kind == "PR":
+            _t.sleep(0.003)
+        return orig(self)
+
+    monkeypatch.setattr(_FakeServer, "step", step)
+    kb = {}
+    serving.serve_batched(g, req, keep=kb)
+    assert all(kb[i] == WANT[r.kind](r) for i, r in enumerate(req))
+    steps = [e[1] for e in g.log if e[0] == "step"]
+    # every query takes 3 steps: the 20 BFS queries finish within the first
+    # PR step or two, not interleaved one-for-one with PR's three
+    last_bfs = max(i for i, k in enumerate(steps) if k == "BFS")
+    assert steps[:last_bfs + 1].count("PR") <= 2, steps
 
 
 def test_serve_loop_budget_fallback():
