@@ -67,13 +67,15 @@ inline int32_t flat_stride_for(int64_t contacts) {
 //   h = t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77;  h ^= h >> 16;
 //   h *= 0x7FEB352D;  h ^= h >> 15          (all uint32)
 TGA_HD uint32_t contact_key_x(int32_t x) { return (uint32_t)x * 0x85EBCA77u; }
-TGA_HD uint64_t contact_term(uint32_t kx, int32_t r, int32_t t) {
-  uint32_t h = (uint32_t)t ^ ((uint32_t)r * 0x9E3779B1u) ^ kx;
+TGA_HD uint32_t contact_key_r(uint32_t kx, int32_t r) { return ((uint32_t)r * 0x9E3779B1u) ^ kx; }
+TGA_HD uint64_t contact_mix(uint32_t kr, int32_t t) {
+  uint32_t h = (uint32_t)t ^ kr;
   h ^= h >> 16;
   h *= 0x7FEB352Du;
   h ^= h >> 15;
   return h;
 }
+TGA_HD uint64_t contact_term(uint32_t kx, int32_t r, int32_t t) { return contact_mix(contact_key_r(kx, r), t); }
 TGA_HD uint64_t contact_hash(int32_t x, int32_t r, int32_t t) { return contact_term(contact_key_x(x), r, t); }
 
 // Error bits reported through the per-handle device error word.
@@ -843,21 +845,28 @@ struct TsStage {
   int32_t prev;   // the timestamp before the next code
 };
 
-// Stages the 16-byte unit at relative dword wend (stream words 2u, 2u + 1).
+// Stages the 16-byte unit at relative dword wend (stream words 2u, 2u + 1)
+// with one 16-byte load.  A unit is staged only while its first dword lies
+// before the item's end (stage_fill), and the last word of a unit then lies
+// within the stream's five words of padding, so no bound check is needed.
 TGA_HD void stage_unit(const Streams& S, TsStage& st) {
   const uint64_t u = (st.base + st.wend) >> 2;
-  const uint64_t lastw = (uint64_t)S.t_last;
-  const uint64_t w0 = S.t[2 * u <= lastw ? 2 * u : lastw];
-  const uint64_t w1 = S.t[2 * u + 1 <= lastw ? 2 * u + 1 : lastw];
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint4 v = reinterpret_cast<const uint4*>(S.t)[u];
+  const uint32_t d0 = v.y, d1 = v.x, d2 = v.w, d3 = v.z;  // each word's high dword first
+#else
+  const uint64_t w0 = S.t[2 * u], w1 = S.t[2 * u + 1];
+  const uint32_t d0 = (uint32_t)(w0 >> 32), d1 = (uint32_t)w0, d2 = (uint32_t)(w1 >> 32), d3 = (uint32_t)w1;
+#endif
   const uint32_t slot = st.wend & (kStageDw - 1);
   uint32_t* p = st.s + (size_t)slot * st.stride;
-  p[0] = (uint32_t)(w0 >> 32);
-  p[st.stride] = (uint32_t)w0;
-  p[2 * st.stride] = (uint32_t)(w1 >> 32);
-  p[3 * st.stride] = (uint32_t)w1;
+  p[0] = d0;
+  p[st.stride] = d1;
+  p[2 * st.stride] = d2;
+  p[3 * st.stride] = d3;
   if (slot == 0) {
-    st.s[(size_t)kStageDw * st.stride] = (uint32_t)(w0 >> 32);
-    st.s[(size_t)(kStageDw + 1) * st.stride] = (uint32_t)w0;
+    st.s[(size_t)kStageDw * st.stride] = d0;
+    st.s[(size_t)(kStageDw + 1) * st.stride] = d1;
   }
   st.wend += 4;
 }
@@ -889,25 +898,64 @@ TGA_HD void stage_fill(const Streams& S, TsStage& st) {
   while (st.wend + 4 <= d + kStageDw && st.base + st.wend < st.lim) stage_unit(S, st);
 }
 
-// 64 stream bits at the current position (the window must hold dwords
-// pos/32 .. pos/32 + 2).
-TGA_HD uint64_t stage_peek(const TsStage& st) {
-  const uint32_t d = (st.pos >> 5) & (kStageDw - 1);
-  const uint32_t sh = st.pos & 31u;
-  const uint32_t* p = st.s + (size_t)d * st.stride;
-  const uint64_t hi = ((uint64_t)p[0] << 32) | p[st.stride];
-  const uint64_t lo = (uint64_t)p[2 * st.stride] << sh;
-  return (hi << sh) | (lo >> 32);
+// The 32-bit funnel shift ({hi, lo} >> (n mod 32)) mod 2^32: one
+// v_alignbit_b32 on the device.
+TGA_HD uint32_t funnel32(uint32_t hi, uint32_t lo, uint32_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, n);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (n & 31u));
+#endif
+}
+
+// A valid zeta_2 code of the timestamp stream codes an int2nat'ed int32
+// difference, below 2^32, so its unary prefix has at most 16 zeros.
+constexpr uint32_t kZeta2MaxPrefix = 16;
+
+// Decodes the zeta_2 codeword at the start of the 64-bit window (hi, lo) and
+// returns the int2nat inverse of its value modulo 2^32; `len` receives the
+// codeword length and `hmax` the largest unary prefix seen (a prefix above
+// kZeta2MaxPrefix marks an invalid stream; the caller checks it once).
+//
+// With h the prefix zeros, the code is h zeros, a one, the 2h + 1 bits m of
+// the short form and, when m >= 4^h (the bit s after the one is set), one
+// more bit b (BVMultiGraph's minimal binary coding, bfs.cu:640-665).  Let B
+// be the len - h = 2h + 2 + s bits after the zeros, read as a number: the
+// short form has B = 2^(2h+1) + m and the value m + 4^h - 1, the long form
+// B = 2^(2h+2) + 2m + b and the value 2m + b - 1.  Both are B - 4^(h+s) - 1,
+// and only B's low 32 bits are needed for a value below 2^32.
+TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hmax) {
+  const uint32_t h = (uint32_t)__builtin_clz(hi | 1u);  // hi == 0: h = 31, an invalid prefix
+  const uint32_t s = ((hi << h) >> 30) & 1u;
+  const uint32_t tl = 3 * h + s;
+  len = tl + 2;
+  const uint64_t p = ((uint64_t)hi << 32) | lo;
+  const uint32_t b = (uint32_t)(p >> ((62u - tl) & 63u));
+  const uint32_t pw = (uint32_t)(1ull << ((2 * (h + s)) & 63u));
+  const uint32_t v = b - pw - 1u;
+  hmax = hmax > h ? hmax : h;
+  return (int32_t)((v >> 1) ^ (0u - (v & 1u)));
 }
 
 // Next timestamp: zeta_2 of the int2nat delta from the previous one
-// (EvolvingMultiGraph.java:131-144; bfs.cu:640-665).
-TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st) {
+// (EvolvingMultiGraph.java:131-144; bfs.cu:640-665), decoded from the window.
+//
+// The window's 64 bits at bit offset pos are two funnel shifts of three
+// staged dwords: with t = pos - 1, dwords t/32 .. t/32 + 2 shifted by
+// (-pos) mod 32.  When pos is a multiple of 32 the shift is 0 and the first
+// dword (one before the current one, possibly already replaced) drops out;
+// otherwise t/32 is pos's own dword.  At pos = 0, t/32 maps to slot 15 and
+// the next two to the mirror slots of 0 and 1.
+TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st, uint32_t& hmax) {
   if ((st.pos >> 5) + 3 > st.wend) stage_fill(S, st);  // codes longer than the burst's share
+  const uint32_t t = st.pos - 1u;
+  const uint32_t* p = st.s + (size_t)((t >> 5) & (kStageDw - 1)) * st.stride;
+  const uint32_t w0 = p[0], w1 = p[st.stride], w2 = p[2 * st.stride];
+  const uint32_t hi = funnel32(w0, w1, ~t), lo = funnel32(w1, w2, ~t);
   uint32_t len;
-  const uint64_t v = dec_zeta<2>(S, stage_peek(st), len);
+  const int32_t d = zeta2_delta(hi, lo, len, hmax);
   st.pos += len;
-  st.prev = (int32_t)((uint32_t)st.prev + (uint32_t)nat2int(v));
+  st.prev = (int32_t)((uint32_t)st.prev + (uint32_t)d);
   return st.prev;
 }
 
@@ -925,6 +973,15 @@ struct RunRing {
   int32_t stride;
 };
 
+// Actions that cache per-neighbor state define run(r); it runs when the walk
+// moves to a run of contacts to neighbor r.
+template <class Act>
+TGA_HD auto act_run(Act& a, int32_t r, int) -> decltype(a.run(r), void()) {
+  a.run(r);
+}
+template <class Act>
+TGA_HD void act_run(Act&, int32_t, long) {}
+
 // Walks contacts [k, kend) of one work item and hands (x, r, t, k) to
 // act.contact(); a false return stops the walk.
 //
@@ -934,7 +991,9 @@ struct RunRing {
 // branches run once per run, not once per contact), and a consume phase that
 // decodes one timestamp per contact and takes the neighbor from the current
 // run.  On graphs whose pairs repeat (runs of tens of contacts) the refill
-// takes one or two passes per burst.
+// takes one or two passes per burst.  The consume phase takes the contacts
+// the ring holds, at most a burst, so it never waits on the ring; the
+// timestamp codes' validity is checked once per walk.
 template <int MAXD, class Act>
 TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t k, int32_t kend, Act& act,
                  RunRing ring) {
@@ -942,6 +1001,7 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
   int32_t have = 0;             // contacts of the held runs
   int32_t r = 0, c = 0;         // current run: neighbor, contacts left in it
   bool more = true;             // the record has runs left
+  uint32_t hmax = 0;            // largest zeta_2 prefix decoded
   while (k < kend) {
     while (more && c + have < S.burst && tail - head < (uint32_t)kRing) {
       int32_t cnt;
@@ -955,26 +1015,31 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
       ++tail;
       have += cnt;
     }
+    const int32_t n = min32(min32(kend - k, S.burst), c + have);
+    if (n <= 0) {
+      raise_err(S, kErrExhausted);  // fewer successors than the outdegree
+      break;
+    }
     stage_fill(S, ts);
-    const int32_t stop = min32(kend, k + S.burst);
-    for (; k < stop; ++k) {
+    for (int32_t j = 0; j < n; ++j) {
       if (c == 0) {
-        if (head == tail) {
-          if (more) break;  // the ring's runs covered fewer contacts than a burst
-          raise_err(S, kErrExhausted);  // fewer successors than the outdegree
-          return;
-        }
         const int2 e = ring.e[(head & (kRing - 1)) * ring.stride];
         ++head;
         r = e.x;
         c = e.y;
         have -= c;
+        act_run(act, r, 0);
       }
       --c;
-      const int32_t t = stage_ts_next(S, ts);
-      if (!act.contact(x, r, t, k)) return;
+      const int32_t t = stage_ts_next(S, ts, hmax);
+      if (!act.contact(x, r, t, k + j)) {
+        kend = k;  // stop after the validity check
+        break;
+      }
     }
+    k += n;
   }
+  if (hmax > kZeta2MaxPrefix) raise_err(S, kErrCode);
 }
 
 // Actions that keep per-item state define item(code); it runs first.

@@ -120,9 +120,9 @@ struct Items {
 #define LANE_RING()                                                       \
   __shared__ int2 ring_lds_[kRing * kBlock];                              \
   __shared__ uint32_t stage_lds_[(kStageDw + 2) * kBlock];                \
-  const RunRing ring_{ring_lds_ + threadIdx.x, (int32_t)blockDim.x};      \
+  const RunRing ring_{ring_lds_ + threadIdx.x, kBlock};                   \
   const TsStage stage_ {                                                  \
-    stage_lds_ + threadIdx.x, (int32_t)blockDim.x, 0, 0, 0, 0, 0          \
+    stage_lds_ + threadIdx.x, kBlock, 0, 0, 0, 0, 0                       \
   }
 
 #define FOR_ITEMS(I, i) \
@@ -336,12 +336,14 @@ struct RaAct {
   uint64_t cs = 0;
   uint32_t cnt = 0;  // contacts of this lane (far below 2^32 per pass)
   uint32_t kx = 0;
+  uint32_t kr = 0;   // contact_term's key of the current run's neighbor
   __device__ bool begin(int32_t x, bool, Split) {
     kx = contact_key_x(x);
     return true;
   }
-  __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
-    cs += contact_term(kx, r, t);
+  __device__ void run(int32_t r) { kr = contact_key_r(kx, r); }
+  __device__ bool contact(int32_t, int32_t, int32_t t, int32_t) {
+    cs += contact_mix(kr, t);
     ++cnt;
     return true;
   }
