@@ -256,7 +256,7 @@ struct BitBuf {
       a = b;
       b = c;
       ++idx;
-      c = (idx + 2 <= last) ? w[idx + 2] : 0;
+      c = w[minu32(idx + 2, last)];  // past the end: the last padding word (zero)
     }
   }
   TGA_HD uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
@@ -973,14 +973,17 @@ struct RunRing {
   int32_t stride;
 };
 
-// Actions that cache per-neighbor state define run(r); it runs when the walk
-// moves to a run of contacts to neighbor r.
+// Actions may define ring_val(v): the value the walk hands to contact() for
+// neighbor v (a per-neighbor key computed once per run of contacts when the
+// run is decoded); without it contact() gets the neighbor itself.
 template <class Act>
-TGA_HD auto act_run(Act& a, int32_t r, int) -> decltype(a.run(r), void()) {
-  a.run(r);
+TGA_HD auto act_ring_val(Act& a, int32_t v, int) -> decltype(a.ring_val(v)) {
+  return a.ring_val(v);
 }
 template <class Act>
-TGA_HD void act_run(Act&, int32_t, long) {}
+TGA_HD int32_t act_ring_val(Act&, int32_t v, long) {
+  return v;
+}
 
 // Walks contacts [k, kend) of one work item and hands (x, r, t, k) to
 // act.contact(); a false return stops the walk.
@@ -998,12 +1001,12 @@ template <int MAXD, class Act>
 TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t k, int32_t kend, Act& act,
                  RunRing ring) {
   uint32_t head = 0, tail = 0;  // ring positions; tail - head runs held
-  int32_t have = 0;             // contacts of the held runs
-  int32_t r = 0, c = 0;         // current run: neighbor, contacts left in it
+  int32_t avail = 0;            // contacts held: the current run's and the ring's
+  int32_t r = 0, c = 0;         // current run: neighbor (or its ring_val), contacts left in it
   bool more = true;             // the record has runs left
   uint32_t hmax = 0;            // largest zeta_2 prefix decoded
   while (k < kend) {
-    while (more && c + have < S.burst && tail - head < (uint32_t)kRing) {
+    while (more && avail < S.burst && tail - head < (uint32_t)kRing) {
       int32_t cnt;
       const int32_t v = dec_next_run<MAXD>(S, D, cnt);
       if ((uint32_t)v >= (uint32_t)S.n) {
@@ -1011,11 +1014,11 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
         more = false;
         break;
       }
-      ring.e[(tail & (kRing - 1)) * ring.stride] = make_int2(v, cnt);
+      ring.e[(tail & (kRing - 1)) * ring.stride] = make_int2(act_ring_val(act, v, 0), cnt);
       ++tail;
-      have += cnt;
+      avail += cnt;
     }
-    const int32_t n = min32(min32(kend - k, S.burst), c + have);
+    const int32_t n = min32(min32(kend - k, S.burst), avail);
     if (n <= 0) {
       raise_err(S, kErrExhausted);  // fewer successors than the outdegree
       break;
@@ -1027,8 +1030,6 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
         ++head;
         r = e.x;
         c = e.y;
-        have -= c;
-        act_run(act, r, 0);
       }
       --c;
       const int32_t t = stage_ts_next(S, ts, hmax);
@@ -1037,6 +1038,7 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
         break;
       }
     }
+    avail -= n;
     k += n;
   }
   if (hmax > kZeta2MaxPrefix) raise_err(S, kErrCode);

@@ -336,14 +336,14 @@ struct RaAct {
   uint64_t cs = 0;
   uint32_t cnt = 0;  // contacts of this lane (far below 2^32 per pass)
   uint32_t kx = 0;
-  uint32_t kr = 0;   // contact_term's key of the current run's neighbor
   __device__ bool begin(int32_t x, bool, Split) {
     kx = contact_key_x(x);
     return true;
   }
-  __device__ void run(int32_t r) { kr = contact_key_r(kx, r); }
-  __device__ bool contact(int32_t, int32_t, int32_t t, int32_t) {
-    cs += contact_mix(kr, t);
+  // The walk hands contact() the neighbor's checksum key, computed once per run.
+  __device__ int32_t ring_val(int32_t r) const { return (int32_t)contact_key_r(kx, r); }
+  __device__ bool contact(int32_t, int32_t kr, int32_t t, int32_t) {
+    cs += contact_mix((uint32_t)kr, t);
     ++cnt;
     return true;
   }
