@@ -24,11 +24,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -2882,8 +2884,16 @@ extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
 struct NoAllocScratch;
 static int scratch_for(int device, NoAllocScratch** out);
 
+// Flat records longer than this many contacts have their checkpoints built
+// on the host, while the device builds the others.  The build decodes a
+// record sequentially, and one host core does that about ten times faster
+// than one GPU lane; the longest records (hubs of ~10^6 contacts) set the
+// build time.
+constexpr int64_t kHostCkptMin = 1 << 16;
+
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
-static int build_schedule(tga_graph* g) {
+// HS holds host copies of the streams and indexes (its vpos unset).
+static int build_schedule(tga_graph* g, Streams HS, int talk) {
   const int64_t n = g->n;
   Streams& S = g->S;
   int32_t* d_deg = nullptr;
@@ -2958,18 +2968,36 @@ static int build_schedule(tga_graph* g) {
   int32_t* d_nch = nullptr;
   if ((rc = g->alloc(d_nch, nbig_all))) return rc;
   size_t nch_at[kClasses];
+  // Flat records above kHostCkptMin contacts: built on the host (below),
+  // overlapping the device's build of the rest; the device list of class 0
+  // is the others, their chunk counts mapped back through dev0.
+  int64_t host_min = kHostCkptMin;
+  if (const char* e = std::getenv("TGA_HOST_CKPT_MIN")) host_min = std::max<long long>(0, std::atoll(e));  // tests
+  std::vector<int32_t> host_x, dev0_x;
+  std::vector<int64_t> host_slot, dev0_slot;
+  std::vector<size_t> host_at, dev0_at;
+  for (size_t i = 0; i < bigv[0].size(); ++i) {
+    const bool host = deg[(size_t)bigv[0][i]] > host_min;
+    (host ? host_x : dev0_x).push_back(bigv[0][i]);
+    (host ? host_slot : dev0_slot).push_back(slotv[0][i]);
+    (host ? host_at : dev0_at).push_back(i);
+  }
   {
     size_t at = 0;
     for (int c = 0; c < kClasses; ++c) {
       nch_at[c] = at;
       at += bigv[c].size();
       const int64_t* d_slot = nullptr;
-      if ((rc = g->upload(g->big[c], bigv[c])) || (rc = g->upload_const(d_slot, slotv[c]))) return rc;
+      if ((rc = g->upload(g->big[c], bigv[c]))) return rc;
       g->n_big[c] = (int32_t)bigv[c].size();
-      if (bigv[c].empty()) continue;
-      const int blocks = (int)((bigv[c].size() + 63) / 64);
-      const int32_t nb = (int32_t)bigv[c].size();
-      if (c == 0) k_ckpt_build<0><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
+      const std::vector<int32_t>& lx = c == 0 ? dev0_x : bigv[c];
+      if (lx.empty()) continue;
+      const int32_t* d_x = g->big[c];
+      if (c == 0 && (rc = g->upload_const(d_x, dev0_x))) return rc;
+      if ((rc = g->upload_const(d_slot, c == 0 ? dev0_slot : slotv[c]))) return rc;
+      const int blocks = (int)((lx.size() + 63) / 64);
+      const int32_t nb = (int32_t)lx.size();
+      if (c == 0) k_ckpt_build<0><<<blocks, 64, 0, g->stream>>>(S, d_x, d_slot, nb, g->ck, d_nch + nch_at[c]);
       if (c == 1) k_ckpt_build<1><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
       if (c == 2) k_ckpt_build<2><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
       if (c == 3) k_ckpt_build<3><<<blocks, 64, 0, g->stream>>>(S, g->big[c], d_slot, nb, g->ck, d_nch + nch_at[c]);
@@ -2977,8 +3005,70 @@ static int build_schedule(tga_graph* g) {
     }
   }
   std::vector<int32_t> nch_host(nbig_all);
-  if (!nch_host.empty())
-    HIP_TRY(hipMemcpyAsync(nch_host.data(), d_nch, nch_host.size() * 4, hipMemcpyDeviceToHost, g->stream));
+  if (!host_x.empty()) {
+    // Host build of the longest flat records (the device works meanwhile):
+    // their record starts from the host indexes, then the checkpoints into
+    // one host block per record, uploaded into their slots.
+    const auto t0 = std::chrono::steady_clock::now();
+    int32_t herr = 0;
+    HS.err = &herr;
+    HS.flat_stride = S.flat_stride;
+    HS.burst = S.burst;
+    std::unique_ptr<uint64_t[]> hvpos(new uint64_t[2 * (size_t)n + 4]);  // only the entries used are written
+    for (const int32_t x : host_x)
+      for (int64_t v = x; v <= std::min<int64_t>(x + 1, n - 1); ++v) {
+        hvpos[2 * (size_t)v] = ef_get(HS, HS.gi, (uint64_t)v);
+        hvpos[2 * (size_t)v + 1] = ef_get(HS, HS.ti, (uint64_t)v);
+      }
+    HS.vpos = hvpos.get();
+    int64_t hslots = 0;
+    for (const int32_t x : host_x) hslots += (deg[(size_t)x] + S.flat_stride - 1) / S.flat_stride;
+    // slots of the host records are not contiguous: build into one host block
+    // ordered like host_x, then upload record by record
+    std::vector<Ckpt> hck((size_t)hslots);
+    std::vector<int64_t> hpos(host_x.size());
+    {
+      int64_t at = 0;
+      for (size_t i = 0; i < host_x.size(); ++i) {
+        hpos[i] = at;
+        at += (deg[(size_t)host_x[i]] + S.flat_stride - 1) / S.flat_stride;
+      }
+    }
+    std::vector<int32_t> hnch(host_x.size(), 0);
+    {
+      // ckpt_build_record writes split-run slot numbers relative to the
+      // record's own first slot (slot0 argument) and the checkpoints at out
+      std::atomic<size_t> next{0};
+      const int nt = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+      auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1)) < host_x.size();)
+          hnch[i] = ckpt_build_record<0>(HS, host_x[i], host_slot[i], hck.data() + hpos[i]);
+      };
+      std::vector<std::thread> pool;
+      for (int k = 1; k < nt && k < (int)host_x.size(); ++k) pool.emplace_back(work);
+      work();
+      for (auto& t : pool) t.join();
+    }
+    if (herr) return fail(TGA_EFORMAT, "decode error in the host checkpoint build (bits " + std::to_string(herr) + ")");
+    for (size_t i = 0; i < host_x.size(); ++i) {
+      HIP_TRY(hipMemcpyAsync(g->ck + host_slot[i], hck.data() + hpos[i], sizeof(Ckpt) * (size_t)hnch[i],
+                             hipMemcpyHostToDevice, g->stream));
+      nch_host[nch_at[0] + host_at[i]] = hnch[i];
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));  // the uploads read hck, which ends here
+    if (talk)
+      std::fprintf(stderr, "[tga_open] host checkpoint build: %zu records, %lld chunks, %.3f s\n", host_x.size(),
+                   (long long)hslots,
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
+  if (!nch_host.empty()) {
+    std::vector<int32_t> dn(nbig_all);
+    HIP_TRY(hipMemcpyAsync(dn.data(), d_nch, dn.size() * 4, hipMemcpyDeviceToHost, g->stream));
+    if ((rc = g->check_err())) return rc;  // synchronizes the stream (the uploads of hck included)
+    for (size_t j = 0; j < dev0_at.size(); ++j) nch_host[nch_at[0] + dev0_at[j]] = dn[nch_at[0] + j];
+    for (int c = 1; c < kClasses; ++c)
+      for (size_t i = 0; i < bigv[c].size(); ++i) nch_host[nch_at[c] + i] = dn[nch_at[c] + i];
+  }
   if ((rc = g->check_err())) return rc;
 
   // Static item lists.  Flat group: chunks first, then whole records longest
@@ -3176,7 +3266,18 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   if (hipMemset(g->d_err, 0, sizeof(int32_t)) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipMemset failed"));
   phase("device+upload");
 
-  if ((rc = build_schedule(g))) return bail(rc);
+    {
+    Streams HS = S;  // the same fields over the host copies
+    HS.g = gw.data();
+    HS.t = tw.data();
+    HS.gi = EFDir{glw.data(), guw.data(), gsamp.data(), gspill.data(), S.gi.upper_words, lg};
+    HS.ti = EFDir{tlw.data(), tuw.data(), tsamp.data(), tspill.data(), S.ti.upper_words, lt};
+    HS.err = nullptr;
+    HS.ck = nullptr;
+    HS.vinfo = nullptr;
+    HS.vpos = nullptr;
+    if ((rc = build_schedule(g, HS, talk ? 1 : 0))) return bail(rc);
+  }
   if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipStreamSynchronize failed"));
   phase("schedule");
 
