@@ -922,19 +922,20 @@ constexpr uint32_t kZeta2MaxPrefix = 16;
 // more bit b (BVMultiGraph's minimal binary coding, bfs.cu:640-665).  Let B
 // be the len - h = 2h + 2 + s bits after the zeros, read as a number: the
 // short form has B = 2^(2h+1) + m and the value m + 4^h - 1, the long form
-// B = 2^(2h+2) + 2m + b and the value 2m + b - 1.  Both are B - 4^(h+s) - 1,
-// and only B's low 32 bits are needed for a value below 2^32.
+// B = 2^(2h+2) + 2m + b and the value 2m + b - 1.  Both are B - 4^(h+s) - 1.
+// The value needs 33 bits (two int32 timestamps can differ by up to
+// 2^32 - 1), and its int2nat inverse is taken modulo 2^32, as the
+// timestamps are.
 TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hmax) {
   const uint32_t h = (uint32_t)__builtin_clz(hi | 1u);  // hi == 0: h = 31, an invalid prefix
   const uint32_t s = ((hi << h) >> 30) & 1u;
   const uint32_t tl = 3 * h + s;
   len = tl + 2;
   const uint64_t p = ((uint64_t)hi << 32) | lo;
-  const uint32_t b = (uint32_t)(p >> ((62u - tl) & 63u));
-  const uint32_t pw = (uint32_t)(1ull << ((2 * (h + s)) & 63u));
-  const uint32_t v = b - pw - 1u;
+  const uint64_t b = p >> ((62u - tl) & 63u);
+  const uint64_t v = b - (1ull << ((2 * (h + s)) & 63u)) - 1u;
   hmax = hmax > h ? hmax : h;
-  return (int32_t)((v >> 1) ^ (0u - (v & 1u)));
+  return (int32_t)((uint32_t)(v >> 1) ^ (0u - ((uint32_t)v & 1u)));
 }
 
 // Next timestamp: zeta_2 of the int2nat delta from the previous one

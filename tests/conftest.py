@@ -140,3 +140,20 @@ def ds_nf(data_root):
 def ds_cl(data_root):
     s, d, t = fmt.gen_chung_lu(1 << 14, 1 << 18, exponent=2.1, t_span=1 << 20, period=(1 << 20) // 64, seed=5)
     return DS(data_root, "cl14", s, d, t, n_min=1 << 14)
+
+
+@pytest.fixture(scope="session")
+def ds_ext(data_root):
+    """Timestamps across the whole int32 range: consecutive contacts of a
+    record jump between INT32_MIN, INT32_MAX and values near 0, so deltas
+    reach +-(2^32 - 1) (33-bit int2nat values, the longest zeta_2 codes);
+    one record long enough to be chunked."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    ext = np.array([-2**31, 2**31 - 1, -1, 0, 1, -2**31 + 1, 2**31 - 2], np.int64)
+    src = np.concatenate([rng.integers(0, n, 40000), np.full(3000, 7)])
+    dst = rng.integers(0, n, len(src))
+    t = np.where(rng.random(len(src)) < 0.7, ext[rng.integers(0, len(ext), len(src))],
+                 rng.integers(-2**31, 2**31, len(src)))
+    s, d, tt = sort_contacts(src, dst, t)
+    return DS(data_root, "ext32", s, d, tt, n_min=n)

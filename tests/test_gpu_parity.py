@@ -19,14 +19,23 @@ WINDOWS = [None, (1000, 3000), (0, 0)]
 
 
 def rel_linf(a, b):
+    """max |a - b| / max |b| over the finite entries of b; non-finite entries
+    (PageRank's weight 1 - exp(-0.1 t) overflows for very negative times)
+    must be the same non-finite value in a, or the error is infinite."""
+    a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
+    fin = np.isfinite(b)
+    if not np.array_equal(a[~fin], b[~fin], equal_nan=True) or not np.isfinite(a[fin]).all():
+        return float("inf")
+    if not fin.any():
+        return 0.0
+    return float(np.abs(a[fin] - b[fin]).max() / max(np.abs(b[fin]).max(), 1e-300))
 
 
-@pytest.fixture(scope="module", params=["sample", "adv", "rmat16", "cl14", "nf14"])
+@pytest.fixture(scope="module", params=["sample", "adv", "rmat16", "cl14", "nf14", "ext32"])
 def pair(request):
     ds = request.getfixturevalue({"sample": "ds_sample", "adv": "ds_adv", "rmat16": "ds_rmat",
-                                  "cl14": "ds_cl", "nf14": "ds_nf"}[request.param])
+                                  "cl14": "ds_cl", "nf14": "ds_nf", "ext32": "ds_ext"}[request.param])
     print(f"\n[gpu] dataset {ds.name} n={ds.n} m={len(ds.src)}", flush=True)
     g = TemporalGraph(ds.dir, ds.name)
     yield ds, g
@@ -38,6 +47,15 @@ def windows_for(ds):
     lo, hi = int(t.min()), int(t.max())
     span = hi - lo
     return [None, (lo + span // 4, lo + span // 2), (lo + span // 3, lo + span // 3)]
+
+
+def skip_nonfinite(ds):
+    """The ext32 graph's timestamps reach -2^31: PageRank's weight
+    1 - exp(-0.1 t) (pagerank.cu) overflows to -inf below t of about -7100,
+    so its ranks are not finite numbers to compare; the graph is there for the
+    decoder (see the integer entry points)."""
+    if ds.name == "ext32":
+        pytest.skip("PageRank/HITS ranks are not finite for timestamps near -2^31")
 
 
 PATHS = pytest.mark.parametrize("path", ["lists", "decode"])
@@ -235,6 +253,7 @@ def test_pagerank_server_mixed_admission(pair):
     pagerank_batch (the same per-column arithmetic) and within the PageRank
     bar of the oracle."""
     ds, g = pair
+    skip_nonfinite(ds)
     rng, wins = _server_queries(ds, 90, 37)
     iters = [int(v) for v in rng.choice([0, 1, 3, 20], len(wins))]
     with g.pagerank_server(damping=0.85) as srv:
@@ -346,6 +365,7 @@ def test_graph_coloring(pair, path, monkeypatch):
 @PATHS
 def test_pagerank(pair, path, monkeypatch):
     ds, g = pair
+    skip_nonfinite(ds)
     use_path(path, monkeypatch)
     for w in windows_for(ds)[:2]:
         got = g.pagerank(w, iters=20)
@@ -360,6 +380,7 @@ def test_pagerank_batch(pair):
     of the oracle's single-window FP32 result (and of the FP64 oracle), and
     equal to the single-window engine result up to FP64 summation order."""
     ds, g = pair
+    skip_nonfinite(ds)
     rng = np.random.default_rng(17)
     lo, hi = int(ds.t.min()), int(ds.t.max())
     wins = []
@@ -394,6 +415,7 @@ def test_pagerank_batch(pair):
 @PATHS
 def test_pagerank_tol(pair, path, monkeypatch):
     ds, g = pair
+    skip_nonfinite(ds)
     use_path(path, monkeypatch)
     got = g.pagerank(None, iters=100, tol=1e-6)
     ref, it = ds.oracle().pagerank(100, tol=1e-6, dtype=np.float32)
@@ -411,6 +433,7 @@ def test_hits_batch(pair):
     """Up to 64 windows together: every row within the HITS bar of the
     oracle's single-window FP32 result and within 1e-6 of the engine's."""
     ds, g = pair
+    skip_nonfinite(ds)
     rng = np.random.default_rng(29)
     lo, hi = int(ds.t.min()), int(ds.t.max())
     wins = []
@@ -440,6 +463,7 @@ def test_hits_batch(pair):
 @PATHS
 def test_hits(pair, path, monkeypatch):
     ds, g = pair
+    skip_nonfinite(ds)
     use_path(path, monkeypatch)
     for w in windows_for(ds)[:2]:
         hub, auth = g.hits(w, iters=20)
@@ -457,6 +481,7 @@ def test_served_mix(pair, budget):
     its bar)."""
     from compresstemporalgraph_amd import serving
     ds, g = pair
+    skip_nonfinite(ds)
     lo, hi = int(ds.t.min()), int(ds.t.max())
     req = serving.mixed_requests(rate=2e4, count=80, n=ds.n, t_lo=lo, t_hi=hi + 1, seed=13,
                                  shares={"DC": 0.15, "PR": 0.25, "CC": 0.2})
@@ -501,8 +526,9 @@ def test_autotune_keeps_results(pair):
     s = int(ds.src[len(ds.src) // 2])
     np.testing.assert_array_equal(g.bfs(s), o.bfs(s))
     np.testing.assert_array_equal(g.connected_components(), o.cc())
-    ref, _ = o.pagerank(20, dtype=np.float32)
-    assert rel_linf(g.pagerank(iters=20), ref) <= REL_TOL
+    if ds.name != "ext32":  # its ranks are not finite (skip_nonfinite)
+        ref, _ = o.pagerank(20, dtype=np.float32)
+        assert rel_linf(g.pagerank(iters=20), ref) <= REL_TOL
 
 
 def test_graph_coloring_batch(pair):

@@ -66,7 +66,7 @@ def _raw_sums(ds):
     return np.bincount(ds.src, minlength=ds.n).astype(np.int64), sums
 
 
-@pytest.mark.parametrize("which", ["ds_sample", "ds_adv", "ds_cl", "ds_nf", "ds_rmat"])
+@pytest.mark.parametrize("which", ["ds_sample", "ds_adv", "ds_cl", "ds_nf", "ds_rmat", "ds_ext"])
 def test_host_decode_matches_raw(harness, which, request, tmp_path):
     ds = request.getfixturevalue(which)
     cnt, sums, err = _run(harness, ds, tmp_path)
