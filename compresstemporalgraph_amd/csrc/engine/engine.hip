@@ -2881,6 +2881,11 @@ struct AllocTxn {
 
 extern "C" const char* tga_last_error(void) { return g_err.c_str(); }
 
+// The error text of the other translation units' entry points (encode.hip).
+namespace tga {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace tga
+
 struct NoAllocScratch;
 static int scratch_for(int device, NoAllocScratch** out);
 

@@ -302,19 +302,25 @@ static bool progress_on() {
 
 extern "C" const char* tgaf_last_error(void) { return g_err.c_str(); }
 
-extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, int64_t n_min,
-                           const char* dir, const char* name, tgaf_stats* st) {
-  if (!src || !dst || !t || m <= 0 || !dir || !name) return fail(TGAF_EINVAL, "tgaf_encode: bad arguments");
+static int write_properties(const char* dir, const std::string& base, int64_t n, int64_t m, uint64_t graph_bits,
+                            const tgaf_stats& s, tgaf_stats* st);
+
+// The encoder behind tgaf_encode and tgaf_encode_graph: with t, every file;
+// without (t == nullptr), the graph stream and .properties only, the record
+// bit offsets going to goff_out (n + 1 entries).
+static int encode_impl(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, int64_t n_min,
+                       const char* dir, const char* name, tgaf_stats* st, uint64_t* goff_out) {
+  if (!src || !dst || m <= 0 || !dir || !name || (!t && !goff_out)) return fail(TGAF_EINVAL, "tgaf_encode: bad arguments");
   int64_t n = n_min;
   int64_t tmin = INT64_MAX;
   for (int64_t i = 0; i < m; ++i) {
     if (src[i] < 0 || dst[i] < 0) return fail(TGAF_EFORMAT, "negative node identifier at contact " + std::to_string(i));
     if (i > 0 && (src[i] < src[i - 1] || (src[i] == src[i - 1] && dst[i] < dst[i - 1])))
       return fail(TGAF_EFORMAT, "contacts must be sorted by (src, dst); violated at contact " + std::to_string(i));
-    if (t[i] < INT32_MIN || t[i] > INT32_MAX)
+    if (t && (t[i] < INT32_MIN || t[i] > INT32_MAX))
       return fail(TGAF_ERANGE, "timestamp outside int32 at contact " + std::to_string(i));
     n = std::max<int64_t>(n, (int64_t)std::max(src[i], dst[i]) + 1);
-    tmin = std::min(tmin, t[i]);
+    if (t) tmin = std::min(tmin, t[i]);
   }
   if (n < 2) return fail(TGAF_ERANGE, "a graph needs at least two vertex ids");
 
@@ -381,6 +387,17 @@ extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t
   std::vector<uint8_t>& graph = gw.finish();
   std::vector<uint8_t>& offs = ow.finish();
 
+  const std::string base(name);
+  s.nodes = n;
+  s.arcs = m;
+  s.graph_bits = graph_bits;
+  if (!t) {
+    if (!write_file(join(dir, base + ".graph"), graph.data(), graph.size()))
+      return fail(TGAF_EIO, std::string("cannot write dataset files in ") + dir);
+    std::copy(goff.begin(), goff.end(), goff_out);
+    return write_properties(dir, base, n, m, graph_bits, s, st);
+  }
+
   // ---- .timestamps (EvolvingMultiGraph.storeTimestampsAndIndex) -----------
   BitWriter tw;
   tw.put((uint64_t)tmin, 64);
@@ -399,21 +416,22 @@ extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t
   const uint64_t ts_bits = tw.bits();
   std::vector<uint8_t>& ts = tw.finish();
 
-  const std::string base(name);
   if (!write_file(join(dir, base + ".graph"), graph.data(), graph.size()) ||
       !write_file(join(dir, base + ".offsets"), offs.data(), offs.size()) ||
       !write_file(join(dir, base + ".timestamps"), ts.data(), ts.size()))
     return fail(TGAF_EIO, std::string("cannot write dataset files in ") + dir);
 
-  s.nodes = n;
-  s.arcs = m;
   s.min_timestamp = tmin;
-  s.graph_bits = graph_bits;
   s.timestamps_bits = ts_bits;
   const int rc = write_indexes(dir, goff, graph.size(), toff, ts.size(), n, &s);
   if (rc != TGAF_OK) return rc;
+  return write_properties(dir, base, n, m, graph_bits, s, st);
+}
 
-  // .properties (BVMultiGraph.java:2113-2150; keys the loader checks)
+// .properties (BVMultiGraph.java:2113-2150; keys the loader checks), then the
+// statistics to the caller.
+static int write_properties(const char* dir, const std::string& base, int64_t n, int64_t m, uint64_t graph_bits,
+                            const tgaf_stats& s, tgaf_stats* st) {
   {
     char buf[2048];
     std::snprintf(buf, sizeof buf,
@@ -431,6 +449,17 @@ extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t
   }
   if (st) *st = s;
   return TGAF_OK;
+}
+
+extern "C" int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, int64_t n_min,
+                           const char* dir, const char* name, tgaf_stats* st) {
+  if (!t) return fail(TGAF_EINVAL, "tgaf_encode: bad arguments");
+  return encode_impl(src, dst, t, m, n_min, dir, name, st, nullptr);
+}
+
+extern "C" int tgaf_encode_graph(const int32_t* src, const int32_t* dst, int64_t m, int64_t n_min, const char* dir,
+                                 const char* name, uint64_t* goff, tgaf_stats* st) {
+  return encode_impl(src, dst, nullptr, m, n_min, dir, name, st, goff);
 }
 
 extern "C" int tgaf_build_gpu_index(const char* dir, const char* name, tgaf_stats* st) {

@@ -105,6 +105,14 @@ def lib():
         L.tga_hits_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, vp, vp, C.c_uint32]
         L.tga_profile.argtypes = [vp, C.c_int]
         L.tga_profile_read.argtypes = [vp, C.POINTER(C.c_double), p64, C.c_int]
+        L.tga_encode_timestamps.argtypes = [p32, p64, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.POINTER(vp), pu64]
+        L.tga_encode_gamma_gaps.argtypes = [pu64, C.c_int64, C.c_int, C.POINTER(vp)]
+        L.tga_encode_ef_index.argtypes = [pu64, C.c_int64, C.c_int32, C.c_int, C.POINTER(vp), C.POINTER(vp)]
+        L.tga_stream_bytes.argtypes = [vp]
+        L.tga_stream_bytes.restype = C.c_uint64
+        L.tga_stream_read.argtypes = [vp, C.POINTER(C.c_uint8), C.c_uint64]
+        L.tga_stream_free.argtypes = [vp]
+        L.tga_stream_free.restype = None
         L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
         L.tga_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
         L.tga_neighbor_access_noalloc.argtypes = [vp, p32, C.c_int64, C.POINTER(C.c_uint64), p64]
@@ -506,3 +514,75 @@ class PrServer(_LaneServer):
 
     def admit(self, window=None, iters: int = 20) -> int:
         return self._admit(_win(window), int(iters))
+
+
+# ------------------------------------------------------------ GPU encoder --
+
+def _take_stream(h) -> bytes:
+    """Reads an encoded device stream into bytes and frees it."""
+    try:
+        nbytes = lib().tga_stream_bytes(h)
+        buf = np.empty(max(int(nbytes), 1), np.uint8)
+        _check(lib().tga_stream_read(h, buf.ctypes.data_as(C.POINTER(C.c_uint8)), nbytes))
+        return buf[:nbytes].tobytes()
+    finally:
+        lib().tga_stream_free(h)
+
+
+def encode_timestamps(src, t, min_ts: int, n: int, device: int = 0):
+    """The .timestamps stream of contacts sorted by (src, dst, t), written on
+    the GPU (tga_encode_timestamps).  Returns (bytes, offsets): the stream and
+    the n + 1 bit offsets of every vertex's first code and of the end."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    t = np.ascontiguousarray(t, dtype=np.int64)
+    off = np.empty(n + 1, np.uint64)
+    h = C.c_void_p()
+    _check(lib().tga_encode_timestamps(src.ctypes.data_as(C.POINTER(C.c_int32)), t.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       len(src), int(min_ts), int(n), device, C.byref(h),
+                                       off.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return _take_stream(h), off
+
+
+def encode_gamma_gaps(values, device: int = 0) -> bytes:
+    """gamma codes of values[0] and of every later difference (the .offsets
+    stream of record bit offsets), written on the GPU."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    h = C.c_void_p()
+    _check(lib().tga_encode_gamma_gaps(v.ctypes.data_as(C.POINTER(C.c_uint64)), len(v), device, C.byref(h)))
+    return _take_stream(h)
+
+
+def encode_ef_index(values, l: int, device: int = 0):
+    """The GPU index files (lower, upper) of non-decreasing offsets with
+    lower-bit width l, written on the GPU."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    lo, up = C.c_void_p(), C.c_void_p()
+    _check(lib().tga_encode_ef_index(v.ctypes.data_as(C.POINTER(C.c_uint64)), len(v), int(l), device, C.byref(lo),
+                                     C.byref(up)))
+    return _take_stream(lo), _take_stream(up)
+
+
+def encode_gpu(src, dst, t, directory: str, name: str, n_min: int = 0, device: int = 0):
+    """Writes the dataset files ``directory/name.*`` as fmt.encode does, byte
+    for byte: the graph stream and .properties on the host (the record
+    planner, fmt.encode_graph), then .offsets, .timestamps and the four GPU
+    index files on the GPU.  Returns the format kit's statistics."""
+    from dataclasses import replace
+    from . import fmt
+    t = np.ascontiguousarray(t, dtype=np.int64)
+    goff, st = fmt.encode_graph(src, dst, directory, name, n_min=n_min)
+    n = len(goff) - 1
+    tmin = int(t.min())
+    ts, toff = encode_timestamps(src, t, tmin, n, device)
+    offs = encode_gamma_gaps(goff, device)
+    graph_bytes = os.path.getsize(os.path.join(directory, name + ".graph"))
+    lg = fmt.ef_low_width(graph_bytes, n)
+    lt = fmt.ef_low_width(len(ts), n)
+    glo, gup = encode_ef_index(goff, lg, device)
+    tlo, tup = encode_ef_index(toff, lt, device)
+    for fname, data in ((name + ".timestamps", ts), (name + ".offsets", offs), ("graph_lower_bits.bin", glo),
+                        ("graph_upper_bits.bin", gup), ("timestamp_lower_bits.bin", tlo),
+                        ("timestamp_upper_bits.bin", tup)):
+        with open(os.path.join(directory, fname), "wb") as f:
+            f.write(data)
+    return replace(st, min_timestamp=tmin, timestamps_bits=int(toff[-1]), l_graph=lg, l_timestamps=lt)

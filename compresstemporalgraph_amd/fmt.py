@@ -60,6 +60,8 @@ def _lib():
         p32, p64 = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
         lib.tgaf_last_error.restype = C.c_char_p
         lib.tgaf_encode.argtypes = [p32, p32, p64, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p, C.POINTER(_Stats)]
+        lib.tgaf_encode_graph.argtypes = [p32, p32, C.c_int64, C.c_int64, C.c_char_p, C.c_char_p,
+                                          C.POINTER(C.c_uint64), C.POINTER(_Stats)]
         lib.tgaf_build_gpu_index.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(_Stats)]
         lib.tgaf_gen_rmat.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                       C.c_uint64, C.c_int64, p32, p32, p64]
@@ -103,6 +105,33 @@ def encode(src, dst, t, directory: str, name: str, n_min: int = 0) -> EncodeStat
     _check(_lib().tgaf_encode(_p(src, C.c_int32), _p(dst, C.c_int32), _p(t, C.c_int64), len(src), n_min,
                               directory.encode(), name.encode(), C.byref(st)))
     return _stats(st)
+
+
+def encode_graph(src, dst, directory: str, name: str, n_min: int = 0):
+    """The graph half of ``encode``: ``directory/name.graph`` and
+    ``.properties`` only.  Returns (goff, stats): the n + 1 record bit
+    offsets of the graph stream (uint64) and the statistics (timestamp fields
+    zero)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    if len(src) != len(dst):
+        raise ValueError("src and dst must have equal length")
+    n = max(int(n_min), int(max(src.max(), dst.max())) + 1 if len(src) else 0)
+    os.makedirs(directory, exist_ok=True)
+    goff = np.empty(n + 1, np.uint64)
+    st = _Stats()
+    _check(_lib().tgaf_encode_graph(_p(src, C.c_int32), _p(dst, C.c_int32), len(src), n_min, directory.encode(),
+                                    name.encode(), _p(goff, C.c_uint64), C.byref(st)))
+    return goff, _stats(st)
+
+
+def ef_low_width(stream_bytes: int, n: int) -> int:
+    """Lower-bit width of a GPU index: floor(log2(bits / MAX_LABEL)), MAX_LABEL
+    = n - 1 (bfs.cu:1785-1786); raises for a log2(0)."""
+    q = stream_bytes * 8 // (n - 1)
+    if q == 0:
+        raise FormatError("stream has fewer bits than MAX_LABEL: log2(0) in the index width")
+    return q.bit_length() - 1
 
 
 def build_gpu_index(directory: str, name: str) -> EncodeStats:

@@ -62,6 +62,14 @@ const char* tgaf_last_error(void);
 int tgaf_encode(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, int64_t n_min,
                 const char* dir, const char* name, tgaf_stats* stats);
 
+/* The graph half of tgaf_encode: <dir>/<name>.graph and .properties only, the
+ * record bit offsets of the graph stream into goff (n + 1 entries, n =
+ * max(n_min, largest id + 1)).  The timestamp stream, .offsets and the four
+ * GPU index files are then written on the GPU from goff and the contacts
+ * (tga_encode.h), byte for byte as tgaf_encode writes them. */
+int tgaf_encode_graph(const int32_t* src, const int32_t* dst, int64_t m, int64_t n_min, const char* dir,
+                      const char* name, uint64_t* goff, tgaf_stats* stats);
+
 /* Rebuild the four GPU index files of an existing dataset from its .graph,
  * .offsets and .timestamps (the reference ships no generator for them). */
 int tgaf_build_gpu_index(const char* dir, const char* name, tgaf_stats* stats);

@@ -20,7 +20,8 @@ def declared(header):
     return sorted(set(re.findall(r"\b(tgaf?_[a-z_0-9]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("tga.h", build.ENGINE_LIB), ("tga_format.h", build.FORMAT_LIB)])
+@pytest.mark.parametrize("header,lib", [("tga.h", build.ENGINE_LIB), ("tga_encode.h", build.ENGINE_LIB),
+                                        ("tga_format.h", build.FORMAT_LIB)])
 def test_exports(header, lib):
     assert os.path.exists(lib), f"{lib} not built"
     L = C.CDLL(lib)
