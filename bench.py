@@ -197,6 +197,58 @@ def large_legs(args) -> int:
     return 0
 
 
+def encode_leg(args) -> int:
+    """Times writing a workload's dataset files with the host encoder alone
+    (fmt.encode) and with the split encoder (the graph stream on the host,
+    .offsets, .timestamps and the GPU index files on the GPU:
+    engine.encode_gpu), checks that the two write the same bytes, and prints
+    the times as JSON."""
+    import hashlib
+    import shutil
+    from compresstemporalgraph_amd import datasets, fmt
+    from compresstemporalgraph_amd.engine import encode_gpu
+    root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
+    w = datasets.WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    src, dst, t = datasets.generate(w)
+    gen_s = time.perf_counter() - t0
+    out = {"workload": args.workload, "contacts": int(len(src)), "generate_s": round(gen_s, 2)}
+    dirs = {}
+    for how in ("host", "gpu"):
+        d = os.path.join(root, "encode_" + how)
+        shutil.rmtree(d, ignore_errors=True)
+        os.makedirs(d)
+        t0 = time.perf_counter()
+        if how == "host":
+            fmt.encode(src, dst, t, d, args.workload, n_min=w.n)
+        else:
+            encode_gpu(src, dst, t, d, args.workload, n_min=w.n)
+        out[how + "_s"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] {how} encoder: {out[how + '_s']} s")
+        dirs[how] = d
+    # the graph half alone, as the split encoder runs it on the host
+    d = os.path.join(root, "encode_graph_only")
+    shutil.rmtree(d, ignore_errors=True)
+    t0 = time.perf_counter()
+    fmt.encode_graph(src, dst, d, args.workload, n_min=w.n)
+    out["host_graph_half_s"] = round(time.perf_counter() - t0, 2)
+    same = True
+    for f in sorted(os.listdir(dirs["host"])):
+        digests = []
+        for how in ("host", "gpu"):
+            h = hashlib.sha256()
+            with open(os.path.join(dirs[how], f), "rb") as fh:
+                for chunk in iter(lambda: fh.read(1 << 24), b""):
+                    h.update(chunk)
+            digests.append(h.hexdigest())
+        same &= digests[0] == digests[1]
+    out["identical_files"] = same
+    for d in list(dirs.values()) + [d]:
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
 def batched_bfs(g, t_lo: int, t_hi: int, nq: int = 64) -> dict:
     """64 BFS queries with random sources and windows whose lengths are
     log-uniform in [2^10, 2^24] and starts uniform (the batching mix's BFS
@@ -670,6 +722,9 @@ def main() -> int:
     ap.add_argument("--mix-only", action="store_true",
                     help="with --query-legs: only the 256-query batched mix and the served mix at 100/s, 1k/s "
                          "and 10k/s")
+    ap.add_argument("--encode-leg", action="store_true",
+                    help="only time the host and the split (host graph + GPU streams) encoders on --workload, "
+                         "check they write the same bytes, and print their JSON")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -698,6 +753,8 @@ def main() -> int:
 
     if args.query_legs:
         return query_legs(args)
+    if args.encode_leg:
+        return encode_leg(args)
     if args.large_legs:
         return large_legs(args)
 
@@ -758,7 +815,7 @@ def main() -> int:
         side_rec = rec
 
     t0 = time.perf_counter()
-    desc = datasets.ensure(root, args.workload)
+    desc = datasets.ensure(root, args.workload, gpu=args.workload == "netflow25")  # GPU writers for the 1B graph
     log(f"[bench] dataset {args.workload}: n={desc['max_vertex']} contacts={desc['contacts']} "
         f"ready in {time.perf_counter() - t0:.1f} s")
     ds_dir = datasets.dataset_dir(root, args.workload)

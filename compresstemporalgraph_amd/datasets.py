@@ -71,11 +71,17 @@ def generate(w: Workload):
     raise ValueError(w.kind)
 
 
-def write_dataset(root: str, name: str, src, dst, t, n_min: int = 0, raw: bool = False) -> dict:
-    """Encodes a contact list and writes the dataset plus a descriptor JSON."""
+def write_dataset(root: str, name: str, src, dst, t, n_min: int = 0, raw: bool = False, gpu: bool = False) -> dict:
+    """Encodes a contact list and writes the dataset plus a descriptor JSON;
+    with gpu, the per-contact streams and the index files are written on the
+    GPU (engine.encode_gpu, the same bytes)."""
     d = dataset_dir(root, name)
     os.makedirs(d, exist_ok=True)
-    st = fmt.encode(src, dst, t, d, name, n_min=n_min)
+    if gpu:
+        from .engine import encode_gpu
+        st = encode_gpu(src, dst, t, d, name, n_min=n_min)
+    else:
+        st = fmt.encode(src, dst, t, d, name, n_min=n_min)
     if raw:
         os.makedirs(os.path.dirname(raw_path(root, name)), exist_ok=True)
         fmt.write_raw_gz(raw_path(root, name), src, dst, t)
@@ -97,8 +103,9 @@ def write_dataset(root: str, name: str, src, dst, t, n_min: int = 0, raw: bool =
     return desc
 
 
-def ensure(root: str, name: str) -> dict:
-    """Generates workload ``name`` under ``root`` unless it is already there."""
+def ensure(root: str, name: str, gpu: bool = False) -> dict:
+    """Generates workload ``name`` under ``root`` unless it is already there
+    (gpu: encode with the GPU writers, as write_dataset)."""
     d = dataset_dir(root, name)
     meta = os.path.join(d, name + ".dataset.json")
     if os.path.exists(meta):
@@ -106,7 +113,7 @@ def ensure(root: str, name: str) -> dict:
             return json.load(f)
     w = WORKLOADS[name]
     src, dst, t = generate(w)
-    return write_dataset(root, name, src, dst, t, n_min=w.n, raw=w.write_raw)
+    return write_dataset(root, name, src, dst, t, n_min=w.n, raw=w.write_raw, gpu=gpu)
 
 
 def load_descriptor(root: str, name: str) -> dict:
