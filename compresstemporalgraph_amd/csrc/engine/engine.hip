@@ -136,6 +136,14 @@ __device__ __forceinline__ int32_t item_index(int32_t code, int32_t n) { return 
 
 // ------------------------------------------------------- schedule kernels ---
 
+// Stream words in place: the file's bytes were copied into the word array,
+// and each word's bytes are swapped so that its most significant byte is the
+// first stream byte (the layout of decode.hpp).
+__global__ void __launch_bounds__(kBlock) k_bswap_words(uint64_t* w, int64_t nw) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nw; i += (int64_t)gridDim.x * kBlock)
+    w[i] = __builtin_bswap64(w[i]);
+}
+
 // Record header of every vertex: outdegree and reference distance.
 // Also fills the record start table (Streams::vpos) from the two
 // Elias-Fano indexes.
@@ -2723,6 +2731,22 @@ struct tga_graph {
     if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     return TGA_OK;
   }
+  // A stream file's bytes as device words (decode.hpp layout) plus
+  // kPadWords zero words: the bytes are copied as they are and swapped on the
+  // device.
+  int upload_stream(const uint64_t*& dst, const std::vector<uint8_t>& bytes) {
+    const size_t nw = (bytes.size() + 7) / 8;
+    uint64_t* p = nullptr;
+    int rc = alloc(p, nw + kPadWords);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(p, 0, (nw + kPadWords) * 8, stream));
+    if (!bytes.empty()) HIP_TRY(hipMemcpyAsync(p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, stream));
+    if (nw) k_bswap_words<<<grid((int64_t)nw), kBlock, 0, stream>>>(p, (int64_t)nw);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));  // the copy reads pageable host memory
+    dst = p;
+    return TGA_OK;
+  }
   template <class T>
   int upload_const(const T*& dst, const std::vector<T>& src) {
     T* p = nullptr;
@@ -2897,8 +2921,10 @@ static int scratch_for(int device, NoAllocScratch** out);
 constexpr int64_t kHostCkptMin = 1 << 16;
 
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
-// HS holds host copies of the streams and indexes (its vpos unset).
-static int build_schedule(tga_graph* g, Streams HS, int talk) {
+// HS holds host copies of the indexes (its streams and vpos unset); graph and
+// ts are the stream files' bytes.
+static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& graph, const std::vector<uint8_t>& ts,
+                          int talk) {
   const int64_t n = g->n;
   Streams& S = g->S;
   int32_t* d_deg = nullptr;
@@ -3026,6 +3052,31 @@ static int build_schedule(tga_graph* g, Streams HS, int talk) {
         hvpos[2 * (size_t)v + 1] = ef_get(HS, HS.ti, (uint64_t)v);
       }
     HS.vpos = hvpos.get();
+    // Host words of the streams, converted only over the host records (and a
+    // few words past each, which bit buffers read ahead); the rest of the
+    // arrays is never read.
+    const size_t gwn = (size_t)S.g_last + 1, twn = (size_t)S.t_last + 1;
+    std::unique_ptr<uint64_t[]> hg(new uint64_t[gwn]), ht(new uint64_t[twn]);
+    auto words = [](uint64_t* w, size_t nwords, const std::vector<uint8_t>& b, uint64_t bit0, uint64_t bit1) {
+      // from kPadWords words before (a buffer started near the stream's end
+      // is moved back to stay inside it) to 8 words past
+      const size_t lo = (size_t)(bit0 >> 6) > kPadWords ? (size_t)(bit0 >> 6) - kPadWords : 0;
+      const size_t hi = std::min(nwords, (size_t)(bit1 >> 6) + 8);
+      for (size_t i = lo; i < hi; ++i) {
+        uint64_t v = 0;
+        for (size_t k = 0; k < 8; ++k) v = (v << 8) | (i * 8 + k < b.size() ? b[i * 8 + k] : 0);
+        w[i] = v;
+      }
+    };
+    for (const int32_t x : host_x) {
+      const size_t v = (size_t)x, v1 = (size_t)std::min<int64_t>(x + 1, n - 1);
+      const uint64_t gend = x + 1 < n ? hvpos[2 * v1] : (uint64_t)graph.size() * 8;
+      const uint64_t tend = x + 1 < n ? hvpos[2 * v1 + 1] : (uint64_t)ts.size() * 8;
+      words(hg.get(), gwn, graph, hvpos[2 * v], gend);
+      words(ht.get(), twn, ts, hvpos[2 * v + 1], tend);
+    }
+    HS.g = hg.get();
+    HS.t = ht.get();
     int64_t hslots = 0;
     for (const int32_t x : host_x) hslots += (deg[(size_t)x] + S.flat_stride - 1) / S.flat_stride;
     // slots of the host records are not contiguous: build into one host block
@@ -3211,7 +3262,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   if (desc && desc->min_timestamp != TGA_FROM_FILE && desc->min_timestamp != min_ts)
     return fail(TGA_EINVAL, "minTimestamp in the descriptor does not match the .timestamps header");
 
-  const std::vector<uint64_t> gw = be_words(graph), tw = be_words(ts), glw = be_words(glo), tlw = be_words(tlo);
+  const std::vector<uint64_t> glw = be_words(glo), tlw = be_words(tlo);
   const std::vector<uint64_t> guw = le_words(gup), tuw = le_words(tup);
   phase("words");
   uint64_t gones = 0, tones = 0;
@@ -3246,7 +3297,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   g->min_ts = (int32_t)min_ts;
 
   Streams& S = g->S;
-  if ((rc = g->upload_const(S.g, gw)) || (rc = g->upload_const(S.t, tw)) || (rc = g->upload_const(S.gi.lower, glw)) ||
+  if ((rc = g->upload_stream(S.g, graph)) || (rc = g->upload_stream(S.t, ts)) || (rc = g->upload_const(S.gi.lower, glw)) ||
       (rc = g->upload_const(S.gi.upper, guw)) || (rc = g->upload_const(S.gi.samp, gsamp)) ||
       (rc = g->upload_const(S.gi.spill, gspill)) || (rc = g->upload_const(S.ti.lower, tlw)) ||
       (rc = g->upload_const(S.ti.upper, tuw)) || (rc = g->upload_const(S.ti.samp, tsamp)) ||
@@ -3255,10 +3306,11 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   S.g_bits = (uint64_t)graph.size() * 8;
   S.t_bits = (uint64_t)ts.size() * 8;
   // Bit buffers index stream words with 32 bits (streams up to 32 GiB).
-  if (gw.size() >= 0xffffffffull || tw.size() >= 0xffffffffull)
+  const size_t gwords = (graph.size() + 7) / 8 + kPadWords, twords = (ts.size() + 7) / 8 + kPadWords;
+  if (gwords >= 0xffffffffull || twords >= 0xffffffffull)
     return bail(fail(TGA_ERANGE, "stream longer than 2^32 words"));
-  S.g_last = (uint32_t)(gw.size() - 1);
-  S.t_last = (uint32_t)(tw.size() - 1);
+  S.g_last = (uint32_t)(gwords - 1);
+  S.t_last = (uint32_t)(twords - 1);
   S.gi.upper_words = (gup.size() + 7) / 8;
   S.ti.upper_words = (tup.size() + 7) / 8;
   S.gi.l = lg;
@@ -3272,16 +3324,16 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   phase("device+upload");
 
     {
-    Streams HS = S;  // the same fields over the host copies
-    HS.g = gw.data();
-    HS.t = tw.data();
+    Streams HS = S;  // the same fields over host copies (streams: built where needed)
+    HS.g = nullptr;
+    HS.t = nullptr;
     HS.gi = EFDir{glw.data(), guw.data(), gsamp.data(), gspill.data(), S.gi.upper_words, lg};
     HS.ti = EFDir{tlw.data(), tuw.data(), tsamp.data(), tspill.data(), S.ti.upper_words, lt};
     HS.err = nullptr;
     HS.ck = nullptr;
     HS.vinfo = nullptr;
     HS.vpos = nullptr;
-    if ((rc = build_schedule(g, HS, talk ? 1 : 0))) return bail(rc);
+    if ((rc = build_schedule(g, HS, graph, ts, talk ? 1 : 0))) return bail(rc);
   }
   if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipStreamSynchronize failed"));
   phase("schedule");
