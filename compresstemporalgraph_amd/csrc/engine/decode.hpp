@@ -834,6 +834,14 @@ TGA_HD int32_t dec_next_run(const Streams& S, Dec<MAXD>& D, int32_t& cnt) {
 // (dword index mod kStageDw), slots kStageDw and kStageDw + 1 mirroring slots
 // 0 and 1 so a 96-bit read never wraps; slot i of a lane at s[i * stride]
 // (lanes interleaved: every lane of a wave hits its own bank).
+//
+// The units come either from the linear stream (uw = the stream words,
+// ustride 1, positions absolute) or from a column copy of the item's codes
+// made at open (uw = the item's first unit, ustride = the units between two
+// of its consecutive units, positions relative to the item's first code,
+// nbits its code bits): there the 64 items a wave walks together have their
+// units interleaved, so the wave's top-up loads fall into a few adjacent
+// lines instead of 64 separate streams.
 constexpr int32_t kStageDw = 16;  // staged dwords per lane (power of two, multiple of 4)
 struct TsStage {
   uint32_t* s;
@@ -843,19 +851,24 @@ struct TsStage {
   uint32_t pos;   // bit offset of the next code from dword `base`
   uint32_t wend;  // relative dword index one past the last staged dword
   int32_t prev;   // the timestamp before the next code
+  const uint64_t* uw = nullptr;  // unit source (word pairs); nullptr: set to the linear stream by stage_init
+  uint32_t ustride = 1;          // unit pitch of the source, in units
+  uint32_t nbits = 0;            // column source: the item's code bits
 };
 
-// Stages the 16-byte unit at relative dword wend (stream words 2u, 2u + 1)
-// with one 16-byte load.  A unit is staged only while its first dword lies
-// before the item's end (stage_fill), and the last word of a unit then lies
-// within the stream's five words of padding, so no bound check is needed.
+// Stages the 16-byte unit at relative dword wend (source words 2u, 2u + 1 of
+// unit u, at unit u * ustride) with one 16-byte load.  A unit is staged only
+// while its first dword lies before the item's end (stage_fill); in the
+// linear stream the last word of a unit then lies within the stream's five
+// words of padding, and a column holds every unit its item's codes touch, so
+// no bound check is needed.
 TGA_HD void stage_unit(const Streams& S, TsStage& st) {
-  const uint64_t u = (st.base + st.wend) >> 2;
+  const uint64_t u = ((st.base + st.wend) >> 2) * st.ustride;
 #if defined(__HIP_DEVICE_COMPILE__)
-  const uint4 v = reinterpret_cast<const uint4*>(S.t)[u];
+  const uint4 v = reinterpret_cast<const uint4*>(st.uw)[u];
   const uint32_t d0 = v.y, d1 = v.x, d2 = v.w, d3 = v.z;  // each word's high dword first
 #else
-  const uint64_t w0 = S.t[2 * u], w1 = S.t[2 * u + 1];
+  const uint64_t w0 = st.uw[2 * u], w1 = st.uw[2 * u + 1];
   const uint32_t d0 = (uint32_t)(w0 >> 32), d1 = (uint32_t)w0, d2 = (uint32_t)(w1 >> 32), d3 = (uint32_t)w1;
 #endif
   const uint32_t slot = st.wend & (kStageDw - 1);
@@ -875,10 +888,22 @@ TGA_HD void stage_unit(const Streams& S, TsStage& st) {
 // first unit holds the code at pos).
 TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end, int32_t prev) {
   if (pos >= S.t_bits) raise_err(S, kErrOverrun);
+  st.uw = S.t;
+  st.ustride = 1;
   st.base = (pos >> 5) & ~(uint64_t)3;
   st.pos = (uint32_t)(pos - st.base * 32);
   st.wend = 0;
   st.lim = (end + 31) >> 5;
+  st.prev = prev;
+}
+
+// Starts the window at the first code of a column source (uw, ustride and
+// nbits set by the caller).
+TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
+  st.base = 0;
+  st.pos = 0;
+  st.wend = 0;
+  st.lim = ((uint64_t)st.nbits + 31) >> 5;
   st.prev = prev;
 }
 
@@ -1055,7 +1080,9 @@ TGA_HD void act_item(Act&, int32_t, long) {}
 
 // Runs one work item.  code >= 0: the whole record of vertex `code`;
 // code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked,
-// split) may veto the item; act.end(x, chunked) runs after the walk.
+// split) may veto the item; act.end(x, chunked) runs after the walk.  The
+// timestamps come from the linear stream unless ts.uw names the item's
+// column copy.
 template <int MAXD, class Act>
 TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, RunRing ring, TsStage ts) {
   Dec<MAXD> D;
@@ -1073,7 +1100,10 @@ TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, R
       act.end(x, false);
       return;
     }
-    stage_init(S, ts, tpos, rec_tend(S, x), S.min_ts);
+    if (ts.uw)
+      stage_init_col(ts, S.min_ts);
+    else
+      stage_init(S, ts, tpos, rec_tend(S, x), S.min_ts);
   } else {
     const Ckpt* c = ck + (-code - 1);
     x = c->x;
@@ -1081,7 +1111,10 @@ TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, R
     k = c->k0;
     kend = c->k1;
     restore_levels<0, 0, MAXD>(S, D, c->lv, c->depth);
-    stage_init(S, ts, c->tpos, rec_tend(S, x), c->tprev);
+    if (ts.uw)
+      stage_init_col(ts, c->tprev);
+    else
+      stage_init(S, ts, c->tpos, rec_tend(S, x), c->tprev);
   }
   walk<MAXD>(S, D, ts, x, k, kend, act, ring);
   act.end(x, chunked);

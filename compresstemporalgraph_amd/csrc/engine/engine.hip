@@ -110,10 +110,34 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Accumulator pairs of the neighbor-access checksum and count.
 constexpr int kSumSlots = 256;
 
-// An item list and its length.
+// Column copy of the timestamp codes of a static item list (built at open,
+// tcol_build): list positions 64q .. 64q + 63 -- the items one wave walks
+// together, as FOR_ITEMS hands them out with whole waves of consecutive
+// positions -- form group q, whose region starts at unit goff[q] and holds
+// rows of 64 16-byte units: unit u of position 64q + l at goff[q] + 64u + l,
+// the item's code bits from bit 0 of its unit 0 on.  A group has as many rows
+// as its longest item needs.
+struct TsCols {
+  const uint64_t* w = nullptr;     // units as word pairs (stream word layout)
+  const uint64_t* goff = nullptr;  // per group: first unit
+  const uint32_t* nbits = nullptr; // per list position: the item's code bits
+};
+
+// An item list and its length; static lists carry their column copy.
 struct Items {
   const int32_t* code;
   uint32_t count;
+  TsCols tc{};
+  // The lane's timestamp window for list position i: from the column copy
+  // where the list has one, else from the linear stream.
+  __device__ __forceinline__ TsStage stage(TsStage st, uint32_t i) const {
+    if (tc.w) {
+      st.uw = tc.w + 2 * (tc.goff[i >> 6] + (i & 63));
+      st.ustride = 64;
+      st.nbits = tc.nbits[i];
+    }
+    return st;
+  }
 };
 
 // The calling lane's ring of decoded runs and its timestamp window
@@ -142,6 +166,43 @@ __device__ __forceinline__ int32_t item_index(int32_t code, int32_t n) { return 
 __global__ void __launch_bounds__(kBlock) k_bswap_words(uint64_t* w, int64_t nw) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < nw; i += (int64_t)gridDim.x * kBlock)
     w[i] = __builtin_bswap64(w[i]);
+}
+
+// First timestamp bit and code bits of every position of a static item list
+// (a whole record: its record's codes; a chunk: from its checkpoint's cursor
+// to the next chunk's of the same record, or to the record's end).
+__global__ void __launch_bounds__(kBlock) k_tcol_len(Streams S, const Ckpt* ck, int64_t nck, Items I, uint64_t* tstart,
+                                                     uint32_t* nbits) {
+  FOR_ITEMS(I, i) {
+    const int32_t code = I.code[i];
+    uint64_t b, e;
+    if (code >= 0) {
+      b = S.vpos[2 * (size_t)code + 1];
+      e = rec_tend(S, code);
+    } else {
+      const int64_t j = -(int64_t)code - 1;
+      const Ckpt& c = ck[j];
+      b = c.tpos;
+      e = (j + 1 < nck && ck[j + 1].x == c.x && ck[j + 1].k0 == c.k1) ? ck[j + 1].tpos : rec_tend(S, c.x);
+    }
+    tstart[i] = b;
+    nbits[i] = (uint32_t)(e - b);
+  }
+}
+
+// Copies every item's codes into its column (TsCols): unit u of position i
+// holds the 128 stream bits from tstart[i] + 128u on, as two stream-layout
+// words.  Units past an item's last code are not written (never read).
+__global__ void __launch_bounds__(kBlock) k_tcol_fill(Streams S, Items I, const uint64_t* tstart, uint64_t* w) {
+  FOR_ITEMS(I, i) {
+    const uint64_t b = tstart[i];
+    const uint32_t nb = I.tc.nbits[i];
+    uint64_t* out = w + 2 * (I.tc.goff[i >> 6] + (i & 63));
+    for (uint32_t u = 0; 128ull * u < nb; ++u) {
+      out[128ull * u] = peek64(S.t, b + 128ull * u);
+      out[128ull * u + 1] = peek64(S.t, b + 128ull * u + 64);
+    }
+  }
 }
 
 // Record header of every vertex: outdegree and reference distance.
@@ -364,7 +425,7 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_ra(Streams S, const Ckpt* ck, Items I, unsigned long long* out) {
   RaAct a;
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
   const unsigned long long cs = wave_sum((unsigned long long)a.cs);
   const unsigned long long cnt = wave_sum((unsigned long long)a.cnt);
   // Spread over kSumSlots accumulator pairs: tens of thousands of waves
@@ -410,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_ctimes(Streams S, const Ckpt* ck, It
                                                    const uint64_t* coff) {
   CtAct a{W, ct, coff};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Degree count and Java running mean of one record (whole records also get
@@ -463,7 +524,7 @@ __global__ void __launch_bounds__(kBlock) k_dc(Streams S, const Ckpt* ck, Items 
                                                int32_t* centr) {
   DcAct a{W, count, centr};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Java's long division a / k (truncating toward zero) for k >= 1 and
@@ -719,7 +780,7 @@ __global__ void __launch_bounds__(kBlock) k_runs(Streams S, const Ckpt* ck, Item
                                                  const uint64_t* off, int32_t* rx, int32_t* rr, int32_t* rk) {
   RunAct a{A, cnt, off, rx, rr, rk, S.n};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Run ends (the next run of the record, or its outdegree) and sort keys
@@ -1532,7 +1593,7 @@ __global__ void __launch_bounds__(kBlock) k_bfs(Streams S, const Ckpt* ck, Items
                                                 int32_t lvl) {
   BfsAct a{W, level, lvl};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Min-label propagation: a vertex whose label drops in round r is stamped r
@@ -1563,7 +1624,7 @@ __global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items 
                                                int32_t* stamp, int32_t round) {
   CcAct a{W, label, stamp, round};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
@@ -1616,7 +1677,7 @@ __global__ void __launch_bounds__(kBlock) k_gc(Streams S, const Ckpt* ck, Items 
                                                int32_t* color, int32_t* blocked, int32_t* blocker, int32_t round) {
   GcAct a{W, prio, color, blocked, blocker, round};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 __global__ void __launch_bounds__(kBlock) k_gc_finish(const int32_t* bigq, uint32_t nbig, Window W, int32_t* color,
@@ -1687,7 +1748,7 @@ __global__ void __launch_bounds__(kBlock) k_gc_list(Streams S, const Ckpt* ck, I
                                                     uint32_t* cnt, const uint64_t* off, int32_t* hp) {
   GcListAct a{W, prio, cnt, off, hp, S.n};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Exclusive prefix of v over the block (thread order); total receives the sum.
@@ -1945,7 +2006,7 @@ __global__ void __launch_bounds__(kBlock) k_bfsb(Streams S, const Ckpt* ck, Item
   __syncthreads();
   BfsbAct a{W, front, vis, next, bound, mask, tab.nb};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Ends step `lvl` of a batched BFS: the bits newly set in next[v] become the
@@ -2058,7 +2119,7 @@ __global__ void __launch_bounds__(kBlock) k_ea(Streams S, const Ckpt* ck, Items 
                                                int32_t* stamp, int32_t round, uint64_t* runkey) {
   EaAct a{W, arr, stamp, round, runkey};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Resolved split runs: the slot's pair (x, r) and its first window contact t.
@@ -2126,7 +2187,7 @@ __global__ void __launch_bounds__(kBlock) k_ld(Streams S, const Ckpt* ck, Items 
                                                uint32_t* changed) {
   LdAct a{W, ld, changed};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Window contact count (PageRank divisor).
@@ -2155,7 +2216,7 @@ template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_wdeg(Streams S, const Ckpt* ck, Items I, Window W, int32_t* du) {
   WdegAct a{W, du};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 __device__ __forceinline__ float pr_coef(float damping, const float* rank, const int32_t* du, int32_t x) {
@@ -2199,7 +2260,7 @@ __global__ void __launch_bounds__(kBlock) k_pr_push(Streams S, const Ckpt* ck, I
                                                     const int32_t* du, float damping, double* acc, uint64_t* runkey) {
   PrAct a{W, rank, du, damping, acc, runkey};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 __global__ void __launch_bounds__(kBlock) k_pr_fix(const Ckpt* ck, int32_t nck, uint64_t* runkey, Window W,
@@ -2245,7 +2306,7 @@ __global__ void __launch_bounds__(kBlock) k_hits_pull(Streams S, const Ckpt* ck,
                                                       double* acc, uint64_t* runkey) {
   HubAct a{W, auth, acc, runkey};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 struct AuthAct {
@@ -2278,7 +2339,7 @@ __global__ void __launch_bounds__(kBlock) k_hits_push(Streams S, const Ckpt* ck,
                                                       double* acc, uint64_t* runkey) {
   AuthAct a{W, hub, acc, runkey};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // mode 0: hub[x] += auth[r]; mode 1: auth[r] += hub[x] (per resolved split run).
@@ -2365,7 +2426,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Streams S, const Ckpt* ck, Ite
                                                   uint64_t* runkey) {
   PairAct a{W, cnt, off, key, val, runkey, S.n};
   LANE_RING();
-  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, stage_);
+  FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
 }
 
 // Split-run pairs into key/val[0, nck) (one slot per checkpoint).  An empty
@@ -2543,6 +2604,7 @@ struct tga_graph {
   int32_t* vdeg = nullptr;   // per vertex outdegree
   int32_t* items[kGroups] = {};  // static item lists by group
   uint32_t n_items[kGroups] = {};
+  TsCols tcols[kGroups];  // their timestamp codes in column order (TGA_TCOL=0: none)
   int32_t* big[kClasses] = {};  // chunked records by class
   int32_t n_big[kClasses] = {};
   // scratch (n-sized unless stated)
@@ -3207,6 +3269,48 @@ static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& 
   return TGA_OK;
 }
 
+// Builds the column copy of the timestamp codes of every static item list
+// (TsCols): per-position bit ranges on the device, group rows and offsets on
+// the host, then the copy.  TGA_TCOL=0 leaves the lists on the linear stream.
+static int tcol_build(tga_graph* g) {
+  if (const char* e = std::getenv("TGA_TCOL"))
+    if (*e == '0') return TGA_OK;
+  for (int c = 0; c < kGroups; ++c) {
+    const uint32_t ni = g->n_items[c];
+    if (ni == 0) continue;
+    const Items I{g->items[c], ni};
+    uint64_t* tstart = nullptr;
+    uint32_t* nbits = nullptr;
+    int rc;
+    if ((rc = g->alloc(tstart, ni)) || (rc = g->alloc(nbits, ni))) return rc;
+    k_tcol_len<<<g->grid(ni), kBlock, 0, g->stream>>>(g->S, g->ck, g->nck, I, tstart, nbits);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> hb(ni);
+    HIP_TRY(hipMemcpyAsync(hb.data(), nbits, (size_t)ni * 4, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const size_t ng = ((size_t)ni + 63) / 64;
+    std::vector<uint64_t> goff(ng + 1);
+    uint64_t at = 0;
+    for (size_t q = 0; q < ng; ++q) {
+      uint32_t rows = 0;
+      for (size_t i = 64 * q; i < std::min<size_t>(ni, 64 * q + 64); ++i) rows = std::max(rows, (hb[i] + 127) / 128);
+      goff[q] = at;
+      at += 64ull * rows;
+    }
+    goff[ng] = at;
+    uint64_t* w = nullptr;
+    uint64_t* dgoff = nullptr;
+    if ((rc = g->alloc(w, 2 * std::max<uint64_t>(at, 1))) || (rc = g->upload(dgoff, goff))) return rc;
+    const TsCols tc{w, dgoff, nbits};
+    k_tcol_fill<<<g->grid(ni), kBlock, 0, g->stream>>>(g->S, Items{g->items[c], ni, tc}, tstart, w);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->release(tstart);
+    g->tcols[c] = tc;
+  }
+  return TGA_OK;
+}
+
 static int open_impl(const std::string& dir, const std::string& name, const tga_dataset_desc* desc, int device,
                      tga_graph** out) {
   if (!out) return fail(TGA_EINVAL, "tga_open: out is NULL");
@@ -3337,6 +3441,8 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   }
   if (hipStreamSynchronize(g->stream) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipStreamSynchronize failed"));
   phase("schedule");
+  if ((rc = tcol_build(g))) return bail(rc);
+  phase("timestamp columns");
 
   const size_t qn = (size_t)n + (size_t)g->nck;
   if ((rc = g->alloc(g->q0, (size_t)n)) || (rc = g->alloc(g->i0, (size_t)n)) ||
@@ -3419,7 +3525,7 @@ static bool bad_window(tga_window w) { return w.start > w.end; }
     HIP_TRY(hipGetLastError());                                                             \
   } while (0)
 
-#define STATIC_ITEMS(c) (Items{g->items[c], g->n_items[c]})
+#define STATIC_ITEMS(c) (Items{g->items[c], g->n_items[c], g->tcols[c]})
 #define QUEUED_ITEMS(c) (Items{g->xq.q[c], hc_[c]})
 #define LAUNCH_STATIC(K, ...) LAUNCH_CLASSES(K, STATIC_ITEMS, __VA_ARGS__)
 #define LISTED_ITEMS(c) (Items{qs_.q[c], hc_[c]})

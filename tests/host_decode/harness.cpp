@@ -11,7 +11,8 @@
 // past a stream aborts or changes the output, instead of being masked by
 // whatever a register held.
 //
-// Usage: harness <dataset dir> <basename> <out file> [flat chunk stride [burst]]
+// Usage: harness <dataset dir> <basename> <out file> [flat chunk stride [burst [col]]]
+// (col = 1: timestamps from the column copy the engine builds at open)
 // Writes, per vertex, the decoded contact count (int64) and the sum of the
 // neighbor-access checksum terms of its contacts (uint64), then the error
 // word (int64).  Exit status 0 when the walk finished; the caller compares
@@ -72,7 +73,7 @@ struct Act {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 4 || argc > 6) return die("usage: harness <dir> <basename> <out> [flat stride [burst]]");
+  if (argc < 4 || argc > 7) return die("usage: harness <dir> <basename> <out> [flat stride [burst [col]]]");
   const std::string dir = argv[1], name = argv[2], base = dir + "/" + name;
   std::vector<uint8_t> graph, ts, glo, gup, tlo, tup;
   const std::pair<std::string, std::vector<uint8_t>*> files[] = {
@@ -145,8 +146,9 @@ int main(int argc, char** argv) {
   if (err) return die("decode error in the headers", err);
   int64_t contacts = 0;
   for (int64_t x = 0; x < n; ++x) contacts += deg[(size_t)x];
-  S.flat_stride = argc >= 5 ? std::atoi(argv[4]) : flat_stride_for(contacts);
+  S.flat_stride = argc >= 5 && std::atoi(argv[4]) > 0 ? std::atoi(argv[4]) : flat_stride_for(contacts);
   S.burst = argc >= 6 ? std::atoi(argv[5]) : kBurstDefault;
+  const bool col = argc >= 7 && std::atoi(argv[6]) != 0;
 
   // Checkpoints of long records, class by class (a record's references have
   // smaller depth, so their checkpoints exist when it is built).
@@ -186,21 +188,77 @@ int main(int argc, char** argv) {
   std::vector<uint32_t> stage_store(kStageDw + 2);
   const RunRing ring{ring_store.data(), 1};
   const TsStage stage{stage_store.data(), 1, 0, 0, 0, 0, 0};
-  auto run = [&](int32_t code, int c) {
-    Act a{&cnt, &sum, &bad};
-    if (c == 0) run_item<0>(S, ck.data(), code, a, ring, stage);
-    else if (c == 1) run_item<1>(S, ck.data(), code, a, ring, stage);
-    else run_item<kMaxDepth>(S, ck.data(), code, a, ring, stage);
-  };
+  std::vector<int32_t> list, lgroup;  // items in vertex order (chunks in order) and their launch group
   for (int64_t x = 0; x < n; ++x) {
     if (deg[(size_t)x] == 0) continue;
     const int c = depth[(size_t)x];
     if (slot[(size_t)x] < 0) {
-      run((int32_t)x, c);
+      list.push_back((int32_t)x);
+      lgroup.push_back(c);
     } else {
       const int32_t nch = ck[(size_t)slot[(size_t)x]].nchunk;
-      for (int32_t j = 0; j < nch; ++j) run(-(int32_t)(slot[(size_t)x] + j) - 1, c);
+      for (int32_t j = 0; j < nch; ++j) {
+        list.push_back(-(int32_t)(slot[(size_t)x] + j) - 1);
+        lgroup.push_back(c);
+      }
     }
+  }
+  // With `col`, the timestamps come from a column copy of the list built as
+  // the engine builds it at open (k_tcol_len, k_tcol_fill: groups of 64
+  // positions, unit u of position 64q + l at goff[q] + 64u + l); units past
+  // an item's codes hold all-ones words, so a walk that read them would
+  // decode wrong times.
+  std::vector<uint64_t> cw, goff;
+  std::vector<uint32_t> nbits(list.size());
+  if (col) {
+    std::vector<uint64_t> tstart(list.size());
+    const int64_t nck = (int64_t)slots;
+    for (size_t i = 0; i < list.size(); ++i) {
+      const int32_t code = list[i];
+      uint64_t b, e;
+      if (code >= 0) {
+        b = vpos[2 * (size_t)code + 1];
+        e = rec_tend(S, code);
+      } else {
+        const int64_t j = -(int64_t)code - 1;
+        b = ck[(size_t)j].tpos;
+        e = (j + 1 < nck && ck[(size_t)j + 1].x == ck[(size_t)j].x && ck[(size_t)j + 1].k0 == ck[(size_t)j].k1)
+                ? ck[(size_t)j + 1].tpos
+                : rec_tend(S, ck[(size_t)j].x);
+      }
+      tstart[i] = b;
+      nbits[i] = (uint32_t)(e - b);
+    }
+    const size_t ng = (list.size() + 63) / 64;
+    goff.resize(ng + 1);
+    uint64_t at = 0;
+    for (size_t q = 0; q < ng; ++q) {
+      uint32_t rows = 0;
+      for (size_t i = 64 * q; i < std::min(list.size(), 64 * q + 64); ++i) rows = std::max(rows, (nbits[i] + 127) / 128);
+      goff[q] = at;
+      at += 64ull * rows;
+    }
+    goff[ng] = at;
+    cw.assign(2 * (size_t)at, ~0ull);
+    for (size_t i = 0; i < list.size(); ++i)
+      for (uint64_t u = 0; 128 * u < nbits[i]; ++u) {
+        const size_t k = 2 * (size_t)(goff[i / 64] + 64 * u + (i % 64));
+        cw[k] = peek64(S.t, tstart[i] + 128 * u);
+        cw[k + 1] = peek64(S.t, tstart[i] + 128 * u + 64);
+      }
+  }
+  for (size_t i = 0; i < list.size(); ++i) {
+    Act a{&cnt, &sum, &bad};
+    TsStage st = stage;
+    if (col) {
+      st.uw = cw.data() + 2 * (goff[i / 64] + (i % 64));
+      st.ustride = 64;
+      st.nbits = nbits[i];
+    }
+    const int c = lgroup[i];
+    if (c == 0) run_item<0>(S, ck.data(), list[i], a, ring, st);
+    else if (c == 1) run_item<1>(S, ck.data(), list[i], a, ring, st);
+    else run_item<kMaxDepth>(S, ck.data(), list[i], a, ring, st);
   }
   if (bad) return die("walk order / split-run links broken in items:", bad);
 

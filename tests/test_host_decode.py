@@ -45,10 +45,12 @@ def harness(request, tmp_path_factory):
     return out
 
 
-def _run(harness, ds, tmp_path, stride=None, burst=None):
+def _run(harness, ds, tmp_path, stride=None, burst=None, col=False):
     out = str(tmp_path / f"{ds.name}.bin")
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    cmd = [harness, ds.dir, ds.name, out] + ([str(stride)] if stride else []) + ([str(burst)] if burst else [])
+    cmd = [harness, ds.dir, ds.name, out]
+    if stride or burst or col:
+        cmd += [str(stride or 0), str(burst or 4)] + (["1"] if col else [])
     p = subprocess.run(cmd, capture_output=True, text=True, env=env)
     assert p.returncode == 0, p.stderr[-4000:]
     raw = np.fromfile(out, np.uint64)
@@ -84,6 +86,21 @@ def test_host_decode_flat_strides(harness, which, stride, burst, request, tmp_pa
     and the walk's burst length from 1 to 32 (the autotuner's range)."""
     ds = request.getfixturevalue(which)
     cnt, sums, err = _run(harness, ds, tmp_path, stride, burst)
+    assert err == 0
+    want_cnt, want_sums = _raw_sums(ds)
+    np.testing.assert_array_equal(cnt, want_cnt)
+    np.testing.assert_array_equal(sums, want_sums)
+
+
+@pytest.mark.parametrize("stride,burst", [(None, None), (16, 4), (1024, 16)])
+@pytest.mark.parametrize("which", ["ds_sample", "ds_adv", "ds_nf", "ds_ext"])
+def test_host_decode_columns(harness, which, stride, burst, request, tmp_path):
+    """Timestamps read from the column copy the engine builds at open
+    (groups of 64 list positions, interleaved 16-byte units, unused units
+    filled with ones) instead of the linear stream: the same per-vertex
+    counts and checksums."""
+    ds = request.getfixturevalue(which)
+    cnt, sums, err = _run(harness, ds, tmp_path, stride, burst, col=True)
     assert err == 0
     want_cnt, want_sums = _raw_sums(ds)
     np.testing.assert_array_equal(cnt, want_cnt)
