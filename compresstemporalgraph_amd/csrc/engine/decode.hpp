@@ -850,6 +850,7 @@ struct TsStage {
   uint64_t lim;   // stream dword index one past the item's last code
   uint32_t pos;   // bit offset of the next code from dword `base`
   uint32_t wend;  // relative dword index one past the last staged dword
+  int32_t wtop;   // 32 * wend - 64: a code at pos >= wtop may reach past the staged dwords
   int32_t prev;   // the timestamp before the next code
   const uint64_t* uw = nullptr;  // unit source (word pairs); nullptr: set to the linear stream by stage_init
   uint32_t ustride = 1;          // unit pitch of the source, in units
@@ -882,6 +883,7 @@ TGA_HD void stage_unit(const Streams& S, TsStage& st) {
     st.s[(size_t)(kStageDw + 1) * st.stride] = d1;
   }
   st.wend += 4;
+  st.wtop += 128;
 }
 
 // Starts the window at bit `pos` for codes ending before bit `end` (the
@@ -893,6 +895,7 @@ TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end
   st.base = (pos >> 5) & ~(uint64_t)3;
   st.pos = (uint32_t)(pos - st.base * 32);
   st.wend = 0;
+  st.wtop = -64;
   st.lim = (end + 31) >> 5;
   st.prev = prev;
 }
@@ -903,6 +906,7 @@ TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
   st.base = 0;
   st.pos = 0;
   st.wend = 0;
+  st.wtop = -64;
   st.lim = ((uint64_t)st.nbits + 31) >> 5;
   st.prev = prev;
 }
@@ -918,6 +922,7 @@ TGA_HD void stage_fill(const Streams& S, TsStage& st) {
     st.base += m;
     st.pos -= m * 32;
     st.wend -= m;
+    st.wtop -= (int32_t)(m * 32);
   }
   const uint32_t d = st.pos >> 5;
   while (st.wend + 4 <= d + kStageDw && st.base + st.wend < st.lim) stage_unit(S, st);
@@ -973,7 +978,8 @@ TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hm
 // otherwise t/32 is pos's own dword.  At pos = 0, t/32 maps to slot 15 and
 // the next two to the mirror slots of 0 and 1.
 TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st, uint32_t& hmax) {
-  if ((st.pos >> 5) + 3 > st.wend) stage_fill(S, st);  // codes longer than the burst's share
+  // (pos >> 5) + 3 > wend: the 96 bits read may pass the staged dwords
+  if ((int32_t)st.pos >= st.wtop) stage_fill(S, st);  // codes longer than the burst's share
   const uint32_t t = st.pos - 1u;
   const uint32_t* p = st.s + (size_t)((t >> 5) & (kStageDw - 1)) * st.stride;
   const uint32_t w0 = p[0], w1 = p[st.stride], w2 = p[2 * st.stride];
@@ -986,7 +992,7 @@ TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st, uint32_t& hmax) {
 }
 
 // Per-lane ring of decoded runs (value, contacts) between the two phases of
-// a walk; entry i of a lane is at e[i * stride] (in the kernels: LDS, the
+// a walk; entry i of a lane is at e[i * stride], stride a power of two (in the kernels: LDS, the
 // lanes of a block interleaved so that lanes at any ring positions touch
 // distinct banks).
 constexpr int32_t kRing = 4;  // runs held (power of two)
@@ -1026,13 +1032,16 @@ TGA_HD int32_t act_ring_val(Act&, int32_t v, long) {
 template <int MAXD, class Act>
 TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t k, int32_t kend, Act& act,
                  RunRing ring) {
-  uint32_t head = 0, tail = 0;  // ring positions; tail - head runs held
+  // Ring positions in units of ring.stride (a power of two), so an entry's
+  // offset is the position masked: tail - head = stride * runs held.
+  const uint32_t rstep = (uint32_t)ring.stride, rmask = (uint32_t)(kRing - 1) * rstep;
+  uint32_t head = 0, tail = 0;
   int32_t avail = 0;            // contacts held: the current run's and the ring's
   int32_t r = 0, c = 0;         // current run: neighbor (or its ring_val), contacts left in it
   bool more = true;             // the record has runs left
   uint32_t hmax = 0;            // largest zeta_2 prefix decoded
   while (k < kend) {
-    while (more && avail < S.burst && tail - head < (uint32_t)kRing) {
+    while (more && avail < S.burst && tail - head < (uint32_t)kRing * rstep) {
       int32_t cnt;
       const int32_t v = dec_next_run<MAXD>(S, D, cnt);
       if ((uint32_t)v >= (uint32_t)S.n) {
@@ -1040,8 +1049,8 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
         more = false;
         break;
       }
-      ring.e[(tail & (kRing - 1)) * ring.stride] = make_int2(act_ring_val(act, v, 0), cnt);
-      ++tail;
+      ring.e[tail & rmask] = make_int2(act_ring_val(act, v, 0), cnt);
+      tail += rstep;
       avail += cnt;
     }
     const int32_t n = min32(min32(kend - k, S.burst), avail);
@@ -1052,8 +1061,8 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
     stage_fill(S, ts);
     for (int32_t j = 0; j < n; ++j) {
       if (c == 0) {
-        const int2 e = ring.e[(head & (kRing - 1)) * ring.stride];
-        ++head;
+        const int2 e = ring.e[head & rmask];
+        head += rstep;
         r = e.x;
         c = e.y;
       }

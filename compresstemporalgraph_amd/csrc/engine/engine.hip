@@ -2985,6 +2985,13 @@ constexpr int64_t kHostCkptMin = 1 << 16;
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
 // HS holds host copies of the indexes (its streams and vpos unset); graph and
 // ts are the stream files' bytes.
+// Whether the static item lists get their timestamp codes in column order
+// (TGA_TCOL=0: no).
+static bool tcol_enabled() {
+  const char* e = std::getenv("TGA_TCOL");
+  return !(e && *e == '0');
+}
+
 static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& graph, const std::vector<uint8_t>& ts,
                           int talk) {
   const int64_t n = g->n;
@@ -3227,14 +3234,16 @@ static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& 
       else
         chain[group_of(c)].emplace_back(chain_cost(deg[(size_t)x], c), (int32_t)x);
     }
-    // Longest length class first, id order within a class: a stable
-    // counting sort.  A class is the msb of the degree shifted right by
-    // TGA_FLAT_CLASS_SHIFT (default 0): coarser classes put more consecutive
-    // ids next to each other (shared cache lines of the streams) at the cost
-    // of less even record lengths within a wave.
-    // TGA_FLAT_CLASS_SHIFT = -1 sorts by the exact degree instead (whole
-    // flat records have at most flat_stride contacts).
-    int cshift = 0;
+    // Longest first, by a stable counting sort.  With the timestamp codes in
+    // column order (the default, tcol_build) the key is the exact degree:
+    // the 64 records of a wave then have equal lengths, so no lane idles
+    // while the longest finishes, and only the short graph-stream reads of
+    // neighbouring ids are given up.  On the linear stream the key is the
+    // length class, the msb of the degree (id order within a class keeps
+    // the lanes' timestamp reads on shared cache lines).
+    // TGA_FLAT_CLASS_SHIFT overrides: -1 the exact degree, k >= 0 the msb
+    // shifted right by k (coarser classes).
+    int cshift = tcol_enabled() ? -1 : 0;
     if (const char* e = std::getenv("TGA_FLAT_CLASS_SHIFT")) cshift = std::max(-1, std::min(6, std::atoi(e)));
     const int32_t ncls = cshift < 0 ? S.flat_stride + 1 : 64;
     auto cls = [&](int32_t x) {
@@ -3273,8 +3282,7 @@ static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& 
 // (TsCols): per-position bit ranges on the device, group rows and offsets on
 // the host, then the copy.  TGA_TCOL=0 leaves the lists on the linear stream.
 static int tcol_build(tga_graph* g) {
-  if (const char* e = std::getenv("TGA_TCOL"))
-    if (*e == '0') return TGA_OK;
+  if (!tcol_enabled()) return TGA_OK;
   for (int c = 0; c < kGroups; ++c) {
     const uint32_t ni = g->n_items[c];
     if (ni == 0) continue;
