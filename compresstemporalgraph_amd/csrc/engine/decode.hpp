@@ -104,6 +104,10 @@ struct Streams {
   const uint64_t* t;  // timestamp stream, byte-swapped words (+5 zero words)
   uint64_t g_bits, t_bits;
   uint32_t g_last, t_last;  // index of the last padding word of each stream
+  // Word i of the graph stream is g[i << g_sh]: 0 for the stream itself; a
+  // walk over a record's column copy (run_item) sets g to the record's first
+  // word, g_sh to 6 and g_bits / g_last to the copy's extent.
+  uint32_t g_sh;
   EFDir gi, ti;
   int32_t n;       // vertices; ids are [0, n)
   int32_t min_ts;  // .timestamps header value
@@ -130,12 +134,12 @@ TGA_HD void raise_err(const Streams& S, int32_t bit) {
 #endif
 }
 
-// 64 stream bits starting at bit `pos` (MSB-first).
-TGA_HD uint64_t peek64(const uint64_t* w, uint64_t pos) {
+// 64 stream bits starting at bit `pos` (MSB-first), word i at w[i << sh].
+TGA_HD uint64_t peek64(const uint64_t* w, uint64_t pos, uint32_t sh = 0) {
   const uint64_t i = pos >> 6;
   const uint32_t s = (uint32_t)pos & 63u;
-  const uint64_t a = w[i];
-  const uint64_t b = w[i + 1];
+  const uint64_t a = w[i << sh];
+  const uint64_t b = w[(i + 1) << sh];
   return s ? (a << s) | (b >> (64u - s)) : a;
 }
 
@@ -189,6 +193,7 @@ TGA_HD uint64_t dec_zeta(const Streams& S, uint64_t p, uint32_t& len) {
 struct Rd {
   const uint64_t* w;
   uint64_t bits;
+  uint32_t sh;  // word i at w[i << sh]
 };
 
 TGA_HD uint64_t safe_peek(const Streams& S, Rd r, uint64_t pos) {
@@ -196,7 +201,7 @@ TGA_HD uint64_t safe_peek(const Streams& S, Rd r, uint64_t pos) {
     raise_err(S, kErrOverrun);
     return ~0ull;  // all ones: every code decodes to 0 with length 1
   }
-  return peek64(r.w, pos);
+  return peek64(r.w, pos, r.sh);
 }
 
 TGA_HD uint32_t rd_unary(const Streams& S, Rd r, uint64_t& pos) {
@@ -228,20 +233,21 @@ TGA_HD uint64_t rd_zeta(const Streams& S, Rd r, uint64_t& pos) {
 // the codewords in between.  The stream pointer and the index of its last
 // padding word are passed in (they are the same for every buffer over one
 // stream); stream word arrays carry five zero words of padding, and reads
-// past `last` return 0 (every code then fails its length check).
+// past `last` return 0 (every code then fails its length check).  Word i of
+// the stream is w[i << ws] (ws: Streams::g_sh).
 struct BitBuf {
   uint64_t a, b, c;
   uint32_t idx;  // word index of a
   uint32_t sh;   // bit offset into a
 
-  TGA_HD void init(const uint64_t* w, uint32_t last, uint64_t pos) {
+  TGA_HD void init(const uint64_t* w, uint32_t last, uint64_t pos, uint32_t ws = 0) {
     uint64_t i = pos >> 6;
     if (i + 2 > last) i = last - 2;
     idx = (uint32_t)i;
     sh = (uint32_t)pos & 63u;
-    a = w[idx];
-    b = w[idx + 1];
-    c = w[idx + 2];
+    a = w[(uint64_t)idx << ws];
+    b = w[(uint64_t)(idx + 1) << ws];
+    c = w[(uint64_t)(idx + 2) << ws];
   }
   // A buffer that will not be read (its position is recorded as 0).
   TGA_HD void clear() {
@@ -249,14 +255,14 @@ struct BitBuf {
     idx = sh = 0;
   }
   TGA_HD uint64_t peek() const { return sh ? (a << sh) | (b >> (64u - sh)) : a; }
-  TGA_HD void skip(uint32_t n, const uint64_t* w, uint32_t last) {  // n <= 64
+  TGA_HD void skip(uint32_t n, const uint64_t* w, uint32_t last, uint32_t ws = 0) {  // n <= 64
     sh += n;
     if (sh >= 64) {
       sh -= 64;
       a = b;
       b = c;
       ++idx;
-      c = w[minu32(idx + 2, last)];  // past the end: the last padding word (zero)
+      c = w[(uint64_t)minu32(idx + 2, last) << ws];  // past the end: the last padding word
     }
   }
   TGA_HD uint64_t pos() const { return (uint64_t)idx * 64 + sh; }
@@ -336,27 +342,27 @@ struct PBuf {
 template <class B>
 TGA_HD void g_init(const Streams& S, B& b, uint64_t pos) {
   if (pos >= S.g_bits) raise_err(S, kErrOverrun);
-  b.init(S.g, S.g_last, pos);
+  b.init(S.g, S.g_last, pos, S.g_sh);
 }
 template <class B>
 TGA_HD uint64_t g_gamma(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_gamma(S, b.peek(), len);
-  b.skip(len, S.g, S.g_last);
+  b.skip(len, S.g, S.g_last, S.g_sh);
   return v;
 }
 template <class B>
 TGA_HD uint32_t g_unary(const Streams& S, B& b) {
   uint32_t len;
   const uint32_t v = dec_unary(S, b.peek(), len);
-  b.skip(len, S.g, S.g_last);
+  b.skip(len, S.g, S.g_last, S.g_sh);
   return v;
 }
 template <int K, class B>
 TGA_HD uint64_t g_zeta(const Streams& S, B& b) {
   uint32_t len;
   const uint64_t v = dec_zeta<K>(S, b.peek(), len);
-  b.skip(len, S.g, S.g_last);
+  b.skip(len, S.g, S.g_last, S.g_sh);
   return v;
 }
 template <class B>
@@ -582,7 +588,7 @@ TGA_HD int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
     return kInf;
   } else {
     Level& v = D.lv[L];
-    const Rd g{S.g, S.g_bits};
+    const Rd g{S.g, S.g_bits, S.g_sh};
     for (;;) {
       if (v.cleft <= 0) return kInf;
       if (v.brem == 0) {
@@ -620,7 +626,7 @@ TGA_HD int32_t dec_masked(const Streams& S, Dec<MAXD>& D) {
 template <int L, int MAXD>
 TGA_HD int32_t dec_next(const Streams& S, Dec<MAXD>& D) {
   Level& v = D.lv[L];
-  const Rd g{S.g, S.g_bits};
+  const Rd g{S.g, S.g_bits, S.g_sh};
   // The deepest level has no copy source.
   const int32_t m = (L < MAXD) ? min32(min32(v.hm, v.hi), min32(v.hr, v.hc)) : min32(min32(v.hm, v.hi), v.hr);
   if (m == kInf) return kInf;
@@ -745,7 +751,7 @@ TGA_HD int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos
   if (extra > 0) {  // intervals
     const int32_t ic = (int32_t)g_gamma(S, h);
     if (ic > 0) {
-      const Rd g{S.g, S.g_bits};
+      const Rd g{S.g, S.g_bits, S.g_sh};
       uint64_t first = h.pos();
       for (int32_t i = 0; i < ic; ++i) {
         if (extra <= 0) {  // more intervals than contacts: malformed
@@ -855,6 +861,9 @@ struct TsStage {
   const uint64_t* uw = nullptr;  // unit source (word pairs); nullptr: set to the linear stream by stage_init
   uint32_t ustride = 1;          // unit pitch of the source, in units
   uint32_t nbits = 0;            // column source: the item's code bits
+  // The item's graph column copy (run_item), or nullptr.
+  const uint64_t* gw = nullptr;
+  uint32_t gbits = 0;            // the record's graph bits
 };
 
 // Stages the 16-byte unit at relative dword wend (source words 2u, 2u + 1 of
@@ -1091,9 +1100,13 @@ TGA_HD void act_item(Act&, int32_t, long) {}
 // code < 0: chunk -code-1 of the checkpoint table.  act.begin(x, chunked,
 // split) may veto the item; act.end(x, chunked) runs after the walk.  The
 // timestamps come from the linear stream unless ts.uw names the item's
-// column copy.
+// column copy; a whole record without references is read from its graph
+// column copy when ts.gw names one (ts.gbits: the record's bits; the copy
+// holds ceil(gbits / 64) + 5 words, the stream's bits from the record's
+// first on, so every read the decoder makes stays within it).
 template <int MAXD, class Act>
-TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, RunRing ring, TsStage ts) {
+TGA_HD void run_item(const Streams& S0, const Ckpt* ck, int32_t code, Act& act, RunRing ring, TsStage ts) {
+  Streams S = S0;
   Dec<MAXD> D;
   int32_t x, k = 0, kend;
   const bool chunked = code < 0;
@@ -1102,8 +1115,16 @@ TGA_HD void run_item(const Streams& S, const Ckpt* ck, int32_t code, Act& act, R
     x = code;
     if (!act.begin(x, false, Split{-1, -1, -1, -1})) return;
     int32_t depth = 0;
-    uint64_t gpos, tpos;
-    rec_pos(S, x, gpos, tpos);
+    uint64_t gpos = 0, tpos = 0;
+    if (MAXD == 0 && ts.gw) {
+      const uint32_t words = (ts.gbits + 63) / 64;
+      S.g = ts.gw;
+      S.g_sh = 6;
+      S.g_last = words + 4;
+      S.g_bits = 64ull * (words + 4);
+    } else {
+      rec_pos(S, x, gpos, tpos);
+    }
     kend = dec_init<0, MAXD>(S, D, x, gpos, depth);
     if (kend == 0) {
       act.end(x, false);

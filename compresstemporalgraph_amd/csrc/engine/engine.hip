@@ -121,6 +121,13 @@ struct TsCols {
   const uint64_t* w = nullptr;     // units as word pairs (stream word layout)
   const uint64_t* goff = nullptr;  // per group: first unit
   const uint32_t* nbits = nullptr; // per list position: the item's code bits
+  // The flat list's whole records also have their graph bits in columns of
+  // words: word j of position 64q + l at gw[ggoff[q] + 64j + l], the
+  // record's bits from bit 0 of word 0 on, ceil(gbits / 64) + 5 words
+  // (gbits 0: a chunk, read from the stream).
+  const uint64_t* gw = nullptr;
+  const uint64_t* ggoff = nullptr;  // per group: first word
+  const uint32_t* gbits = nullptr;  // per list position
 };
 
 // An item list and its length; static lists carry their column copy.
@@ -135,6 +142,10 @@ struct Items {
       st.uw = tc.w + 2 * (tc.goff[i >> 6] + (i & 63));
       st.ustride = 64;
       st.nbits = tc.nbits[i];
+    }
+    if (tc.gw) {
+      st.gbits = tc.gbits[i];
+      st.gw = st.gbits ? tc.gw + tc.ggoff[i >> 6] + (i & 63) : nullptr;
     }
     return st;
   }
@@ -205,11 +216,47 @@ __global__ void __launch_bounds__(kBlock) k_tcol_fill(Streams S, Items I, const 
   }
 }
 
+// Graph bits of every whole record of a flat list (0 for a chunk), and where
+// they start.
+__global__ void __launch_bounds__(kBlock) k_gcol_len(Streams S, Items I, uint64_t* gstart, uint32_t* gbits) {
+  FOR_ITEMS(I, i) {
+    const int32_t x = I.code[i];
+    uint64_t b = 0, e = 0;
+    if (x >= 0) {
+      b = S.vpos[2 * (size_t)x];
+      e = x + 1 < S.n ? S.vpos[2 * (size_t)x + 2] : S.g_bits;
+    }
+    gstart[i] = b;
+    gbits[i] = (uint32_t)(e - b);
+  }
+}
+
+// Copies every whole record's graph bits into its column (TsCols::gw):
+// ceil(gbits / 64) + 5 words of the stream from the record's first bit on
+// (words past the stream's padding read as 0).
+__global__ void __launch_bounds__(kBlock) k_gcol_fill(Streams S, Items I, const uint64_t* gstart, uint64_t* w) {
+  FOR_ITEMS(I, i) {
+    const uint32_t nb = I.tc.gbits[i];
+    if (nb == 0) continue;
+    const uint64_t b = gstart[i];
+    uint64_t* out = w + I.tc.ggoff[i >> 6] + (i & 63);
+    const uint32_t words = (nb + 63) / 64 + 5;
+    for (uint32_t j = 0; j < words; ++j) {
+      const uint64_t p = b + 64ull * j;
+      const uint64_t wi = p >> 6;
+      const uint32_t sh = (uint32_t)p & 63u;
+      const uint64_t a = wi <= S.g_last ? S.g[wi] : 0;
+      const uint64_t c = wi + 1 <= S.g_last ? S.g[wi + 1] : 0;
+      out[64ull * j] = sh ? (a << sh) | (c >> (64u - sh)) : a;
+    }
+  }
+}
+
 // Record header of every vertex: outdegree and reference distance.
 // Also fills the record start table (Streams::vpos) from the two
 // Elias-Fano indexes.
 __global__ void __launch_bounds__(kBlock) k_headers(Streams S, int32_t* deg, int8_t* ref, uint64_t* vpos) {
-  const Rd g{S.g, S.g_bits};
+  const Rd g{S.g, S.g_bits, 0};
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < S.n; x += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t gpos = ef_get(S, S.gi, (uint64_t)x);
     const uint64_t tpos = ef_get(S, S.ti, (uint64_t)x);
@@ -2985,12 +3032,14 @@ constexpr int64_t kHostCkptMin = 1 << 16;
 // Builds the work schedule: headers, chain depths, checkpoints, item lists.
 // HS holds host copies of the indexes (its streams and vpos unset); graph and
 // ts are the stream files' bytes.
-// Whether the static item lists get their timestamp codes in column order
-// (TGA_TCOL=0: no).
-static bool tcol_enabled() {
+// Column copies of the static item lists (tcol_build): 2 (default) the
+// timestamp codes and the flat records' graph bits, 1 the timestamp codes
+// only, 0 none (TGA_TCOL).
+static int tcol_level() {
   const char* e = std::getenv("TGA_TCOL");
-  return !(e && *e == '0');
+  return e && *e ? std::max(0, std::min(2, std::atoi(e))) : 2;
 }
+static bool tcol_enabled() { return tcol_level() > 0; }
 
 static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& graph, const std::vector<uint8_t>& ts,
                           int talk) {
@@ -3280,7 +3329,8 @@ static int build_schedule(tga_graph* g, Streams HS, const std::vector<uint8_t>& 
 
 // Builds the column copy of the timestamp codes of every static item list
 // (TsCols): per-position bit ranges on the device, group rows and offsets on
-// the host, then the copy.  TGA_TCOL=0 leaves the lists on the linear stream.
+// the host, then the copy; then the same for the graph bits of the flat
+// list's whole records.
 static int tcol_build(tga_graph* g) {
   if (!tcol_enabled()) return TGA_OK;
   for (int c = 0; c < kGroups; ++c) {
@@ -3316,6 +3366,43 @@ static int tcol_build(tga_graph* g) {
     g->release(tstart);
     g->tcols[c] = tc;
   }
+  // Graph columns of the flat list's whole records (records without
+  // references: every read of their walk stays within their own bits).
+  const uint32_t ni = g->n_items[0];
+  if (ni == 0 || tcol_level() < 2) return TGA_OK;
+  const Items I{g->items[0], ni};
+  uint64_t* gstart = nullptr;
+  uint32_t* gbits = nullptr;
+  int rc;
+  if ((rc = g->alloc(gstart, ni)) || (rc = g->alloc(gbits, ni))) return rc;
+  k_gcol_len<<<g->grid(ni), kBlock, 0, g->stream>>>(g->S, I, gstart, gbits);
+  HIP_TRY(hipGetLastError());
+  std::vector<uint32_t> hb(ni);
+  HIP_TRY(hipMemcpyAsync(hb.data(), gbits, (size_t)ni * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const size_t ng = ((size_t)ni + 63) / 64;
+  std::vector<uint64_t> goff(ng + 1);
+  uint64_t at = 0;
+  for (size_t q = 0; q < ng; ++q) {
+    uint32_t rows = 0;
+    for (size_t i = 64 * q; i < std::min<size_t>(ni, 64 * q + 64); ++i)
+      if (hb[i]) rows = std::max(rows, (hb[i] + 63) / 64 + 5);
+    goff[q] = at;
+    at += 64ull * rows;
+  }
+  goff[ng] = at;
+  uint64_t* w = nullptr;
+  uint64_t* dgoff = nullptr;
+  if ((rc = g->alloc(w, std::max<uint64_t>(at, 1))) || (rc = g->upload(dgoff, goff))) return rc;
+  TsCols tc = g->tcols[0];
+  tc.gw = w;
+  tc.ggoff = dgoff;
+  tc.gbits = gbits;
+  k_gcol_fill<<<g->grid(ni), kBlock, 0, g->stream>>>(g->S, Items{g->items[0], ni, tc}, gstart, w);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->release(gstart);
+  g->tcols[0] = tc;
   return TGA_OK;
 }
 

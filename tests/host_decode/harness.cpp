@@ -128,7 +128,7 @@ int main(int argc, char** argv) {
   std::vector<uint64_t> vpos(2 * (size_t)n);
   std::vector<int32_t> deg((size_t)n);
   std::vector<int8_t> depth((size_t)n, 0);
-  const Rd g{S.g, S.g_bits};
+  const Rd g{S.g, S.g_bits, 0};
   for (int64_t x = 0; x < n; ++x) {
     vpos[2 * (size_t)x] = ef_get(S, S.gi, (uint64_t)x);
     vpos[2 * (size_t)x + 1] = ef_get(S, S.ti, (uint64_t)x);
@@ -208,9 +208,40 @@ int main(int argc, char** argv) {
   // positions, unit u of position 64q + l at goff[q] + 64u + l); units past
   // an item's codes hold all-ones words, so a walk that read them would
   // decode wrong times.
-  std::vector<uint64_t> cw, goff;
-  std::vector<uint32_t> nbits(list.size());
+  std::vector<uint64_t> cw, goff, gcw, ggoff;
+  std::vector<uint32_t> nbits(list.size()), gbits(list.size(), 0);
   if (col) {
+    // Graph columns of the whole records without references (k_gcol_len,
+    // k_gcol_fill): ceil(bits / 64) + 5 words each, word j of position
+    // 64q + l at ggoff[q] + 64j + l; other words all ones.
+    const size_t gng = (list.size() + 63) / 64;
+    ggoff.resize(gng + 1);
+    uint64_t gat = 0;
+    for (size_t i = 0; i < list.size(); ++i)
+      if (list[i] >= 0 && lgroup[i] == 0) {
+        const int32_t x = list[i];
+        const uint64_t e = x + 1 < n ? vpos[2 * (size_t)x + 2] : S.g_bits;
+        gbits[i] = (uint32_t)(e - vpos[2 * (size_t)x]);
+      }
+    for (size_t q = 0; q < gng; ++q) {
+      uint32_t rows = 0;
+      for (size_t i = 64 * q; i < std::min(list.size(), 64 * q + 64); ++i)
+        if (gbits[i]) rows = std::max(rows, (gbits[i] + 63) / 64 + 5);
+      ggoff[q] = gat;
+      gat += 64ull * rows;
+    }
+    ggoff[gng] = gat;
+    gcw.assign((size_t)gat, ~0ull);
+    for (size_t i = 0; i < list.size(); ++i) {
+      if (!gbits[i]) continue;
+      const uint64_t b = vpos[2 * (size_t)list[i]];
+      for (uint64_t j = 0; j < (gbits[i] + 63) / 64 + 5; ++j) {
+        const uint64_t p = b + 64 * j, wi = p >> 6;
+        const uint32_t sh = (uint32_t)p & 63u;
+        const uint64_t a = wi <= S.g_last ? S.g[wi] : 0, c = wi + 1 <= S.g_last ? S.g[wi + 1] : 0;
+        gcw[(size_t)(ggoff[i / 64] + 64 * j + (i % 64))] = sh ? (a << sh) | (c >> (64u - sh)) : a;
+      }
+    }
     std::vector<uint64_t> tstart(list.size());
     const int64_t nck = (int64_t)slots;
     for (size_t i = 0; i < list.size(); ++i) {
@@ -254,6 +285,10 @@ int main(int argc, char** argv) {
       st.uw = cw.data() + 2 * (goff[i / 64] + (i % 64));
       st.ustride = 64;
       st.nbits = nbits[i];
+      if (gbits[i]) {
+        st.gw = gcw.data() + ggoff[i / 64] + (i % 64);
+        st.gbits = gbits[i];
+      }
     }
     const int c = lgroup[i];
     if (c == 0) run_item<0>(S, ck.data(), list[i], a, ring, st);

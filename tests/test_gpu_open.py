@@ -43,18 +43,20 @@ def test_host_checkpoints_equal_device(which, request, monkeypatch):
         dev.close()
 
 
+@pytest.mark.parametrize("level", ["0", "1"])
 @pytest.mark.parametrize("which", ["ds_adv", "ds_nf", "ds_cl", "ds_ext"])
-def test_timestamp_columns_equal_linear(which, request, monkeypatch):
-    """The static item lists read their timestamp codes from the column copy
-    built at open (64 items of a wave interleaved in 16-byte units); a handle
-    opened with TGA_TCOL=0 reads the linear stream.  Every pass over the
-    static lists must give the same results both ways and match the oracle:
-    neighbor access, degree counts (full and windowed), PageRank and CC over
-    the full graph, and the batched PageRank over the run structure."""
+def test_column_copies_equal_linear(which, level, request, monkeypatch):
+    """The static item lists read their timestamp codes, and the flat list's
+    whole records their graph bits, from column copies built at open (the 64
+    items of a wave interleaved); TGA_TCOL=1 keeps only the timestamp copy
+    and TGA_TCOL=0 reads both streams directly.  Every pass over the static
+    lists must give the same results each way and match the oracle: neighbor
+    access, degree counts (full and windowed), PageRank and CC over the full
+    graph, and the batched PageRank over the run structure."""
     ds = request.getfixturevalue(which)
-    print(f"\n[gpu] timestamp columns on {ds.name}", flush=True)
+    print(f"\n[gpu] column copies on {ds.name} against TGA_TCOL={level}", flush=True)
     col = TemporalGraph(ds.dir, ds.name)
-    monkeypatch.setenv("TGA_TCOL", "0")
+    monkeypatch.setenv("TGA_TCOL", level)
     lin = TemporalGraph(ds.dir, ds.name)
     monkeypatch.delenv("TGA_TCOL")
     o = ds.oracle()
