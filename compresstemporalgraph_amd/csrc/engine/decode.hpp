@@ -852,13 +852,12 @@ constexpr int32_t kStageDw = 16;  // staged dwords per lane (power of two, multi
 struct TsStage {
   uint32_t* s;
   int32_t stride;
-  uint64_t base;  // stream dword index of relative dword 0 (a multiple of 4)
-  uint64_t lim;   // stream dword index one past the item's last code
-  uint32_t pos;   // bit offset of the next code from dword `base`
-  uint32_t wend;  // relative dword index one past the last staged dword
-  int32_t wtop;   // 32 * wend - 64: a code at pos >= wtop may reach past the staged dwords
-  int32_t prev;   // the timestamp before the next code
-  const uint64_t* uw = nullptr;  // unit source (word pairs); nullptr: set to the linear stream by stage_init
+  uint32_t pos;    // bit offset of the next code from relative dword 0 (the first unit's first dword)
+  uint32_t wend;   // relative dword index one past the last staged dword
+  int32_t wtop;    // 32 * wend - 64: a code at pos >= wtop may reach past the staged dwords
+  int32_t prev;    // the timestamp before the next code
+  uint32_t uleft;  // units holding codes of the item not yet staged
+  const uint64_t* uw = nullptr;  // next unit to stage (word pairs); nullptr: the linear stream (stage_init)
   uint32_t ustride = 1;          // unit pitch of the source, in units
   uint32_t nbits = 0;            // column source: the item's code bits
   // The item's graph column copy (run_item), or nullptr.
@@ -866,21 +865,21 @@ struct TsStage {
   uint32_t gbits = 0;            // the record's graph bits
 };
 
-// Stages the 16-byte unit at relative dword wend (source words 2u, 2u + 1 of
-// unit u, at unit u * ustride) with one 16-byte load.  A unit is staged only
-// while its first dword lies before the item's end (stage_fill); in the
-// linear stream the last word of a unit then lies within the stream's five
-// words of padding, and a column holds every unit its item's codes touch, so
-// no bound check is needed.
-TGA_HD void stage_unit(const Streams& S, TsStage& st) {
-  const uint64_t u = ((st.base + st.wend) >> 2) * st.ustride;
+// Stages the next unit (source words uw[0], uw[1]) at relative dword wend
+// with one 16-byte load.  Only units whose first dword lies before the item's
+// end are staged (uleft); in the linear stream the last word of such a unit
+// lies within the stream's five words of padding, and a column holds every
+// unit its item's codes touch, so no bound check is needed.
+TGA_HD void stage_unit(TsStage& st) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const uint4 v = reinterpret_cast<const uint4*>(st.uw)[u];
+  const uint4 v = *reinterpret_cast<const uint4*>(st.uw);
   const uint32_t d0 = v.y, d1 = v.x, d2 = v.w, d3 = v.z;  // each word's high dword first
 #else
-  const uint64_t w0 = st.uw[2 * u], w1 = st.uw[2 * u + 1];
+  const uint64_t w0 = st.uw[0], w1 = st.uw[1];
   const uint32_t d0 = (uint32_t)(w0 >> 32), d1 = (uint32_t)w0, d2 = (uint32_t)(w1 >> 32), d3 = (uint32_t)w1;
 #endif
+  st.uw += 2 * (size_t)st.ustride;
+  --st.uleft;
   const uint32_t slot = st.wend & (kStageDw - 1);
   uint32_t* p = st.s + (size_t)slot * st.stride;
   p[0] = d0;
@@ -899,42 +898,41 @@ TGA_HD void stage_unit(const Streams& S, TsStage& st) {
 // first unit holds the code at pos).
 TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end, int32_t prev) {
   if (pos >= S.t_bits) raise_err(S, kErrOverrun);
-  st.uw = S.t;
+  const uint64_t base = (pos >> 5) & ~(uint64_t)3;  // stream dword of the first unit
+  const uint64_t lim = (end + 31) >> 5;             // one past the item's last dword
+  st.uw = S.t + (base >> 1);
   st.ustride = 1;
-  st.base = (pos >> 5) & ~(uint64_t)3;
-  st.pos = (uint32_t)(pos - st.base * 32);
+  st.uleft = lim > base ? (uint32_t)((lim - base + 3) >> 2) : 0u;
+  st.pos = (uint32_t)(pos - base * 32);
   st.wend = 0;
   st.wtop = -64;
-  st.lim = (end + 31) >> 5;
   st.prev = prev;
 }
 
 // Starts the window at the first code of a column source (uw, ustride and
 // nbits set by the caller).
 TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
-  st.base = 0;
+  st.uleft = (st.nbits + 127) >> 7;
   st.pos = 0;
   st.wend = 0;
   st.wtop = -64;
-  st.lim = ((uint64_t)st.nbits + 31) >> 5;
   st.prev = prev;
 }
 
 // Tops the window up: whole units while they fit behind the current dword
 // and hold codes of the item.  A 64-bit peek near the item's end may then see
-// stale dwords after its code, which no valid code depends on.  Positions are
-// kept relative to `base`, which moves up in steps of kStageDw dwords (slot
-// numbers stay put) before they could overflow.
-TGA_HD void stage_fill(const Streams& S, TsStage& st) {
+// stale dwords after its code, which no valid code depends on.  Positions
+// move down in steps of kStageDw dwords (slot numbers stay put) before they
+// could overflow.
+TGA_HD void stage_fill(const Streams&, TsStage& st) {
   if (st.pos >= (1u << 30)) {
     const uint32_t m = (st.pos >> 5) & ~(uint32_t)(kStageDw - 1);
-    st.base += m;
     st.pos -= m * 32;
     st.wend -= m;
     st.wtop -= (int32_t)(m * 32);
   }
   const uint32_t d = st.pos >> 5;
-  while (st.wend + 4 <= d + kStageDw && st.base + st.wend < st.lim) stage_unit(S, st);
+  while (st.wend + 4 <= d + kStageDw && st.uleft) stage_unit(st);
 }
 
 // The 32-bit funnel shift ({hi, lo} >> (n mod 32)) mod 2^32: one

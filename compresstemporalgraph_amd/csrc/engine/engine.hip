@@ -3777,7 +3777,7 @@ extern "C" int tga_neighbor_access_noalloc(tga_graph* g, const int32_t* verts, i
 extern "C" int tga_autotune(tga_graph* g, int32_t passes, tga_tune_report* report) {
   if (!g || passes < 1 || passes > 1000) return fail(TGA_EINVAL, "tga_autotune: bad arguments");
   HIP_TRY(hipSetDevice(g->device));
-  static const int32_t cand[] = {4, 8, 16, 32};
+  static const int32_t cand[] = {4, 8, 12, 16, 20, 24, 32};
   constexpr int nc = (int)(sizeof cand / sizeof cand[0]);
   static_assert(nc <= TGA_TUNE_MAX, "report too small");
   tga_tune_report rep{};
