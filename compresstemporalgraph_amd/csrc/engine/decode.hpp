@@ -774,6 +774,91 @@ TGA_HD int32_t dec_init(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t gpos
   return d;
 }
 
+// Start word of a record without references: what dec_init learns by
+// parsing the header and skipping the multiples and intervals sections
+// (their codes are read again, one run at a time, as the walk reaches them),
+// packed into 64 bits at open so that a walk can start the three sub-streams
+// directly: d (12 bits) | residuals (12) | bit offset of the intervals
+// section + 1, 0 when the record has none (20) | bit offset of the first
+// residual (20), offsets from the record's first bit.  kNoStart when the
+// record has a reference, does not fit the fields or is malformed (the walk
+// then parses it with dec_init, which reports the error).
+constexpr uint64_t kNoStart = ~0ull;
+TGA_HD uint32_t gamma_len(uint32_t v) { return 2u * (31u - (uint32_t)__builtin_clz(v + 1u)) + 1u; }
+
+TGA_HD uint64_t flat_start_word(const Streams& S, uint64_t gpos) {
+  const Rd g{S.g, S.g_bits, S.g_sh};
+  uint64_t p = gpos;
+  const uint64_t d = rd_gamma(S, g, p);
+  if (d == 0 || d >= 4096) return kNoStart;
+  if (rd_unary(S, g, p) != 0) return kNoStart;
+  int64_t extra = (int64_t)d;
+  const uint64_t mc = rd_gamma(S, g, p);
+  for (uint64_t i = 0; i < mc; ++i) {
+    if (extra <= 0 || p >= S.g_bits) return kNoStart;
+    rd_gamma(S, g, p);
+    extra -= (int64_t)rd_gamma(S, g, p) + 2;
+  }
+  uint64_t ip = 0, rp = 0;
+  if (extra > 0) {
+    ip = p - gpos + 1;
+    const uint64_t ic = rd_gamma(S, g, p);
+    for (uint64_t i = 0; i < ic; ++i) {
+      if (extra <= 0 || p >= S.g_bits) return kNoStart;
+      rd_gamma(S, g, p);
+      extra -= (int64_t)rd_gamma(S, g, p) + 4;
+    }
+  }
+  if (extra > 0) rp = p - gpos;
+  if (extra < 0 || ip >= (1u << 20) || rp >= (1u << 20)) return kNoStart;
+  return d | (uint64_t)extra << 12 | ip << 24 | rp << 44;
+}
+
+// dec_init of a record without references from its start word (Streams
+// positions relative to the record's first bit): the same level state, the
+// header and the skipped sections left unread.
+template <int MAXD>
+TGA_HD int32_t dec_init_flat(const Streams& S, Dec<MAXD>& D, int32_t x, uint64_t w) {
+  Level& v = D.lv[0];
+  auto& h = rbuf<0>(D);
+  v.x = x;
+  v.hm = v.hi = v.hr = v.hc = kInf;
+  v.mrem = v.mleft = v.irem = v.ileft = v.rleft = 0;
+  v.cleft = v.bleft = v.brem = 0;
+  v.kpos = 0;
+  v.bidx = -1;
+  v.mpos = v.ipos = v.rpos = v.bpos = 0;
+  mbuf<0>(D).clear();
+  const uint32_t d = (uint32_t)w & 4095u;
+  const int32_t nres = (int32_t)(w >> 12) & 4095;
+  const uint64_t ip = (w >> 24) & 0xFFFFFu, rp = w >> 44;
+  g_init(S, h, gamma_len(d) + 1);  // past d and the reference 0: the multiples count
+  const int32_t mc = (int32_t)g_gamma(S, h);
+  if (mc > 0) {
+    mbuf<0>(D) = h;
+    v.hm = (int32_t)(nat2int(g_gamma(S, mbuf<0>(D))) + x);
+    v.mrem = (int32_t)g_gamma(S, mbuf<0>(D)) + 2;
+    v.mleft = mc - 1;
+  }
+  if (ip) {
+    const Rd g{S.g, S.g_bits, S.g_sh};
+    uint64_t p = ip - 1;
+    const int32_t ic = (int32_t)rd_gamma(S, g, p);
+    if (ic > 0) {
+      v.hi = (int32_t)(nat2int(rd_gamma(S, g, p)) + x);
+      v.irem = (int32_t)rd_gamma(S, g, p) + 4;
+      v.ileft = ic - 1;
+      v.ipos = p;
+    }
+  }
+  if (nres > 0) {
+    g_init(S, h, rp);
+    v.hr = (int32_t)(nat2int(g_zeta<3>(S, h)) + x);
+    v.rleft = nres - 1;
+  }
+  return (int32_t)d;
+}
+
 // Timestamp cursor: zeta_2 of int2nat deltas, reset to the header value per
 // vertex (EvolvingMultiGraph.java:131-144; bfs.cu:1444, :1597).
 struct TsCursor {
@@ -1100,8 +1185,9 @@ TGA_HD void act_item(Act&, int32_t, long) {}
 // timestamps come from the linear stream unless ts.uw names the item's
 // column copy; a whole record without references is read from its graph
 // column copy when ts.gw names one (ts.gbits: the record's bits; the copy
-// holds ceil(gbits / 64) + 5 words, the stream's bits from the record's
-// first on, so every read the decoder makes stays within it).
+// holds the record's start word, then ceil(gbits / 64) + 5 words of the
+// stream's bits from the record's first on, so every read the decoder makes
+// stays within it).
 template <int MAXD, class Act>
 TGA_HD void run_item(const Streams& S0, const Ckpt* ck, int32_t code, Act& act, RunRing ring, TsStage ts) {
   Streams S = S0;
@@ -1114,16 +1200,18 @@ TGA_HD void run_item(const Streams& S0, const Ckpt* ck, int32_t code, Act& act, 
     if (!act.begin(x, false, Split{-1, -1, -1, -1})) return;
     int32_t depth = 0;
     uint64_t gpos = 0, tpos = 0;
+    uint64_t sw = kNoStart;
     if (MAXD == 0 && ts.gw) {
       const uint32_t words = (ts.gbits + 63) / 64;
-      S.g = ts.gw;
+      sw = ts.gw[0];
+      S.g = ts.gw + 64;
       S.g_sh = 6;
       S.g_last = words + 4;
       S.g_bits = 64ull * (words + 4);
     } else {
       rec_pos(S, x, gpos, tpos);
     }
-    kend = dec_init<0, MAXD>(S, D, x, gpos, depth);
+    kend = sw != kNoStart ? dec_init_flat<MAXD>(S, D, x, sw) : dec_init<0, MAXD>(S, D, x, gpos, depth);
     if (kend == 0) {
       act.end(x, false);
       return;

@@ -122,9 +122,10 @@ struct TsCols {
   const uint64_t* goff = nullptr;  // per group: first unit
   const uint32_t* nbits = nullptr; // per list position: the item's code bits
   // The flat list's whole records also have their graph bits in columns of
-  // words: word j of position 64q + l at gw[ggoff[q] + 64j + l], the
-  // record's bits from bit 0 of word 0 on, ceil(gbits / 64) + 5 words
-  // (gbits 0: a chunk, read from the stream).
+  // words: word j of position 64q + l at gw[ggoff[q] + 64j + l]; word 0 is
+  // the record's start word (flat_start_word), then the record's bits from
+  // bit 0 of word 1 on, ceil(gbits / 64) + 5 words (gbits 0: a chunk, read
+  // from the stream).
   const uint64_t* gw = nullptr;
   const uint64_t* ggoff = nullptr;  // per group: first word
   const uint32_t* gbits = nullptr;  // per list position
@@ -231,15 +232,17 @@ __global__ void __launch_bounds__(kBlock) k_gcol_len(Streams S, Items I, uint64_
   }
 }
 
-// Copies every whole record's graph bits into its column (TsCols::gw):
-// ceil(gbits / 64) + 5 words of the stream from the record's first bit on
-// (words past the stream's padding read as 0).
+// Copies every whole record's graph bits into its column (TsCols::gw): the
+// record's start word (flat_start_word), then ceil(gbits / 64) + 5 words of
+// the stream from the record's first bit on (words past the stream's
+// padding read as 0).
 __global__ void __launch_bounds__(kBlock) k_gcol_fill(Streams S, Items I, const uint64_t* gstart, uint64_t* w) {
   FOR_ITEMS(I, i) {
     const uint32_t nb = I.tc.gbits[i];
     if (nb == 0) continue;
     const uint64_t b = gstart[i];
     uint64_t* out = w + I.tc.ggoff[i >> 6] + (i & 63);
+    out[0] = flat_start_word(S, b);
     const uint32_t words = (nb + 63) / 64 + 5;
     for (uint32_t j = 0; j < words; ++j) {
       const uint64_t p = b + 64ull * j;
@@ -247,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_gcol_fill(Streams S, Items I, const 
       const uint32_t sh = (uint32_t)p & 63u;
       const uint64_t a = wi <= S.g_last ? S.g[wi] : 0;
       const uint64_t c = wi + 1 <= S.g_last ? S.g[wi + 1] : 0;
-      out[64ull * j] = sh ? (a << sh) | (c >> (64u - sh)) : a;
+      out[64ull * (j + 1)] = sh ? (a << sh) | (c >> (64u - sh)) : a;
     }
   }
 }
@@ -3386,7 +3389,7 @@ static int tcol_build(tga_graph* g) {
   for (size_t q = 0; q < ng; ++q) {
     uint32_t rows = 0;
     for (size_t i = 64 * q; i < std::min<size_t>(ni, 64 * q + 64); ++i)
-      if (hb[i]) rows = std::max(rows, (hb[i] + 63) / 64 + 5);
+      if (hb[i]) rows = std::max(rows, (hb[i] + 63) / 64 + 6);
     goff[q] = at;
     at += 64ull * rows;
   }

@@ -209,6 +209,7 @@ int main(int argc, char** argv) {
   // an item's codes hold all-ones words, so a walk that read them would
   // decode wrong times.
   std::vector<uint64_t> cw, goff, gcw, ggoff;
+  int64_t start_words = 0, start_bad = 0;
   std::vector<uint32_t> nbits(list.size()), gbits(list.size(), 0);
   if (col) {
     // Graph columns of the whole records without references (k_gcol_len,
@@ -226,7 +227,7 @@ int main(int argc, char** argv) {
     for (size_t q = 0; q < gng; ++q) {
       uint32_t rows = 0;
       for (size_t i = 64 * q; i < std::min(list.size(), 64 * q + 64); ++i)
-        if (gbits[i]) rows = std::max(rows, (gbits[i] + 63) / 64 + 5);
+        if (gbits[i]) rows = std::max(rows, (gbits[i] + 63) / 64 + 6);
       ggoff[q] = gat;
       gat += 64ull * rows;
     }
@@ -235,11 +236,36 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < list.size(); ++i) {
       if (!gbits[i]) continue;
       const uint64_t b = vpos[2 * (size_t)list[i]];
+      const size_t at = (size_t)(ggoff[i / 64] + (i % 64));
+      gcw[at] = flat_start_word(S, b);
+      if (gcw[at] != kNoStart) ++start_words;
       for (uint64_t j = 0; j < (gbits[i] + 63) / 64 + 5; ++j) {
         const uint64_t p = b + 64 * j, wi = p >> 6;
         const uint32_t sh = (uint32_t)p & 63u;
         const uint64_t a = wi <= S.g_last ? S.g[wi] : 0, c = wi + 1 <= S.g_last ? S.g[wi + 1] : 0;
-        gcw[(size_t)(ggoff[i / 64] + 64 * j + (i % 64))] = sh ? (a << sh) | (c >> (64u - sh)) : a;
+        gcw[at + 64 * (j + 1)] = sh ? (a << sh) | (c >> (64u - sh)) : a;
+      }
+      // The start word must give the level state dec_init reaches.
+      if (gcw[at] != kNoStart) {
+        Dec<0> A, B;
+        std::memset((void*)&A, 0, sizeof A);
+        std::memset((void*)&B, 0, sizeof B);
+        int32_t depth = 0;
+        const int32_t da = dec_init<0, 0>(S, A, list[i], b, depth);
+        Streams SC = S;
+        SC.g = gcw.data() + at + 64;
+        SC.g_sh = 6;
+        SC.g_last = (gbits[i] + 63) / 64 + 4;
+        SC.g_bits = 64ull * SC.g_last;
+        const int32_t db = dec_init_flat<0>(SC, B, list[i], gcw[at]);
+        const Level& la = A.lv[0];
+        const Level& lb = B.lv[0];
+        const bool same = da == db && la.hm == lb.hm && la.mrem == lb.mrem && la.mleft == lb.mleft &&
+                          la.hi == lb.hi && la.irem == lb.irem && la.ileft == lb.ileft &&
+                          (la.ileft == 0 || la.ipos - b == lb.ipos) && la.hr == lb.hr && la.rleft == lb.rleft &&
+                          (la.rleft == 0 || A.rb0.pos() - b == B.rb0.pos()) &&
+                          (la.mleft == 0 || A.mb0.pos() - b == B.mb0.pos());
+        if (!same) ++start_bad;
       }
     }
     std::vector<uint64_t> tstart(list.size());
@@ -296,6 +322,8 @@ int main(int argc, char** argv) {
     else run_item<kMaxDepth>(S, ck.data(), list[i], a, ring, st);
   }
   if (bad) return die("walk order / split-run links broken in items:", bad);
+  if (start_bad) return die("start words that do not give dec_init's state:", start_bad);
+  if (col && start_words == 0) return die("no record got a start word");
 
   FILE* f = std::fopen(argv[3], "wb");
   if (!f) return die("cannot write the output");
