@@ -61,19 +61,14 @@ inline int32_t flat_stride_for(int64_t contacts) {
 
 // Neighbor-access checksum term of one contact (x, r, t): the checksum of a
 // pass is the sum of the terms of its contacts modulo 2^64, so it does not
-// depend on the order contacts are visited in.  The term is a 32-bit mix
-// (one multiply per contact beyond the per-record x * K: the decode, not the
-// checksum, is what the pass measures):
-//   h = t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77;  h ^= h >> 16;
-//   h *= 0x7FEB352D;  h ^= h >> 15          (all uint32)
+// depend on the order contacts are visited in.  The term is one multiply of
+// the time and a key of the pair (r * K1 once per run, x * K2 once per
+// record: the decode, not the checksum, is what the pass measures):
+//   h = (t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77) * 0x7FEB352D   (all uint32)
 TGA_HD uint32_t contact_key_x(int32_t x) { return (uint32_t)x * 0x85EBCA77u; }
 TGA_HD uint32_t contact_key_r(uint32_t kx, int32_t r) { return ((uint32_t)r * 0x9E3779B1u) ^ kx; }
 TGA_HD uint64_t contact_mix(uint32_t kr, int32_t t) {
-  uint32_t h = (uint32_t)t ^ kr;
-  h ^= h >> 16;
-  h *= 0x7FEB352Du;
-  h ^= h >> 15;
-  return h;
+  return ((uint32_t)t ^ kr) * 0x7FEB352Du;
 }
 TGA_HD uint64_t contact_term(uint32_t kx, int32_t r, int32_t t) { return contact_mix(contact_key_r(kx, r), t); }
 TGA_HD uint64_t contact_hash(int32_t x, int32_t r, int32_t t) { return contact_term(contact_key_x(x), r, t); }

@@ -752,9 +752,8 @@ extern "C" int tgaf_gen_netflow(int64_t n, double exponent, double count_min, do
 }
 
 // The neighbor-access checksum of a raw contact list: the wrapping 64-bit sum
-// over its contacts of h = t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77, h ^= h >> 16,
-// h *= 0x7FEB352D, h ^= h >> 15 (uint32 arithmetic; the sum the decoding pass
-// returns).
+// over its contacts of h = (t ^ r * 0x9E3779B1 ^ x * 0x85EBCA77) * 0x7FEB352D
+// (uint32 arithmetic; the sum the decoding pass returns).
 extern "C" int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m,
                                      uint64_t* out) {
   if (m < 0 || (m > 0 && (!src || !dst || !t)) || !out) return fail(TGAF_EINVAL, "tgaf_contact_checksum: bad arguments");
@@ -762,11 +761,7 @@ extern "C" int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, con
   std::vector<uint64_t> part(kChunks, 0);
   std::atomic<int64_t> next{0};
   auto term = [](uint32_t x, uint32_t r, uint32_t t) -> uint64_t {
-    uint32_t h = t ^ (r * 0x9E3779B1u) ^ (x * 0x85EBCA77u);
-    h ^= h >> 16;
-    h *= 0x7FEB352Du;
-    h ^= h >> 15;
-    return h;
+    return (uint64_t)((t ^ (r * 0x9E3779B1u) ^ (x * 0x85EBCA77u)) * 0x7FEB352Du);
   };
   auto work = [&]() {
     for (;;) {

@@ -103,8 +103,8 @@ int tga_info(const tga_graph* g, tga_graph_info* info);
 /* Neighbor access (RA): decodes every contact of the listed vertices (all
  * active vertices when nverts == 0) and returns their count and an
  * order-independent checksum: the wrapping 64-bit sum over contacts (x, r, t)
- * of h = t ^ r*0x9E3779B1 ^ x*0x85EBCA77; h ^= h >> 16; h *= 0x7FEB352D;
- * h ^= h >> 15 (uint32 arithmetic, x, r, t as uint32).
+ * of h = (t ^ r*0x9E3779B1 ^ x*0x85EBCA77) * 0x7FEB352D (uint32 arithmetic,
+ * x, r, t as uint32).
  * Replaces travelGraph/travel (neighbor_access.cu:1438-1642), which decodes and
  * discards. */
 int tga_neighbor_access(tga_graph* g, const int32_t* verts, int64_t nverts, uint64_t* checksum, int64_t* contacts);

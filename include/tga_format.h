@@ -96,10 +96,9 @@ int tgaf_gen_chung_lu(int64_t n, double exponent, int64_t t_span, int64_t period
 int tgaf_gen_netflow(int64_t n, double exponent, double count_min, double gap_min, int64_t t_span, uint64_t seed,
                      int64_t m, int32_t* src, int32_t* dst, int64_t* t);
 
-/* The wrapping 64-bit sum over contacts of h = t ^ r*0x9E3779B1 ^
- * x*0x85EBCA77; h ^= h >> 16; h *= 0x7FEB352D; h ^= h >> 15 (uint32
- * arithmetic, x, r, t as unsigned 32-bit values): the checksum
- * tga_neighbor_access returns for the encoded list. */
+/* The wrapping 64-bit sum over contacts of h = (t ^ r*0x9E3779B1 ^
+ * x*0x85EBCA77) * 0x7FEB352D (uint32 arithmetic, x, r, t as unsigned 32-bit
+ * values): the checksum tga_neighbor_access returns for the encoded list. */
 int tgaf_contact_checksum(const int32_t* src, const int32_t* dst, const int64_t* t, int64_t m, uint64_t* out);
 
 /* Raw 4-column text ("src dst 1 t" per line), gzip-compressed. */

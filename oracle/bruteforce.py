@@ -16,16 +16,14 @@ LD_UNREACHED = -(((1 << 31) - 1) // 2)  # -1073741823
 
 
 def contact_hashes(src, dst, t) -> np.ndarray:
-    """Per-contact checksum terms: h = t ^ r*0x9E3779B1 ^ x*0x85EBCA77;
-    h ^= h >> 16; h *= 0x7FEB352D; h ^= h >> 15 (uint32 arithmetic)."""
+    """Per-contact checksum terms: h = (t ^ r*0x9E3779B1 ^ x*0x85EBCA77) *
+    0x7FEB352D (uint32 arithmetic)."""
     x = np.asarray(src).astype(np.uint32)
     r = np.asarray(dst).astype(np.uint32)
     tt = np.asarray(t).astype(np.int64).astype(np.uint32)
     with np.errstate(over="ignore"):
         h = tt ^ (r * np.uint32(0x9E3779B1)) ^ (x * np.uint32(0x85EBCA77))
-        h ^= h >> np.uint32(16)
         h *= np.uint32(0x7FEB352D)
-        h ^= h >> np.uint32(15)
     return h.astype(np.uint64)
 
 

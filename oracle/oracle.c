@@ -336,14 +336,11 @@ static int act(const orc* g, const orc_q* q, int64_t v) {
 
 /* Contact checksum: wrapping 64-bit sum over contacts of a 32-bit mix of the
  * triple (the reference discards decoded values; this is the build's own RA
- * output): h = t ^ r*0x9E3779B1 ^ x*0x85EBCA77; h ^= h >> 16;
- * h *= 0x7FEB352D; h ^= h >> 15 (uint32 arithmetic). */
+ * output): h = (t ^ r*0x9E3779B1 ^ x*0x85EBCA77) * 0x7FEB352D (uint32
+ * arithmetic). */
 uint64_t orc_contact_hash(int32_t x, int32_t r, int32_t t) {
-  uint32_t h = (uint32_t)t ^ ((uint32_t)r * 0x9E3779B1u) ^ ((uint32_t)x * 0x85EBCA77u);
-  h ^= h >> 16;
-  h *= 0x7FEB352Du;
-  h ^= h >> 15;
-  return h;
+  const uint32_t h = (uint32_t)t ^ ((uint32_t)r * 0x9E3779B1u) ^ ((uint32_t)x * 0x85EBCA77u);
+  return (uint32_t)(h * 0x7FEB352Du);
 }
 
 /* Neighbor access (neighbor_access.cu:1438-1595; BaselineRunner.java:273-297):
