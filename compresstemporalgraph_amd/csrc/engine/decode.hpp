@@ -919,7 +919,8 @@ TGA_HD int32_t dec_next_run(const Streams& S, Dec<MAXD>& D, int32_t& cnt) {
 // Layout: kStageDw stream dwords per lane in stream order at slot
 // (dword index mod kStageDw), slots kStageDw and kStageDw + 1 mirroring slots
 // 0 and 1 so a 96-bit read never wraps; slot i of a lane at s[i * stride]
-// (lanes interleaved: every lane of a wave hits its own bank).
+// (lanes interleaved: every lane of a wave hits its own bank; stride a power
+// of two, at least 8).
 //
 // The units come either from the linear stream (uw = the stream words,
 // ustride 1, positions absolute) or from a column copy of the item's codes
@@ -929,6 +930,7 @@ TGA_HD int32_t dec_next_run(const Streams& S, Dec<MAXD>& D, int32_t& cnt) {
 // units interleaved, so the wave's top-up loads fall into a few adjacent
 // lines instead of 64 separate streams.
 constexpr int32_t kStageDw = 16;  // staged dwords per lane (power of two, multiple of 4)
+constexpr uint32_t kNoEnd = ~0u;
 struct TsStage {
   uint32_t* s;
   int32_t stride;
@@ -937,6 +939,7 @@ struct TsStage {
   int32_t wtop;    // 32 * wend - 64: a code at pos >= wtop may reach past the staged dwords
   int32_t prev;    // the timestamp before the next code
   uint32_t uleft;  // units holding codes of the item not yet staged
+  uint32_t pend;   // pos one past the item's last code (kNoEnd: not known), checked when the walk ends
   const uint64_t* uw = nullptr;  // next unit to stage (word pairs); nullptr: the linear stream (stage_init)
   uint32_t ustride = 1;          // unit pitch of the source, in units
   uint32_t nbits = 0;            // column source: the item's code bits
@@ -976,13 +979,16 @@ TGA_HD void stage_unit(TsStage& st) {
 
 // Starts the window at bit `pos` for codes ending before bit `end` (the
 // first unit holds the code at pos).
-TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end, int32_t prev) {
+// `exact`: the item's codes end at `end` (a whole record; a chunk's codes end
+// at the next chunk's cursor, before `end`).
+TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end, int32_t prev, bool exact) {
   if (pos >= S.t_bits) raise_err(S, kErrOverrun);
   const uint64_t base = (pos >> 5) & ~(uint64_t)3;  // stream dword of the first unit
   const uint64_t lim = (end + 31) >> 5;             // one past the item's last dword
   st.uw = S.t + (base >> 1);
   st.ustride = 1;
   st.uleft = lim > base ? (uint32_t)((lim - base + 3) >> 2) : 0u;
+  st.pend = exact && end - base * 32 < kNoEnd ? (uint32_t)(end - base * 32) : kNoEnd;
   st.pos = (uint32_t)(pos - base * 32);
   st.wend = 0;
   st.wtop = -64;
@@ -993,6 +999,7 @@ TGA_HD void stage_init(const Streams& S, TsStage& st, uint64_t pos, uint64_t end
 // nbits set by the caller).
 TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
   st.uleft = (st.nbits + 127) >> 7;
+  st.pend = st.nbits;
   st.pos = 0;
   st.wend = 0;
   st.wtop = -64;
@@ -1002,15 +1009,10 @@ TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
 // Tops the window up: whole units while they fit behind the current dword
 // and hold codes of the item.  A 64-bit peek near the item's end may then see
 // stale dwords after its code, which no valid code depends on.  Positions
-// move down in steps of kStageDw dwords (slot numbers stay put) before they
-// could overflow.
+// count from the item's first unit: an item holds at most a checkpoint
+// stride of contacts (at most 4096, of at most 96 bits even in a corrupt
+// stream), far below the 2^31 bits the 32-bit positions could hold.
 TGA_HD void stage_fill(const Streams&, TsStage& st) {
-  if (st.pos >= (1u << 30)) {
-    const uint32_t m = (st.pos >> 5) & ~(uint32_t)(kStageDw - 1);
-    st.pos -= m * 32;
-    st.wend -= m;
-    st.wtop -= (int32_t)(m * 32);
-  }
   const uint32_t d = st.pos >> 5;
   while (st.wend + 4 <= d + kStageDw && st.uleft) stage_unit(st);
 }
@@ -1031,8 +1033,9 @@ constexpr uint32_t kZeta2MaxPrefix = 16;
 
 // Decodes the zeta_2 codeword at the start of the 64-bit window (hi, lo) and
 // returns the int2nat inverse of its value modulo 2^32; `len` receives the
-// codeword length and `hmax` the largest unary prefix seen (a prefix above
-// kZeta2MaxPrefix marks an invalid stream; the caller checks it once).
+// codeword length.  A valid code has a prefix of at most kZeta2MaxPrefix; a
+// longer one (a corrupt stream) still decodes to some value and length, and
+// the walk detects it by the item's codes not ending where the item ends.
 //
 // With h the prefix zeros, the code is h zeros, a one, the 2h + 1 bits m of
 // the short form and, when m >= 4^h (the bit s after the one is set), one
@@ -1043,7 +1046,7 @@ constexpr uint32_t kZeta2MaxPrefix = 16;
 // The value needs 33 bits (two int32 timestamps can differ by up to
 // 2^32 - 1), and its int2nat inverse is taken modulo 2^32, as the
 // timestamps are.
-TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hmax) {
+TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len) {
   const uint32_t h = (uint32_t)__builtin_clz(hi | 1u);  // hi == 0: h = 31, an invalid prefix
   const uint32_t s = ((hi << h) >> 30) & 1u;
   const uint32_t tl = 3 * h + s;
@@ -1051,7 +1054,6 @@ TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hm
   const uint64_t p = ((uint64_t)hi << 32) | lo;
   const uint64_t b = p >> ((62u - tl) & 63u);
   const uint64_t v = b - (1ull << ((2 * (h + s)) & 63u)) - 1u;
-  hmax = hmax > h ? hmax : h;
   return (int32_t)((uint32_t)(v >> 1) ^ (0u - ((uint32_t)v & 1u)));
 }
 
@@ -1064,15 +1066,20 @@ TGA_HD int32_t zeta2_delta(uint32_t hi, uint32_t lo, uint32_t& len, uint32_t& hm
 // dword (one before the current one, possibly already replaced) drops out;
 // otherwise t/32 is pos's own dword.  At pos = 0, t/32 maps to slot 15 and
 // the next two to the mirror slots of 0 and 1.
-TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st, uint32_t& hmax) {
+TGA_HD int32_t stage_ts_next(const Streams& S, TsStage& st) {
   // (pos >> 5) + 3 > wend: the 96 bits read may pass the staged dwords
   if ((int32_t)st.pos >= st.wtop) stage_fill(S, st);  // codes longer than the burst's share
   const uint32_t t = st.pos - 1u;
-  const uint32_t* p = st.s + (size_t)((t >> 5) & (kStageDw - 1)) * st.stride;
+  // Slot (t >> 5) mod kStageDw at byte offset slot << lg (lg = log2 of the
+  // slot pitch in bytes, >= 5): computed as (t << (lg - 5)) masked to the
+  // slot bits, whose low bits (t mod 32) fall below them.
+  const uint32_t lg = (uint32_t)__builtin_ctz((uint32_t)st.stride) + 2u;
+  const uint32_t off = (t << (lg - 5u)) & ((uint32_t)(kStageDw - 1) << lg);
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(st.s) + off);
   const uint32_t w0 = p[0], w1 = p[st.stride], w2 = p[2 * st.stride];
   const uint32_t hi = funnel32(w0, w1, ~t), lo = funnel32(w1, w2, ~t);
   uint32_t len;
-  const int32_t d = zeta2_delta(hi, lo, len, hmax);
+  const int32_t d = zeta2_delta(hi, lo, len);
   st.pos += len;
   st.prev = (int32_t)((uint32_t)st.prev + (uint32_t)d);
   return st.prev;
@@ -1115,7 +1122,7 @@ TGA_HD int32_t act_ring_val(Act&, int32_t v, long) {
 // run.  On graphs whose pairs repeat (runs of tens of contacts) the refill
 // takes one or two passes per burst.  The consume phase takes the contacts
 // the ring holds, at most a burst, so it never waits on the ring; the
-// timestamp codes' validity is checked once per walk.
+// timestamp codes' validity is checked once per walk, by where they end.
 template <int MAXD, class Act>
 TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t k, int32_t kend, Act& act,
                  RunRing ring) {
@@ -1126,7 +1133,7 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
   int32_t avail = 0;            // contacts held: the current run's and the ring's
   int32_t r = 0, c = 0;         // current run: neighbor (or its ring_val), contacts left in it
   bool more = true;             // the record has runs left
-  uint32_t hmax = 0;            // largest zeta_2 prefix decoded
+  bool full = true;             // every contact of [k, kend) walked
   while (k < kend) {
     while (more && avail < S.burst && tail - head < (uint32_t)kRing * rstep) {
       int32_t cnt;
@@ -1154,16 +1161,19 @@ TGA_HD void walk(const Streams& S, Dec<MAXD>& D, TsStage& ts, int32_t x, int32_t
         c = e.y;
       }
       --c;
-      const int32_t t = stage_ts_next(S, ts, hmax);
+      const int32_t t = stage_ts_next(S, ts);
       if (!act.contact(x, r, t, k + j)) {
-        kend = k;  // stop after the validity check
+        kend = k;  // stop
+        full = false;
         break;
       }
     }
     avail -= n;
     k += n;
   }
-  if (hmax > kZeta2MaxPrefix) raise_err(S, kErrCode);
+  // Valid codes of a whole item end exactly at its end (a stream's last
+  // record may carry up to 7 bits of byte padding after its codes).
+  if (full && ts.pend != kNoEnd && (ts.pos > ts.pend || ts.pend - ts.pos > 7)) raise_err(S, kErrCode);
 }
 
 // Actions that keep per-item state define item(code); it runs first.
@@ -1214,7 +1224,7 @@ TGA_HD void run_item(const Streams& S0, const Ckpt* ck, int32_t code, Act& act, 
     if (ts.uw)
       stage_init_col(ts, S.min_ts);
     else
-      stage_init(S, ts, tpos, rec_tend(S, x), S.min_ts);
+      stage_init(S, ts, tpos, rec_tend(S, x), S.min_ts, true);
   } else {
     const Ckpt* c = ck + (-code - 1);
     x = c->x;
@@ -1225,7 +1235,7 @@ TGA_HD void run_item(const Streams& S0, const Ckpt* ck, int32_t code, Act& act, 
     if (ts.uw)
       stage_init_col(ts, c->tprev);
     else
-      stage_init(S, ts, c->tpos, rec_tend(S, x), c->tprev);
+      stage_init(S, ts, c->tpos, rec_tend(S, x), c->tprev, false);
   }
   walk<MAXD>(S, D, ts, x, k, kend, act, ring);
   act.end(x, chunked);

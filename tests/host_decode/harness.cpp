@@ -185,9 +185,9 @@ int main(int argc, char** argv) {
   std::vector<uint64_t> sum((size_t)n, 0);
   int64_t bad = 0;
   std::vector<int2> ring_store(kRing);
-  std::vector<uint32_t> stage_store(kStageDw + 2);
+  std::vector<uint32_t> stage_store((kStageDw + 2) * 8);  // slot pitch 8 (the walk's minimum)
   const RunRing ring{ring_store.data(), 1};
-  const TsStage stage{stage_store.data(), 1, 0, 0, 0, 0, 0};
+  const TsStage stage{stage_store.data(), 8, 0, 0, 0, 0, 0};
   std::vector<int32_t> list, lgroup;  // items in vertex order (chunks in order) and their launch group
   for (int64_t x = 0; x < n; ++x) {
     if (deg[(size_t)x] == 0) continue;

@@ -105,3 +105,30 @@ def test_host_decode_columns(harness, which, stride, burst, request, tmp_path):
     want_cnt, want_sums = _raw_sums(ds)
     np.testing.assert_array_equal(cnt, want_cnt)
     np.testing.assert_array_equal(sums, want_sums)
+
+
+@pytest.mark.parametrize("col", [False, True])
+def test_host_decode_flags_corrupt_timestamps(harness, ds_nf, col, tmp_path):
+    """Zeroing a few 16-byte stretches of the .timestamps stream (codes with
+    an over-long prefix, and every later code of the record misframed) must
+    raise the code error (bit 16 of the error word): whole records end their
+    codes exactly where the next record starts.  The flat stride is raised
+    above every degree, so no record is chunked and the walk, not the
+    checkpoint build, meets the corrupt codes."""
+    import shutil
+    d = tmp_path / "corrupt"
+    shutil.copytree(ds_nf.dir, d)
+    p = d / (ds_nf.name + ".timestamps")
+    b = bytearray(p.read_bytes())
+    for k in range(1, 9):
+        at = len(b) * k // 9
+        b[at:at + 16] = bytes(16)
+    p.write_bytes(bytes(b))
+
+    class Corrupt:
+        pass
+    c = Corrupt()
+    c.dir, c.name, c.n = str(d), ds_nf.name, ds_nf.n
+    assert np.bincount(ds_nf.src).max() < 1 << 14
+    cnt, sums, err = _run(harness, c, tmp_path, 1 << 14, None, col=col)
+    assert err & 16
