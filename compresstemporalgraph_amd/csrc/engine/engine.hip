@@ -1623,6 +1623,12 @@ __global__ void __launch_bounds__(64) k_dc_long(Streams S, const int32_t* bigv, 
   centr[x] = W.active(x) ? dc_formula(a.deg, a.avg, W.ws, W.we) : dc_formula(0, 0, W.ws, W.we);
 }
 
+// Adds the wave's counts into *out (one atomic per wave).
+__device__ __forceinline__ void add_wave_count(uint32_t c, uint32_t* out) {
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
 // BFS level step: every unreached active neighbor over a window contact gets
 // level lvl + 1 (all writers of a pass store the same value, so plain stores
 // suffice); the next level is selected by its level value.
@@ -1647,22 +1653,35 @@ __global__ void __launch_bounds__(kBlock) k_bfs(Streams S, const Ckpt* ck, Items
 }
 
 // Min-label propagation: a vertex whose label drops in round r is stamped r
-// and expanded in round r + 1.
+// and expanded in round r + 1.  A round runs either over a queue of the
+// stamped vertices' items (k_expand) or, when many are stamped, over the
+// static lists with `sel` set: an item then runs when its vertex's stamp is
+// r - 1 as the item begins.  A vertex stamped again during the round before
+// its item begins waits for the next round; labels only fall and every value
+// is the label of a vertex reaching the one it is stored at, so the fixpoint
+// (the least active id reaching each vertex) is the same.  The kernel counts
+// the vertices it stamps (nx; a race may count one twice), which the host
+// uses to pick the next round's form and to stop.
 struct CcAct {
   Window W;
   int32_t* label;
   int32_t* stamp;
   int32_t round;
+  const int32_t* sel;  // stamp, in a round over the static lists
+  uint32_t nx = 0;     // vertices stamped
   int32_t lx = 0;
   __device__ bool begin(int32_t x, bool, Split) {
-    if (!W.active(x)) return false;
+    if (!W.active(x) || (sel && sel[x] != round - 1)) return false;
     lx = label[x];
     return true;
   }
   __device__ bool contact(int32_t, int32_t r, int32_t t, int32_t) {
     if (W.in(t) && W.active(r) && lx < label[r]) {
       const int32_t old = atomicMin(&label[r], lx);
-      if (lx < old) stamp[r] = round;
+      if (lx < old && stamp[r] != round) {
+        stamp[r] = round;
+        ++nx;
+      }
     }
     return true;
   }
@@ -1671,10 +1690,12 @@ struct CcAct {
 
 template <int MAXD>
 __global__ void __launch_bounds__(kBlock) k_cc(Streams S, const Ckpt* ck, Items I, Window W, int32_t* label,
-                                               int32_t* stamp, int32_t round) {
-  CcAct a{W, label, stamp, round};
+                                               int32_t* stamp, int32_t round, const int32_t* sel,
+                                               uint32_t* next_count) {
+  CcAct a{W, label, stamp, round, sel};
   LANE_RING();
   FOR_ITEMS(I, i) run_item<MAXD>(S, ck, I.code[i], a, ring_, I.stage(stage_, i));
+  add_wave_count(a.nx, next_count);
 }
 
 // Graph coloring round: a neighbor blocks when it has higher priority and is
@@ -2744,6 +2765,7 @@ struct tga_graph {
   std::vector<int64_t> gcb_level_off;      // level boundaries in gcb_segs
   bool gcb_default_levels = false;         // gcb_segs holds the default priorities' levels
   bool use_graphs = true;  // TGA_GRAPHS=0: launch the loops one kernel at a time
+  int32_t dense_div = 5;   // TGA_DENSE_DIV (dense)
   std::vector<LaneServer*> servers;  // open query servers (orphaned when the handle closes)
   std::vector<int32_t> h_items;  // items each vertex expands to (0 empty, 1 whole record, else its chunks)
   // kernel timing (tga_profile)
@@ -2895,6 +2917,21 @@ struct tga_graph {
     HIP_TRY(hipStreamSynchronize(stream));
     return TGA_OK;
   }
+  // Whether a round of connected components whose frontier holds `nf`
+  // vertices runs over the static lists (which read the column copies and
+  // whose kernels skip the items of vertices outside the frontier) instead
+  // of a queue from k_expand: when nf is at least 1/TGA_DENSE_DIV of the
+  // active vertices (default 5, the best of 2-30 on the 2^30-contact
+  // netflow-like graph; 0: never).  A pass over the static lists
+  // costs about one full decode pass however many of its items run; a queue
+  // costs a scan of every vertex plus its items' walks from the streams.
+  bool dense(uint64_t nf) const { return dense_div > 0 && tcols[0].w && nf * (uint64_t)dense_div >= (uint64_t)(ae - ab); }
+  // Zeroes the next-frontier counter before a round's kernels.
+  uint32_t* next_count() {
+    k_fill<uint32_t><<<1, 64, 0, stream>>>(cnt + 6, 1, 0u);
+    return cnt + 6;
+  }
+  int read_next_count(uint32_t* nf) { return read_counts(cnt + 6, nf, 1); }
   // Expands the selected vertices of a list (front) or of the range
   // [base, base + count) into the group queues (and xq_big); hc[g] receives
   // the count of group g and hc[kGroups] the number of chunked records.
@@ -3486,6 +3523,7 @@ static int open_impl(const std::string& dir, const std::string& name, const tga_
   if (hipSetDevice(device) != hipSuccess) return bail(fail(TGA_EDEVICE, "hipSetDevice failed"));
   g->device = device;
   if (const char* e = std::getenv("TGA_GRAPHS")) g->use_graphs = std::atoi(e) != 0;
+  if (const char* e = std::getenv("TGA_DENSE_DIV")) g->dense_div = std::atoi(e);
   {
     // the allocation-free entry point's device array, found once per device
     NoAllocScratch* sc = nullptr;
@@ -4945,12 +4983,20 @@ extern "C" int tga_connected_components(tga_graph* g, tga_window w, int32_t* lab
   HIP_TRY(hipSetDevice(g->device));
   const Window W = g->window(w);
   k_init_labels<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->i0, g->i1);
-  for (int32_t round = 0;; ++round) {
-    uint32_t hc[kGroups + 1];
-    int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelEq, g->i1, round - 1}, false, false, hc);
+  uint32_t nf = (uint32_t)(g->ae - g->ab);  // every active vertex starts stamped -1
+  for (int32_t round = 0; nf > 0; ++round) {
+    uint32_t* nc = g->next_count();
+    if (g->dense(nf)) {
+      LAUNCH_STATIC(k_cc, W, g->i0, g->i1, round, g->i1, nc);
+    } else {
+      uint32_t hc[kGroups + 1];
+      int rc = g->expand(nullptr, g->ab, (uint32_t)(g->ae - g->ab), Sel{kSelEq, g->i1, round - 1}, false, false, hc);
+      if (rc) return rc;
+      if (total_items(hc) == 0) break;
+      LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round, nullptr, nc);
+    }
+    int rc = g->read_next_count(&nf);
     if (rc) return rc;
-    if (total_items(hc) == 0) break;
-    LAUNCH_QUEUED(k_cc, hc, W, g->i0, g->i1, round);
   }
   int rc = g->out_copy(label, g->i0, (size_t)g->n * 4, flags);
   if (rc) return rc;

@@ -75,3 +75,30 @@ def test_column_copies_equal_linear(which, level, request, monkeypatch):
     finally:
         col.close()
         lin.close()
+
+
+@pytest.mark.parametrize("div", ["0", "1000000"])
+@pytest.mark.parametrize("which", ["ds_adv", "ds_nf", "ds_cl"])
+def test_dense_cc_rounds_match_oracle(which, div, request, monkeypatch):
+    """A connected-components round runs over the static lists (the kernel
+    selects the stamped vertices itself) when at least 1/TGA_DENSE_DIV of
+    the active vertices were stamped, else over a queue from k_expand.  With
+    every round one way (1000000: over the static lists whenever a vertex is
+    stamped) or the other (0: never), the labels must equal the oracle's,
+    over the full window, a window and an active sub-range."""
+    ds = request.getfixturevalue(which)
+    print(f"\n[gpu] CC rounds with TGA_DENSE_DIV={div} on {ds.name}", flush=True)
+    a, b = ds.n // 5, ds.n - ds.n // 7
+    monkeypatch.setenv("TGA_DENSE_DIV", div)
+    g = TemporalGraph(ds.dir, ds.name)
+    ga = TemporalGraph(ds.dir, ds.name, active=(a, b))
+    monkeypatch.delenv("TGA_DENSE_DIV")
+    o = ds.oracle()
+    try:
+        lo, hi = int(ds.t.min()), int(ds.t.max())
+        for win in (None, (lo + (hi - lo) // 4, lo + 3 * (hi - lo) // 4)):
+            np.testing.assert_array_equal(g.connected_components(win), o.cc(window=win))
+            np.testing.assert_array_equal(ga.connected_components(win), o.cc(window=win, active=(a, b)))
+    finally:
+        g.close()
+        ga.close()
