@@ -2594,6 +2594,25 @@ __global__ void __launch_bounds__(kBlock) k_pr_coef(Window W, int32_t n, const f
     coef[v] = (W.active((int32_t)v) && du[v] > 0) ? pr_coef(damping, rank, du, (int32_t)v) : 0.0f;
 }
 
+// End of a PageRank iteration over the pair lists, fused: rank = acc / sum
+// (as k_scale_store), the next iteration's coefficients from it (as
+// k_pr_coef) and acc reset to base (as k_fill_active): one pass over the
+// vertices instead of three, with the same arithmetic.
+__global__ void __launch_bounds__(kBlock) k_pr_next(Window W, int32_t n, double* acc, const double* scale,
+                                                    float* rank, const int32_t* du, float damping, float* coef,
+                                                    double base) {
+  const double s = scale[0];
+  for (int64_t v = blockIdx.x * (int64_t)kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    const double a = acc[v];
+    const float r = (float)(s > 0 ? a / s : a);
+    rank[v] = r;
+    const bool act = W.active((int32_t)v);
+    const int32_t d = du[v];
+    coef[v] = (act && d > 0) ? damping * r / (float)d : 0.0f;
+    acc[v] = act ? base : 0.0;
+  }
+}
+
 // acc[k] = base + sums[i] (or acc[k] += sums[i] when add) for every row run
 // i with key k < n.  Keys must be distinct among the runs of one call.
 __global__ void __launch_bounds__(kBlock) k_scatter_rows(const uint32_t* ukeys, const double* sums,
@@ -5638,11 +5657,15 @@ extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float dam
   const int lp = iters > 0 ? build_pairs(g, W, &extra, &npairs) : 1;
   if (lp < 0) return lp;
   int32_t it = 0;
+  if (lp == 0 && iters > 0) {
+    // the first iteration's coefficients and accumulators; later ones come
+    // from k_pr_next
+    k_pr_coef<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, du, damping, g->f1);
+    k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
+  }
   for (; it < iters; ++it) {
     if (lp == 0) {
       // row sums over the sorted pairs (f1 holds the coefficients)
-      k_pr_coef<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->f0, du, damping, g->f1);
-      k_fill_active<double><<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, (double)base, 0.0);
       int rc = pull_rows(g, 0, g->f1, 0, npairs, (double)base, false);
       if (rc) return rc;
     } else {
@@ -5655,7 +5678,11 @@ extern "C" int tga_pagerank(tga_graph* g, tga_window w, int32_t iters, float dam
     int rc = reduce(g, 0, g->acc, nullptr, nullptr, g->scal, false);
     if (!rc && tol > 0) rc = reduce(g, 2, g->acc, g->f0, g->scal, g->scal + 1, false);
     if (rc) return rc;
-    k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f0);
+    if (lp == 0)
+      k_pr_next<<<g->grid(g->n), kBlock, 0, g->stream>>>(W, (int32_t)g->n, g->acc, g->scal, g->f0, du, damping, g->f1,
+                                                         (double)base);
+    else
+      k_scale_store<<<g->grid(g->n), kBlock, 0, g->stream>>>(g->acc, g->scal, g->n, g->f0);
     HIP_TRY(hipGetLastError());
     if (tol > 0) {
       double diff = 0;
