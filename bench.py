@@ -197,6 +197,32 @@ def large_legs(args) -> int:
     return 0
 
 
+def ts_variants(args) -> int:
+    """Times the two timestamp decode methods of tga_ts_decode_bench over the
+    flat records of --workload (one lane per record from its column copy,
+    and one wave per record decoding speculatively at 64 bit offsets and
+    chaining the codes by shuffles) and prints their JSON; both must decode
+    the same times."""
+    import torch
+    torch.zeros(1, device="cuda:0")
+    from compresstemporalgraph_amd import datasets
+    from compresstemporalgraph_amd.engine import TemporalGraph
+    root = args.data_root or os.path.join(tempfile.gettempdir(), "tga_bench")
+    datasets.ensure(root, args.workload, gpu=True)
+    g = TemporalGraph(datasets.dataset_dir(root, args.workload), args.workload, device=0)
+    out = {"workload": args.workload}
+    sums = set()
+    for method, name in ((0, "lane_per_record_columns"), (1, "wave_per_record_speculative")):
+        ms, sm = g.ts_decode_bench(method, passes=args.steps)
+        out[name] = {"ms": round(ms, 4)}
+        sums.add(sm)
+        log(f"[bench] timestamp decode, {name}: {ms:.3f} ms per pass (sum {sm:#x})")
+    g.close()
+    out["same_times"] = len(sums) == 1
+    print(json.dumps(out), flush=True)
+    return 0 if len(sums) == 1 else 1
+
+
 def encode_leg(args) -> int:
     """Times writing a workload's dataset files with the host encoder alone
     (fmt.encode) and with the split encoder (the graph stream on the host,
@@ -725,6 +751,9 @@ def main() -> int:
     ap.add_argument("--encode-leg", action="store_true",
                     help="only time the host and the split (host graph + GPU streams) encoders on --workload, "
                          "check they write the same bytes, and print their JSON")
+    ap.add_argument("--ts-variants", action="store_true",
+                    help="only time the timestamp decode methods (one lane per record from the column copy, one "
+                         "wave per record decoding speculatively) over --workload's flat records")
     ap.add_argument("--query-legs", action="store_true",
                     help="only run the batched and served BFS legs and print their JSON (run as a child process)")
     ap.add_argument("--traffic-bytes", type=float, default=None,
@@ -755,6 +784,8 @@ def main() -> int:
         return query_legs(args)
     if args.encode_leg:
         return encode_leg(args)
+    if args.ts_variants:
+        return ts_variants(args)
     if args.large_legs:
         return large_legs(args)
 

@@ -948,21 +948,21 @@ struct TsStage {
   uint32_t gbits = 0;            // the record's graph bits
 };
 
-// Stages the next unit (source words uw[0], uw[1]) at relative dword wend
-// with one 16-byte load.  Only units whose first dword lies before the item's
-// end are staged (uleft); in the linear stream the last word of such a unit
-// lies within the stream's five words of padding, and a column holds every
-// unit its item's codes touch, so no bound check is needed.
-TGA_HD void stage_unit(TsStage& st) {
+// One unit's four dwords, each word's high dword first.
+struct Unit {
+  uint32_t d0, d1, d2, d3;
+};
+TGA_HD Unit load_unit(const uint64_t* w) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const uint4 v = *reinterpret_cast<const uint4*>(st.uw);
-  const uint32_t d0 = v.y, d1 = v.x, d2 = v.w, d3 = v.z;  // each word's high dword first
+  const uint4 v = *reinterpret_cast<const uint4*>(w);
+  return Unit{v.y, v.x, v.w, v.z};
 #else
-  const uint64_t w0 = st.uw[0], w1 = st.uw[1];
-  const uint32_t d0 = (uint32_t)(w0 >> 32), d1 = (uint32_t)w0, d2 = (uint32_t)(w1 >> 32), d3 = (uint32_t)w1;
+  return Unit{(uint32_t)(w[0] >> 32), (uint32_t)w[0], (uint32_t)(w[1] >> 32), (uint32_t)w[1]};
 #endif
-  st.uw += 2 * (size_t)st.ustride;
-  --st.uleft;
+}
+// Writes a loaded unit at relative dword wend.
+TGA_HD void put_unit(TsStage& st, Unit u) {
+  const uint32_t d0 = u.d0, d1 = u.d1, d2 = u.d2, d3 = u.d3;
   const uint32_t slot = st.wend & (kStageDw - 1);
   uint32_t* p = st.s + (size_t)slot * st.stride;
   p[0] = d0;
@@ -975,6 +975,17 @@ TGA_HD void stage_unit(TsStage& st) {
   }
   st.wend += 4;
   st.wtop += 128;
+}
+// Stages the next unit (source words uw[0], uw[1]) at relative dword wend
+// with one 16-byte load.  Only units whose first dword lies before the item's
+// end are staged (uleft); in the linear stream the last word of such a unit
+// lies within the stream's five words of padding, and a column holds every
+// unit its item's codes touch, so no bound check is needed.
+TGA_HD void stage_unit(TsStage& st) {
+  const Unit u = load_unit(st.uw);
+  st.uw += 2 * (size_t)st.ustride;
+  --st.uleft;
+  put_unit(st, u);
 }
 
 // Starts the window at bit `pos` for codes ending before bit `end` (the
@@ -1012,9 +1023,23 @@ TGA_HD void stage_init_col(TsStage& st, int32_t prev) {
 // count from the item's first unit: an item holds at most a checkpoint
 // stride of contacts (at most 4096, of at most 96 bits even in a corrupt
 // stream), far below the 2^31 bits the 32-bit positions could hold.
+//
+// Units are loaded two at a time, both loads issued before either is
+// written, so a top-up waits for memory once per pair.
 TGA_HD void stage_fill(const Streams&, TsStage& st) {
   const uint32_t d = st.pos >> 5;
-  while (st.wend + 4 <= d + kStageDw && st.uleft) stage_unit(st);
+  uint32_t k = (d + kStageDw - st.wend) >> 2;  // units that fit behind the current dword
+  k = minu32(k, st.uleft);
+  while (k >= 2) {
+    const Unit a = load_unit(st.uw);
+    const Unit b = load_unit(st.uw + 2 * (size_t)st.ustride);
+    st.uw += 4 * (size_t)st.ustride;
+    st.uleft -= 2;
+    put_unit(st, a);
+    put_unit(st, b);
+    k -= 2;
+  }
+  if (k) stage_unit(st);
 }
 
 // The 32-bit funnel shift ({hi, lo} >> (n mod 32)) mod 2^32: one

@@ -3942,6 +3942,96 @@ extern "C" int tga_autotune(tga_graph* g, int32_t passes, tga_tune_report* repor
   return TGA_OK;
 }
 
+// Timestamp decode methods compared (tga_ts_decode_bench): the codes of the
+// flat list's whole records, summed as uint32 times, decoded
+//  0: one lane per record through the walk's window over the record's
+//     column copy (the method every decode pass uses);
+//  1: one wave per record, speculatively: lane l decodes a code at bit
+//     cursor + l, the true codes are then found by hopping from lane to lane
+//     by code length (a lane index read out per code), and the cursor moves
+//     past the last code that started in the window (the ballot/shuffle
+//     chain of the reference's Yahoo/neighbor_access_supernode_warp.cu:
+//     1697-1725, read from the stream with every lane's loads falling into
+//     the same few words).
+template <int METHOD>
+__global__ void __launch_bounds__(kBlock) k_ts_bench(Streams S, Items I, const int32_t* vdeg,
+                                                     unsigned long long* out) {
+  unsigned long long acc = 0;
+  if constexpr (METHOD == 0) {
+    LANE_RING();
+    (void)ring_;
+    FOR_ITEMS(I, i) {
+      const int32_t x = I.code[i];
+      if (x < 0) continue;
+      TsStage ts = I.stage(stage_, i);
+      if (!ts.uw) continue;
+      stage_init_col(ts, S.min_ts);
+      const int32_t d = vdeg[x];
+      stage_fill(S, ts);
+      for (int32_t j = 0; j < d; ++j) {
+        if ((j & 15) == 0) stage_fill(S, ts);
+        acc += (uint32_t)stage_ts_next(S, ts);
+      }
+    }
+  } else {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < I.count; i += waves) {
+      const int32_t x = I.code[i];
+      if (x < 0) continue;
+      uint64_t cur = S.vpos[2 * (size_t)x + 1];
+      int32_t left = vdeg[x];
+      int32_t prev = S.min_ts;
+      while (left > 0) {
+        const uint64_t w = peek64(S.t, cur + (uint64_t)lane);
+        uint32_t len;
+        const int32_t d = zeta2_delta((uint32_t)(w >> 32), (uint32_t)w, len);
+        uint32_t off = 0;
+        while (off < 64 && left > 0) {
+          const uint32_t l = (uint32_t)__shfl((int32_t)len, (int)off, 64);
+          prev = (int32_t)((uint32_t)prev + (uint32_t)__shfl(d, (int)off, 64));
+          if (lane == 0) acc += (uint32_t)prev;
+          off += l;
+          --left;
+        }
+        cur += off;
+      }
+    }
+  }
+  const unsigned long long t = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0 && t) atomicAdd(out, t);
+}
+
+extern "C" int tga_ts_decode_bench(tga_graph* g, int32_t method, int32_t passes, double* ms, uint64_t* sum) {
+  if (!g || method < 0 || method > 1 || passes < 1 || passes > 1000 || !ms || !sum)
+    return fail(TGA_EINVAL, "tga_ts_decode_bench: bad arguments");
+  if (!g->tcols[0].w) return fail(TGA_EINVAL, "tga_ts_decode_bench: the handle has no column copies (TGA_TCOL=0)");
+  HIP_TRY(hipSetDevice(g->device));
+  const Items I = STATIC_ITEMS(0);
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  const int grid = method == 0 ? g->grid(I.count) : g->grid((int64_t)I.count * 64);
+  for (int32_t p = 0; p <= passes; ++p) {  // pass 0 warms up
+    if (p == 1) HIP_TRY(hipEventRecord(a, g->stream));
+    k_fill<unsigned long long><<<1, 64, 0, g->stream>>>(g->u64s, 1, 0ull);
+    if (method == 0)
+      k_ts_bench<0><<<grid, kBlock, 0, g->stream>>>(g->S, I, g->vdeg, g->u64s);
+    else
+      k_ts_bench<1><<<grid, kBlock, 0, g->stream>>>(g->S, I, g->vdeg, g->u64s);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(b, g->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float t = 0;
+  HIP_TRY(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = (double)t / passes;
+  HIP_TRY(hipMemcpy(sum, g->u64s, sizeof *sum, hipMemcpyDeviceToHost));
+  return g->check_err();
+}
+
 // Scan tiles for m counts (k_scan_tiles).
 static int64_t scan_tiles(int64_t m) { return (m + 1 + kScanTile - 1) / kScanTile; }
 

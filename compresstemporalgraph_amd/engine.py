@@ -115,6 +115,7 @@ def lib():
         L.tga_stream_free.restype = None
         L.tga_debug_fail_alloc.argtypes = [vp, C.c_int64]
         L.tga_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReport)]
+        L.tga_ts_decode_bench.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.tga_neighbor_access_noalloc.argtypes = [vp, p32, C.c_int64, C.POINTER(C.c_uint64), p64]
         L.tga_graph_coloring_batch.argtypes = [vp, C.POINTER(Window), C.c_int32, C.c_int32, vp, vp, C.c_uint32]
         _LIB = L
@@ -225,6 +226,14 @@ class TemporalGraph:
         _check(lib().tga_debug_fail_alloc(self.h, int(nth)))
 
     # -- the seven entry points -------------------------------------------
+    def ts_decode_bench(self, method: int, passes: int = 5):
+        """(ms per pass, sum of the decoded times) of a timestamp decode
+        method over the flat records (tga_ts_decode_bench): 0 one lane per
+        record from its column copy, 1 one wave per record, speculative."""
+        ms, sm = C.c_double(), C.c_uint64()
+        _check(lib().tga_ts_decode_bench(self.h, method, passes, C.byref(ms), C.byref(sm)))
+        return ms.value, sm.value
+
     def neighbor_access(self, verts=None):
         """Returns (checksum, contacts) over the listed vertices (all active when None)."""
         cs, cnt = C.c_uint64(), C.c_int64()

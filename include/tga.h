@@ -303,6 +303,19 @@ typedef struct tga_tune_report {
 } tga_tune_report;
 int tga_autotune(tga_graph* g, int32_t passes, tga_tune_report* report);
 
+/* Timestamp decode methods, timed over the codes of the flat records (whole
+ * records without references) with HIP events on the handle's stream: the
+ * average of `passes` passes after one warm pass in *ms, and the wrapping
+ * sum of the decoded times (as uint32) in *sum, equal for both methods.
+ * method 0: one lane per record, through the walk's LDS window over the
+ * record's column copy (what every decode pass does); method 1: one wave
+ * per record, each lane decoding a code at a speculative bit offset and the
+ * true codes chained by shuffles, as the reference's
+ * gpu_cuda/Yahoo/neighbor_access_supernode_warp.cu:1697-1725 does.  No
+ * counterpart in the reference as an entry point (a measurement of two
+ * decode designs); needs the column copies (not TGA_TCOL=0). */
+int tga_ts_decode_bench(tga_graph* g, int32_t method, int32_t passes, double* ms, uint64_t* sum);
+
 /* Error-path test hook: the nth device allocation the handle makes from now
  * on fails with TGA_ENOMEM, as if hipMalloc had (nth = 0 turns it off).  A
  * call that fails this way leaves no group of first-use buffers half
