@@ -4011,7 +4011,12 @@ extern "C" int tga_ts_decode_bench(tga_graph* g, int32_t method, int32_t passes,
   hipEvent_t a, b;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
-  const int grid = method == 0 ? g->grid(I.count) : g->grid((int64_t)I.count * 64);
+  // Resident grids (eight blocks per compute unit, the items handed out by
+  // grid stride): one short-lived wave per block of items would make the
+  // lane-per-record kernel wait on wave launches rather than decode.
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device);
+  const int grid = std::max(1, std::min(8 * cus, method == 0 ? g->grid(I.count) : g->grid((int64_t)I.count * 64)));
   for (int32_t p = 0; p <= passes; ++p) {  // pass 0 warms up
     if (p == 1) HIP_TRY(hipEventRecord(a, g->stream));
     k_fill<unsigned long long><<<1, 64, 0, g->stream>>>(g->u64s, 1, 0ull);
